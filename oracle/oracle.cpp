@@ -1,0 +1,1187 @@
+// TEST INFRASTRUCTURE ONLY.  This file is the CPU oracle: a from-scratch host
+// C++ restatement of the NEST GPU (v1.0.8) single-GPU simulation step.  It is
+// used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+// leg as the checker and as the reported CPU baseline; the product library
+// never links, loads or calls it.
+//
+// It is templated on the state type: T = float restates the reference float
+// path (including its float/double mixing), T = double is the FP64 host
+// integrator used to state the error of the float path.
+//
+// What each part follows (reference paths relative to the nest-gpu tree):
+//   RK5 step / update loop ...... src/rk5.h:69-207, tableau src/rk5.cu:35-72
+//   aeif_cond_beta_multisynapse . src/aeif_cond_beta_multisynapse_kernel.h:42-226,
+//                                 src/aeif_cond_beta_multisynapse.cu:37-171
+//   aeif_cond_alpha_multisynapse  src/aeif_cond_alpha_multisynapse_kernel.h:163-226,
+//                                 src/aeif_cond_alpha_multisynapse.cu:37-80
+//   aeif_psc_exp ................ src/aeif_psc_exp_kernel.h:145-201, src/aeif_psc_exp.cu:37-80
+//   izhikevich_cond_beta ........ src/izhikevich_cond_beta_kernel.h:152-213,
+//                                 src/izhikevich_cond_beta.cu:37-100
+//   iaf_psc_exp ................. src/iaf_psc_exp.cu:68-175
+//   propagator_32 ............... src/propagator_stability.cu:31-52
+//   spike buffer / send list .... src/spike_buffer.cu:162-317, src/send_spike.cu:45-58
+//   delivery + fold ............. src/get_spike.cu:60-166
+//   poisson_generator ........... src/poiss_gen.cu:55-86, src/nestgpu.cu:1190-1226
+//   spike_generator ............. src/spike_generator.cu:49-67,230-269
+//   spike_detector .............. src/spike_detector.cu:58-79
+//   connection store ............ src/connect.cu:38-74
+//   step order / time ........... src/nestgpu.cu:294-352,359-400,469-635
+//   multimeter rows ............. src/multimeter.cu:76-110, src/base_neuron.cu:1205-1234
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "philox_ref.h"
+
+namespace orc {
+
+enum Model {
+  M_AEIF_COND_BETA_MS = 0,
+  M_AEIF_COND_ALPHA_MS,
+  M_AEIF_PSC_EXP,
+  M_IZH_COND_BETA,
+  M_IAF_PSC_EXP,
+  M_POISSON_GEN,
+  M_SPIKE_GEN,
+  M_SPIKE_DETECTOR,
+  N_MODEL
+};
+
+struct ModelInfo {
+  const char* name;
+  std::vector<std::string> scal_var, port_var, scal_param, port_param;
+  int forced_port;       // -1: user chooses
+  bool rk5;
+  int input_var;         // index of the receiving var (port block var for
+                         // multisynapse models, scalar var otherwise)
+  bool input_in_port_block;
+  int weight_param;      // port param index of the input weight, -1 => 1.0
+};
+
+static const ModelInfo& info(int m) {
+  static const std::vector<ModelInfo> tab = {
+      {"aeif_cond_beta_multisynapse",
+       {"V_m", "w"},
+       {"g", "g1"},
+       {"V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
+        "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"},
+       {"E_rev", "tau_rise", "tau_decay", "g0"},
+       -1, true, 1, true, 3},
+      {"aeif_cond_alpha_multisynapse",
+       {"V_m", "w"},
+       {"g", "g1"},
+       {"V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
+        "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"},
+       {"E_rev", "tau_syn", "g0"},
+       -1, true, 1, true, 2},
+      {"aeif_psc_exp",
+       {"I_syn_ex", "I_syn_in", "V_m", "w"},
+       {},
+       {"tau_syn_ex", "tau_syn_in", "V_th", "Delta_T", "g_L", "E_L", "C_m",
+        "a", "b", "tau_w", "I_e", "V_peak", "V_reset", "t_ref",
+        "refractory_step", "den_delay"},
+       {},
+       2, true, 0, false, -1},
+      {"izhikevich_cond_beta",
+       {"V_m", "u"},
+       {"g", "g1"},
+       {"V_th", "a", "b", "c", "d", "I_e", "t_ref", "refractory_step",
+        "den_delay"},
+       {"E_rev", "tau_rise", "tau_decay", "g0"},
+       -1, true, 1, true, 3},
+      {"iaf_psc_exp",
+       {"I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"},
+       {},
+       {"tau_m", "C_m", "E_L", "I_e", "Theta_rel", "V_reset_rel", "tau_ex",
+        "tau_in", "t_ref", "den_delay", "P20", "P11ex", "P11in", "P21ex",
+        "P21in", "P22"},
+       {},
+       2, false, 0, false, -1},
+      {"poisson_generator", {}, {}, {"rate", "origin", "start", "stop"}, {},
+       0, false, -1, false, -1},
+      {"spike_generator", {}, {}, {}, {}, 0, false, -1, false, -1},
+      {"spike_detector", {"input_spike_height", "spike_height"}, {},
+       {"hold_spike_height"}, {}, 1, false, 0, false, -1},
+  };
+  return tab.at(m);
+}
+
+static int model_by_name(const std::string& s) {
+  for (int m = 0; m < N_MODEL; ++m)
+    if (s == info(m).name) return m;
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Cash-Karp RK5(4) tableau, src/rk5.cu:35-72.  For T=float each constant is
+// the double expression rounded to float, exactly as the reference's
+// `__constant__ float` initialisers.
+template <class T>
+struct Tab {
+  static constexpr T c2 = T(0.2), c3 = T(0.3), c4 = T(0.6), c5 = T(1.0),
+                     c6 = T(0.875);
+  static constexpr T a21 = T(0.2);
+  static constexpr T a31 = T(3.0 / 40.0), a32 = T(9.0 / 40.0);
+  static constexpr T a41 = T(0.3), a42 = T(-0.9), a43 = T(1.2);
+  static constexpr T a51 = T(-11.0 / 54.0), a52 = T(2.5),
+                     a53 = T(-70.0 / 27.0), a54 = T(35.0 / 27.0);
+  static constexpr T a61 = T(1631.0 / 55296.0), a62 = T(175.0 / 512.0),
+                     a63 = T(575.0 / 13824.0), a64 = T(44275.0 / 110592.0),
+                     a65 = T(253.0 / 4096.0);
+  static constexpr T a71 = T(37.0 / 378.0), a73 = T(250.0 / 621.0),
+                     a74 = T(125.0 / 594.0), a76 = T(512.0 / 1771.0);
+  static constexpr T e1 = T(37.0 / 378.0 - 2825.0 / 27648.0),
+                     e3 = T(250.0 / 621.0 - 18575.0 / 48384.0),
+                     e4 = T(125.0 / 594.0 - 13525.0 / 55296.0),
+                     e5 = T(-277.00 / 14336.0), e6 = T(512.0 / 1771.0 - 0.25);
+  static constexpr T eps = T(1.0e-6), coeff = T(0.9), exp_inc = T(-0.2),
+                     exp_dec = T(-0.25), err_min = T(1.889568e-4),
+                     scal_min = T(1.0e-1);
+};
+
+// ---------------------------------------------------------------------------
+// Model right-hand sides.  `y` and `p` follow the reference index layout
+// (scalar block, then `port_var`/`port_param` interleaved per port).
+template <class T>
+struct AeifCondBeta {  // src/aeif_cond_beta_multisynapse_kernel.h:165-188
+  static void deriv(int np, const T* y, T* dy, const T* p) {
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T I_syn = 0;
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
+    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
+    T V_spike = p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
+                            p[I_e]) / p[C_m];
+    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    for (int i = 0; i < np; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 4 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 4 * i + 2];
+    }
+  }
+};
+
+template <class T>
+struct AeifCondAlpha {  // src/aeif_cond_alpha_multisynapse_kernel.h:163-186
+  static void deriv(int np, const T* y, T* dy, const T* p) {
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T I_syn = 0;
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
+    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
+    T V_spike = p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
+                            p[I_e]) / p[C_m];
+    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    for (int i = 0; i < np; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 3 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 3 * i + 1];
+    }
+  }
+};
+
+template <class T>
+struct AeifPscExp {  // src/aeif_psc_exp_kernel.h:145-162
+  static void deriv(int, const T* y, T* dy, const T* p) {
+    enum { I_ex, I_in, V_m, w };
+    enum { tau_ex, tau_in, V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e,
+           V_peak, V_reset, t_ref, ref, den };
+    T I_syn_tot = 0;
+    I_syn_tot += y[I_ex] - y[I_in];
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[V_m], p[V_peak]);
+    T V_spike = p[Delta_T] == 0. ? T(0.)
+                                 : p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+    dy[V_m] = (p[ref] > 0) ? T(0)
+                           : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot -
+                              y[w] + p[I_e]) / p[C_m];
+    dy[w] = (p[a] * (V - p[E_L]) - y[w]) / p[tau_w];
+    dy[I_ex] = -y[I_ex] / p[tau_ex];
+    dy[I_in] = -y[I_in] / p[tau_in];
+  }
+};
+
+template <class T>
+struct IzhCondBeta {  // src/izhikevich_cond_beta_kernel.h:152-170
+  static void deriv(int np, const T* y, T* dy, const T* p) {
+    enum { V_th, a, b, c, d, I_e, t_ref, ref, den };
+    T I_syn = 0;
+    T V = (p[ref] > 0) ? p[c] : y[0];
+    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[9 + 4 * i] - V);
+    // The reference evaluates this polynomial in double (double literals)
+    // and stores the result in the state type.
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : T(0.04 * V * V + 5.0 * V + 140.0 - y[1] + I_syn +
+                             p[I_e]);
+    dy[1] = p[a] * (p[b] * V - y[1]);
+    for (int i = 0; i < np; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[9 + 4 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[9 + 4 * i + 2];
+    }
+  }
+};
+
+// Test-only linear model y' = lambda*y (param[0] = lambda).
+template <class T>
+struct LinearModel {
+  static void deriv(int, const T* y, T* dy, const T* p) { dy[0] = p[0] * y[0]; }
+};
+
+constexpr int kMaxVar = 2 + 2 * 32;
+
+struct StepStats {
+  long long attempts = 0;   // accepted + rejected RK5 attempts
+  long long rejected = 0;
+  long long substeps = 0;   // accepted attempts
+};
+
+// src/rk5.h:69-159.  `nv` state variables, k1 outside the retry loop,
+// y_scal computed once from the pre-loop h.
+template <class T, class M>
+void rk5_step(int np, int nv, double& x, T* y, T& h, T h_min, T h_max,
+              const T* p, StepStats* st) {
+  using K = Tab<T>;
+  T y_new[kMaxVar], k1[kMaxVar], k2[kMaxVar], k3[kMaxVar], k4[kMaxVar],
+      k5[kMaxVar], k6[kMaxVar], y_scal[kMaxVar];
+  M::deriv(np, y, k1, p);
+  for (int i = 0; i < nv; ++i)
+    y_scal[i] = std::fabs(y[i]) + std::fabs(k1[i] * h) + K::scal_min;
+  T err;
+  for (;;) {
+    if (h > h_max) h = h_max;
+    if (h < h_min) h = h_min;
+    for (int i = 0; i < nv; ++i) y_new[i] = y[i] + h * K::a21 * k1[i];
+    M::deriv(np, y_new, k2, p);
+    for (int i = 0; i < nv; ++i)
+      y_new[i] = y[i] + h * (K::a31 * k1[i] + K::a32 * k2[i]);
+    M::deriv(np, y_new, k3, p);
+    for (int i = 0; i < nv; ++i)
+      y_new[i] = y[i] + h * (K::a41 * k1[i] + K::a42 * k2[i] + K::a43 * k3[i]);
+    M::deriv(np, y_new, k4, p);
+    for (int i = 0; i < nv; ++i)
+      y_new[i] = y[i] + h * (K::a51 * k1[i] + K::a52 * k2[i] +
+                             K::a53 * k3[i] + K::a54 * k4[i]);
+    M::deriv(np, y_new, k5, p);
+    for (int i = 0; i < nv; ++i)
+      y_new[i] = y[i] + h * (K::a61 * k1[i] + K::a62 * k2[i] +
+                             K::a63 * k3[i] + K::a64 * k4[i] + K::a65 * k5[i]);
+    M::deriv(np, y_new, k6, p);
+    for (int i = 0; i < nv; ++i)
+      y_new[i] = y[i] + h * (K::a71 * k1[i] + K::a73 * k3[i] +
+                             K::a74 * k4[i] + K::a76 * k6[i]);
+    err = 0.0;
+    for (int i = 0; i < nv; ++i) {
+      T val = h * (K::e1 * k1[i] + K::e3 * k3[i] + K::e4 * k4[i] +
+                   K::e5 * k5[i] + K::e6 * k6[i]);
+      val /= y_scal[i];
+      err = std::max(err, (T)std::fabs(val));
+    }
+    err /= K::eps;
+    if (st) st->attempts++;
+    // double comparison as in the reference (1.0e-5 is a double literal)
+    if (err <= 1.0 || (double)h <= (double)h_min * (1.0 + 1.0e-5)) break;
+    if (st) st->rejected++;
+    T h_new = h * K::coeff * std::pow(err, K::exp_dec);
+    h = (T)std::max((double)h_new, 0.1 * (double)h);
+  }
+  if (st) st->substeps++;
+  x += h;
+  if (err > K::err_min)
+    h = h * K::coeff * std::pow(err, K::exp_inc);
+  else
+    h = (T)(5.0 * (double)h);
+  for (int i = 0; i < nv; ++i) y[i] = y_new[i];
+}
+
+// src/propagator_stability.cu:31-52 (FP64): P32 with the singular-case switch
+static double propagator_32(double tau_syn, double tau, double C, double h) {
+  const double P32_linear = 1.0 / (2.0 * C * tau * tau) * h * h *
+                            (tau_syn - tau) * std::exp(-h / tau);
+  const double P32_singular = h / C * std::exp(-h / tau);
+  const double P32 = -tau / (C * (1.0 - tau / tau_syn)) * std::exp(-h / tau_syn) *
+                     std::expm1(h * (1.0 / tau_syn - 1.0 / tau));
+  const double dev_P32 = std::fabs(P32 - P32_singular);
+  if (tau == tau_syn ||
+      (std::fabs(tau - tau_syn) < 0.1 && dev_P32 > 2.0 * std::fabs(P32_linear)))
+    return P32_singular;
+  return P32;
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+struct Group {
+  int model = 0;
+  int i0 = 0, n = 0, n_port = 0;
+  int nv = 0, np_ = 0;               // vars / params per node
+  std::vector<T> y;                  // AoS [n][nv]
+  std::vector<T> p;                  // AoS [n][np_]
+  std::vector<double> x;             // RK5 time per node
+  std::vector<T> h;                  // RK5 step per node
+  float h_min_rel = 1.0e-3f, h0_rel = 1.0e-2f;
+  float h_min = 0, h0 = 0;
+  std::vector<double> acc;           // get_spike_array [port][n]
+  // spike generator
+  std::vector<std::vector<float>> sg_times, sg_heights;
+  std::vector<std::vector<int>> sg_idx;
+  std::vector<int> sg_next;
+  // poisson generator direct connections (src/dir_connect.h:30-37)
+  struct DirConn { int irel; int target; int port; float weight; float delay;
+                   uint32_t stream; };
+  std::vector<DirConn> dir;
+  std::vector<int> pg_chunks;      // per generator: Poisson chunks
+  std::vector<double> pg_exp;      // per generator: exp(-lambda/chunks)
+};
+
+struct Record { int node; int group; int rel; std::string var; int port;
+                std::vector<float> t, v; };
+
+template <class T>
+class Engine {
+ public:
+  Engine(float dt, uint64_t seed) : dt_(dt), seed_(seed) {}
+
+  int create(const std::string& name, int n, int n_port) {
+    if (calibrated_) throw std::runtime_error("Nodes cannot be created after calibration");
+    int m = model_by_name(name);
+    if (m < 0) throw std::runtime_error("Unknown neuron model name: " + name);
+    const ModelInfo& mi = info(m);
+    if (mi.forced_port >= 0) n_port = mi.forced_port;
+    Group<T> g;
+    g.model = m;
+    g.i0 = n_nodes_;
+    g.n = n;
+    g.n_port = n_port;
+    g.nv = mi.scal_var.size() + mi.port_var.size() * n_port;
+    g.np_ = mi.scal_param.size() + mi.port_param.size() * n_port;
+    g.y.assign((size_t)n * g.nv, T(0));
+    g.p.assign((size_t)n * g.np_, T(0));
+    init_defaults(g);
+    if (m == M_SPIKE_GEN) {
+      g.sg_times.assign(n, {});
+      g.sg_heights.assign(n, {});
+      g.sg_idx.assign(n, {});
+      g.sg_next.assign(n, 0);
+    }
+    groups_.push_back(std::move(g));
+    for (int i = 0; i < n; ++i) node_group_.push_back(groups_.size() - 1);
+    n_nodes_ += n;
+    return n_nodes_ - n;
+  }
+
+  // Index of a named parameter / variable of group g, or -1.
+  int param_index(const Group<T>& g, const std::string& name, int port) const {
+    const ModelInfo& mi = info(g.model);
+    for (size_t i = 0; i < mi.scal_param.size(); ++i)
+      if (mi.scal_param[i] == name) return i;
+    for (size_t i = 0; i < mi.port_param.size(); ++i)
+      if (mi.port_param[i] == name)
+        return mi.scal_param.size() + port * mi.port_param.size() + i;
+    return -1;
+  }
+  int var_index(const Group<T>& g, const std::string& name, int port) const {
+    const ModelInfo& mi = info(g.model);
+    for (size_t i = 0; i < mi.scal_var.size(); ++i)
+      if (mi.scal_var[i] == name) return i;
+    for (size_t i = 0; i < mi.port_var.size(); ++i)
+      if (mi.port_var[i] == name)
+        return mi.scal_var.size() + port * mi.port_var.size() + i;
+    return -1;
+  }
+
+  void set(int node0, int n, const std::string& name, int port,
+           const float* vals) {
+    Group<T>& g = groups_.at(node_group_.at(node0));
+    if (name == "h_min_rel") { g.h_min_rel = vals[0]; return; }
+    if (name == "h0_rel") { g.h0_rel = vals[0]; return; }
+    int ip = param_index(g, name, port);
+    int iv = var_index(g, name, port);
+    if (ip < 0 && iv < 0) throw std::runtime_error("Unrecognized name " + name);
+    for (int k = 0; k < n; ++k) {
+      int rel = node0 + k - g.i0;
+      if (ip >= 0) g.p[(size_t)rel * g.np_ + ip] = (T)vals[k];
+      else g.y[(size_t)rel * g.nv + iv] = (T)vals[k];
+    }
+  }
+
+  double get(int node, const std::string& name, int port) const {
+    const Group<T>& g = groups_.at(node_group_.at(node));
+    int rel = node - g.i0;
+    int ip = param_index(g, name, port);
+    if (ip >= 0) return g.p[(size_t)rel * g.np_ + ip];
+    int iv = var_index(g, name, port);
+    if (iv >= 0) return g.y[(size_t)rel * g.nv + iv];
+    throw std::runtime_error("Unrecognized name " + name);
+  }
+
+  void set_spike_times(int node, int n, const float* t, const float* hgt) {
+    Group<T>& g = groups_.at(node_group_.at(node));
+    int rel = node - g.i0;
+    g.sg_times[rel].assign(t, t + n);
+    if (hgt) g.sg_heights[rel].assign(hgt, hgt + n);
+    else g.sg_heights[rel].clear();
+  }
+
+  // src/connect.cu:38-54: d_int = round(delay/dt) - 1; delay < dt rejected.
+  void connect(long long n, const int* src, const int* tgt,
+               const unsigned char* port, const float* w, const float* delay) {
+    if (calibrated_) throw std::runtime_error("Connections cannot be created after calibration");
+    for (long long i = 0; i < n; ++i) {
+      if (delay[i] < dt_) throw std::runtime_error("Delay must be >= time resolution");
+      int d_int = (int)std::round(delay[i] / dt_) - 1;
+      c_src_.push_back(src[i]);
+      c_tgt_.push_back(tgt[i]);
+      c_port_.push_back(port[i]);
+      c_w_.push_back(w[i]);
+      c_d_.push_back(d_int);
+    }
+  }
+
+  int record(int node, const std::string& var, int port) {
+    Record r;
+    r.node = node;
+    r.group = node_group_.at(node);
+    r.rel = node - groups_[r.group].i0;
+    r.var = var;
+    r.port = port;
+    records_.push_back(r);
+    return records_.size() - 1;
+  }
+
+  void simulate(float sim_time) {
+    if (!calibrated_) calibrate();
+    if (first_sim_) {
+      write_records(neural_time_);
+      first_sim_ = false;
+    }
+    neur_t0_ = neural_time_;
+    long long Nt = (long long)std::round(sim_time / dt_);
+    update_poisson_lambda();
+    for (long long it = 0; it < Nt; ++it) step(it);
+  }
+
+  // public results
+  std::vector<Record> records_;
+  std::vector<long long> ev_t_;   // spike events: time index
+  std::vector<int> ev_node_;      // spike events: node
+  StepStats stats_;
+  long long n_syn_events_ = 0;    // delivered (spike, synapse) pairs
+  long long n_conn() const { return c_src_.size(); }
+
+ private:
+  // ---- defaults, src/*.cu NodeInit / Init -------------------------------
+  void init_defaults(Group<T>& g) {
+    const ModelInfo& mi = info(g.model);
+    // defaults are the reference's float constants
+    auto P = [&](int rel, const char* nm, int port, double v) {
+      g.p[(size_t)rel * g.np_ + param_index(g, nm, port)] = (T)(float)v;
+    };
+    auto V = [&](int rel, const char* nm, int port, double v) {
+      g.y[(size_t)rel * g.nv + var_index(g, nm, port)] = (T)(float)v;
+    };
+    for (int r = 0; r < g.n; ++r) {
+      switch (g.model) {
+        case M_AEIF_COND_BETA_MS:
+        case M_AEIF_COND_ALPHA_MS:
+        case M_AEIF_PSC_EXP:
+          P(r, "V_th", 0, -50.4); P(r, "Delta_T", 0, 2.0); P(r, "g_L", 0, 30.0);
+          P(r, "E_L", 0, -70.6); P(r, "C_m", 0, 281.0); P(r, "a", 0, 4.0);
+          P(r, "b", 0, 80.5); P(r, "tau_w", 0, 144.0); P(r, "I_e", 0, 0.0);
+          P(r, "V_peak", 0, 0.0); P(r, "V_reset", 0, -60.0); P(r, "t_ref", 0, 0.0);
+          P(r, "den_delay", 0, 0.0); P(r, "refractory_step", 0, 0.0);
+          V(r, "V_m", 0, -70.6); V(r, "w", 0, 0.0);  // V_m = E_L
+          if (g.model == M_AEIF_PSC_EXP) {
+            P(r, "tau_syn_ex", 0, 0.2); P(r, "tau_syn_in", 0, 2.0);
+          } else {
+            for (int q = 0; q < g.n_port; ++q) {
+              P(r, "E_rev", q, 0.0);
+              if (g.model == M_AEIF_COND_BETA_MS) {
+                P(r, "tau_decay", q, 20.0); P(r, "tau_rise", q, 2.0);
+              } else {
+                P(r, "tau_syn", q, 2.0);
+              }
+            }
+          }
+          break;
+        case M_IZH_COND_BETA:
+          P(r, "V_th", 0, 30.0); P(r, "a", 0, 0.02); P(r, "b", 0, 0.2);
+          P(r, "c", 0, -65.0); P(r, "d", 0, 8.0); P(r, "I_e", 0, 0.0);
+          P(r, "t_ref", 0, 0.0); P(r, "den_delay", 0, 0.0);
+          V(r, "V_m", 0, -70.0);
+          // u = b*V_m in float, src/izhikevich_cond_beta.cu:57
+          V(r, "u", 0, (double)((float)0.2f * (float)-70.0f));
+          for (int q = 0; q < g.n_port; ++q) {
+            P(r, "E_rev", q, 0.0); P(r, "tau_decay", q, 20.0); P(r, "tau_rise", q, 2.0);
+          }
+          break;
+        case M_IAF_PSC_EXP:  // src/iaf_psc_exp.cu:134-155
+          P(r, "tau_m", 0, 10.0); P(r, "C_m", 0, 250.0); P(r, "E_L", 0, -70.0);
+          P(r, "I_e", 0, 0.0); P(r, "Theta_rel", 0, 15.0); P(r, "V_reset_rel", 0, 0.0);
+          P(r, "tau_ex", 0, 2.0); P(r, "tau_in", 0, 2.0); P(r, "t_ref", 0, 2.0);
+          P(r, "den_delay", 0, 0.0);
+          break;
+        case M_POISSON_GEN:
+          P(r, "rate", 0, 0.0); P(r, "origin", 0, 0.0); P(r, "start", 0, 0.0);
+          P(r, "stop", 0, 1.0e30);
+          break;
+        case M_SPIKE_DETECTOR:
+          P(r, "hold_spike_height", 0, 1.0);
+          break;
+        default:
+          break;
+      }
+    }
+    (void)mi;
+  }
+
+  // ---- calibrate, src/nestgpu.cu:294-352 ----------------------------------
+  void calibrate() {
+    calibrated_ = true;
+    neural_time_ = t_min_;
+    // connection store sorted like NetConnection::Insert: per source,
+    // delay ascending, insertion order within a delay.
+    size_t nc = c_src_.size();
+    std::vector<long long> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) {
+      if (c_src_[a] != c_src_[b]) return c_src_[a] < c_src_[b];
+      return c_d_[a] < c_d_[b];
+    });
+    s_tgt_.resize(nc); s_port_.resize(nc); s_w_.resize(nc);
+    grp_off_.assign(n_nodes_ + 1, 0);
+    std::vector<int> src_sorted(nc), d_sorted(nc);
+    for (size_t k = 0; k < nc; ++k) {
+      long long o = order[k];
+      s_tgt_[k] = c_tgt_[o]; s_port_[k] = c_port_[o]; s_w_[k] = c_w_[o];
+      src_sorted[k] = c_src_[o]; d_sorted[k] = c_d_[o];
+    }
+    // delay groups per source
+    grp_start_.clear(); grp_delay_.clear();
+    size_t k = 0;
+    for (int s = 0; s < n_nodes_; ++s) {
+      grp_off_[s] = grp_start_.size();
+      while (k < nc && src_sorted[k] == s) {
+        int d = d_sorted[k];
+        grp_start_.push_back(k);
+        grp_delay_.push_back(d);
+        while (k < nc && src_sorted[k] == s && d_sorted[k] == d) ++k;
+      }
+    }
+    grp_off_[n_nodes_] = grp_start_.size();
+    grp_start_.push_back(nc);
+    // BuildDirectConnections, src/nestgpu.cu:1190-1226.  The stream id of
+    // each direct connection is its index in the sorted connection store.
+    for (auto& g : groups_) {
+      if (g.model != M_POISSON_GEN) continue;
+      for (int s = g.i0; s < g.i0 + g.n; ++s) {
+        for (int gi = grp_off_[s]; gi < grp_off_[s + 1]; ++gi) {
+          for (long long c = grp_start_[gi]; c < grp_start_[gi + 1]; ++c) {
+            typename Group<T>::DirConn dc;
+            dc.irel = s - g.i0;
+            dc.target = s_tgt_[c];
+            dc.port = s_port_[c];
+            dc.weight = s_w_[c];
+            dc.delay = dt_ * (grp_delay_[gi] + 1);
+            dc.stream = (uint32_t)c;
+            g.dir.push_back(dc);
+          }
+        }
+      }
+    }
+    // spike buffers, src/spike_buffer.cu:331-553
+    sb_size_.assign(n_nodes_, 0);
+    sb_idx0_.assign(n_nodes_, 0);
+    sb_time_.assign((size_t)n_nodes_ * max_sb_, 0);
+    sb_conn_.assign((size_t)n_nodes_ * max_sb_, 0);
+    sb_height_.assign((size_t)n_nodes_ * max_sb_, 0.f);
+    last_spike_idx_.assign(n_nodes_, 0x70000000LL);
+    // group calibrate
+    for (auto& g : groups_) {
+      g.acc.assign((size_t)g.n * g.n_port, 0.0);
+      const ModelInfo& mi = info(g.model);
+      if (mi.rk5) {
+        g.h_min = g.h_min_rel * dt_;
+        g.h0 = g.h0_rel * dt_;
+        g.x.assign(g.n, t_min_);
+        g.h.assign(g.n, (T)g.h0);
+        for (int r = 0; r < g.n; ++r) node_calibrate(g, r);
+      } else if (g.model == M_IAF_PSC_EXP) {
+        for (int r = 0; r < g.n; ++r) iaf_calibrate(g, r);
+      } else if (g.model == M_SPIKE_GEN) {
+        for (int r = 0; r < g.n; ++r) sg_calibrate(g, r);
+      }
+    }
+  }
+
+  // src/aeif_cond_beta_multisynapse.cu:72-99 (and the izhikevich /
+  // alpha / psc_exp counterparts)
+  void node_calibrate(Group<T>& g, int r) {
+    T* p = &g.p[(size_t)r * g.np_];
+    int ref = param_index(g, "refractory_step", 0);
+    p[ref] = 0;
+    int ns = info(g.model).scal_param.size();
+    if (g.model == M_AEIF_COND_BETA_MS || g.model == M_IZH_COND_BETA) {
+      for (int i = 0; i < g.n_port; ++i) {
+        T* pp = p + ns + 4 * i;  // E_rev, tau_rise, tau_decay, g0
+        T tr = pp[1], td = pp[2];
+        T denom1 = td - tr;
+        T denom2 = 0;
+        if (denom1 != 0) {
+          T t_p = td * tr * std::log(td / tr) / denom1;
+          denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
+        }
+        if (denom2 == 0)
+          pp[3] = (T)(M_E / td);
+        else
+          pp[3] = (T)((1. / tr - 1. / td) / denom2);
+      }
+    } else if (g.model == M_AEIF_COND_ALPHA_MS) {
+      for (int i = 0; i < g.n_port; ++i) {
+        T* pp = p + ns + 3 * i;  // E_rev, tau_syn, g0
+        pp[2] = (T)(M_E / pp[1]);
+      }
+    } else if (g.model == M_AEIF_PSC_EXP) {
+      int iDT = param_index(g, "Delta_T", 0);
+      if (p[iDT] <= 0.0) p[param_index(g, "V_peak", 0)] = p[param_index(g, "V_th", 0)];
+    }
+  }
+
+  // src/iaf_psc_exp.cu:68-82 (coefficients in float as the reference)
+  void iaf_calibrate(Group<T>& g, int r) {
+    T* p = &g.p[(size_t)r * g.np_];
+    enum { tau_m, C_m, E_L, I_e, Theta, V_reset, tau_ex, tau_in, t_ref, den,
+           P20, P11ex, P11in, P21ex, P21in, P22 };
+    float h = dt_;
+    float tm = (float)p[tau_m], cm = (float)p[C_m], te = (float)p[tau_ex],
+          ti = (float)p[tau_in];
+    p[P11ex] = (T)std::exp(-h / te);
+    p[P11in] = (T)std::exp(-h / ti);
+    float p22 = std::exp(-h / tm);
+    p[P22] = (T)p22;
+    p[P21ex] = (T)(float)propagator_32(te, tm, cm, h);
+    p[P21in] = (T)(float)propagator_32(ti, tm, cm, h);
+    p[P20] = (T)(float)(tm / cm * (1.0 - p22));
+  }
+
+  // src/spike_generator.cu:230-269
+  void sg_calibrate(Group<T>& g, int r) {
+    auto& t = g.sg_times[r];
+    auto& h = g.sg_heights[r];
+    if (t.empty()) return;
+    if (h.empty()) h.assign(t.size(), 1.0f);
+    else if (h.size() != t.size())
+      throw std::runtime_error("spike time array and spike height array must have the same size");
+    g.sg_idx[r].resize(t.size());
+    float tmin = (float)t_min_;
+    for (size_t i = 0; i < t.size(); ++i) {
+      g.sg_idx[r][i] = (int)std::round((t[i] - tmin) / dt_);
+      if (i > 0 && g.sg_idx[r][i] <= g.sg_idx[r][i - 1])
+        throw std::runtime_error("Spike times must be ordered");
+    }
+  }
+
+  // ---- one simulation step, src/nestgpu.cu:469-635 -------------------------
+  void step(long long it) {
+    // 1. SpikeBufferUpdate -> send list
+    send_src_.clear(); send_grp_.clear(); send_h_.clear();
+    spike_buffer_update();
+    // 2. time
+    neural_time_ = neur_t0_ + (double)dt_ * (it + 1);
+    time_idx_ = (long long)(int)std::round(neur_t0_ / dt_) + it + 1;
+    // 3. node updates, in creation order
+    for (auto& g : groups_) update_group(g, it);
+    // 4. multimeter
+    write_records(neural_time_);
+    // 5. clear accumulators, collect, poisson, fold
+    for (auto& g : groups_) std::fill(g.acc.begin(), g.acc.end(), 0.0);
+    collect();
+    for (auto& g : groups_)
+      if (g.model == M_POISSON_GEN) poisson_send(g);
+    for (auto& g : groups_) fold(g);
+  }
+
+  void spike_buffer_update() {
+    // src/spike_buffer.cu:247-317 (no dendritic delay: den_delay_idx = 0)
+    for (int s = 0; s < n_nodes_; ++s) {
+      int is0 = sb_idx0_[s];
+      int Ns = sb_size_[s];
+      int ngrp = grp_off_[s + 1] - grp_off_[s];
+      for (int is = 0; is < Ns; ++is) {
+        int is1 = (is0 + is) % max_sb_;
+        size_t ia = (size_t)is1 * n_nodes_ + s;
+        int i_conn = sb_conn_[ia];
+        int st = sb_time_[ia];
+        if (i_conn < ngrp && st == grp_delay_[grp_off_[s] + i_conn]) {
+          send_src_.push_back(s);
+          send_grp_.push_back(grp_off_[s] + i_conn);
+          send_h_.push_back(sb_height_[ia]);
+          i_conn++;
+          sb_conn_[ia] = i_conn;
+        }
+        if (is == Ns - 1 && i_conn >= ngrp && st + 1 >= 0)
+          sb_size_[s]--;
+        else
+          sb_time_[ia]++;
+      }
+    }
+  }
+
+  // src/spike_buffer.cu:162-242 (recording parts kept to what the tests read)
+  void push_spike(int node, float height) {
+    last_spike_idx_[node] = time_idx_;
+#pragma omp critical(orc_events)
+    {
+      ev_t_.push_back(time_idx_);
+      ev_node_.push_back(node);
+    }
+    if (grp_off_[node + 1] - grp_off_[node] > 0) {
+      int Ns = sb_size_[node];
+      if (Ns >= max_sb_) return;  // buffer overflow: spike dropped
+      sb_size_[node]++;
+      int is0 = (sb_idx0_[node] + max_sb_ - 1) % max_sb_;
+      sb_idx0_[node] = is0;
+      size_t ia = (size_t)is0 * n_nodes_ + node;
+      sb_time_[ia] = 0;
+      sb_conn_[ia] = 0;
+      sb_height_[ia] = height;
+    }
+  }
+
+  void update_group(Group<T>& g, long long it) {
+    const ModelInfo& mi = info(g.model);
+    if (mi.rk5) {
+      StepStats st_tot;
+#pragma omp parallel
+      {
+        StepStats st;
+#pragma omp for schedule(static)
+        for (int r = 0; r < g.n; ++r) rk5_update(g, r, &st);
+#pragma omp critical(orc_stats)
+        {
+          st_tot.attempts += st.attempts;
+          st_tot.rejected += st.rejected;
+          st_tot.substeps += st.substeps;
+        }
+      }
+      stats_.attempts += st_tot.attempts;
+      stats_.rejected += st_tot.rejected;
+      stats_.substeps += st_tot.substeps;
+    } else if (g.model == M_IAF_PSC_EXP) {
+#pragma omp parallel for schedule(static)
+      for (int r = 0; r < g.n; ++r) iaf_update(g, r);
+    } else if (g.model == M_SPIKE_GEN) {
+      for (int r = 0; r < g.n; ++r) {
+        auto& idx = g.sg_idx[r];
+        int is = g.sg_next[r];
+        if (!idx.empty() && is < (int)idx.size() && idx[is] == it) {
+          push_spike(g.i0 + r, g.sg_heights[r][is]);
+          g.sg_next[r]++;
+        }
+      }
+    } else if (g.model == M_SPIKE_DETECTOR) {
+      for (int r = 0; r < g.n; ++r) {
+        T* y = &g.y[(size_t)r * g.nv];
+        T* p = &g.p[(size_t)r * g.np_];
+        T sh = y[0];
+        if (sh != 0.0) {
+          if (p[0] == 0.0) sh = 1.0;
+          y[0] = 0;
+        }
+        y[1] = sh;
+      }
+    }
+  }
+
+  // src/rk5.h:161-207 with the model's ExternalUpdate
+  void rk5_update(Group<T>& g, int r, StepStats* st) {
+    T* y = &g.y[(size_t)r * g.nv];
+    T* p = &g.p[(size_t)r * g.np_];
+    double& x = g.x[r];
+    T& h = g.h[r];
+    double x1 = neural_time_;
+    T h_min = (T)g.h_min;
+    bool end = false;
+    while (!end) {
+      T hmax = (T)(x1 - x);
+      switch (g.model) {
+        case M_AEIF_COND_BETA_MS:
+          rk5_step<T, AeifCondBeta<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_COND_ALPHA_MS:
+          rk5_step<T, AeifCondAlpha<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_PSC_EXP:
+          rk5_step<T, AeifPscExp<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_IZH_COND_BETA:
+          rk5_step<T, IzhCondBeta<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+      }
+      end = (x >= x1 - (double)h_min);
+      external_update(g, r, y, p, end);
+    }
+  }
+
+  // ExternalUpdate of the four RK5 models
+  void external_update(Group<T>& g, int r, T* y, T* p, bool end) {
+    int iV = (g.model == M_AEIF_PSC_EXP) ? 2 : 0;
+    int iw = iV + 1;
+    int ref = param_index(g, "refractory_step", 0);
+    int i_reset, i_thr, i_jump;
+    if (g.model == M_IZH_COND_BETA) {
+      i_reset = 3; i_thr = 0; i_jump = 4;  // c, V_th, d
+    } else {
+      int off = (g.model == M_AEIF_PSC_EXP) ? 2 : 0;
+      i_reset = 10 + off; i_thr = 9 + off; i_jump = 6 + off;  // V_reset, V_peak, b
+    }
+    int i_tref = ref - 1;
+    if (y[iV] < -1.0e3) { y[iV] = p[i_reset]; y[iw] = 0; return; }
+    if (y[iw] < -1.0e6 || y[iw] > 1.0e6) { y[iV] = p[i_reset]; y[iw] = 0; return; }
+    if (p[ref] > 0.0) {
+      y[iV] = p[i_reset];
+      if (end) p[ref] -= 1.0;
+    } else if (y[iV] >= p[i_thr]) {
+      push_spike(g.i0 + r, 1.0f);
+      y[iV] = p[i_reset];
+      y[iw] += p[i_jump];
+      // (int)round(t_ref/dt) in float
+      p[ref] = (T)(int)std::round((float)p[i_tref] / dt_);
+      if (p[ref] < 0) p[ref] = 0;
+    }
+  }
+
+  // src/iaf_psc_exp.cu:85-110
+  void iaf_update(Group<T>& g, int r) {
+    T* y = &g.y[(size_t)r * g.nv];
+    T* p = &g.p[(size_t)r * g.np_];
+    enum { I_ex, I_in, V, ref };
+    enum { tau_m, C_m, E_L, I_e, Theta, V_reset, tau_ex, tau_in, t_ref, den,
+           P20, P11ex, P11in, P21ex, P21in, P22 };
+    if (y[ref] > 0.0) {
+      y[ref] -= 1.0;
+    } else {
+      y[V] = y[V] * p[P22] + y[I_ex] * p[P21ex] + y[I_in] * p[P21in] + p[I_e] * p[P20];
+    }
+    y[I_ex] *= p[P11ex];
+    y[I_in] *= p[P11in];
+    if (y[V] >= p[Theta]) {
+      push_spike(g.i0 + r, 1.0f);
+      y[V] = p[V_reset];
+      y[ref] = (T)(int)std::round((float)p[t_ref] / dt_);
+    }
+  }
+
+  // src/get_spike.cu:60-108: acc += (double)(height*weight)
+  void collect() {
+    long long nsend = send_src_.size();
+    for (long long k = 0; k < nsend; ++k) {
+      int gi = send_grp_[k];
+      float hgt = send_h_[k];
+      for (long long c = grp_start_[gi]; c < grp_start_[gi + 1]; ++c) {
+        int tgt = s_tgt_[c];
+        Group<T>& tg = groups_[node_group_[tgt]];
+        size_t i = (size_t)s_port_[c] * tg.n + (tgt - tg.i0);
+        tg.acc[i] += (double)(hgt * s_w_[c]);
+        ++n_syn_events_;
+      }
+    }
+  }
+
+  void update_poisson_lambda() {
+    // time_step*rate in float, src/poiss_gen.cu:74 and src/nestgpu.cu:560
+    float time_step = (float)(dt_ / 1000.0);
+    for (auto& g : groups_) {
+      if (g.model != M_POISSON_GEN) continue;
+      g.pg_chunks.resize(g.n);
+      g.pg_exp.resize(g.n);
+      for (int r = 0; r < g.n; ++r) {
+        float lambda = time_step * (float)g.p[(size_t)r * g.np_ + 0];
+        int m = poisson_chunks(lambda);
+        g.pg_chunks[r] = m;
+        g.pg_exp[r] = m > 0 ? std::exp(-(double)lambda / m) : 0.0;
+      }
+    }
+  }
+
+  // src/poiss_gen.cu:55-86
+  void poisson_send(Group<T>& g) {
+    for (auto& dc : g.dir) {
+      const T* p = &g.p[(size_t)dc.irel * g.np_];
+      double t_rel = neural_time_ - (float)p[1] - dc.delay;
+      if (t_rel >= (float)p[2] && t_rel <= (float)p[3]) {
+        int n = poisson_draw(seed_, dc.stream, time_idx_, g.pg_chunks[dc.irel],
+                             g.pg_exp[dc.irel]);
+        if (n > 0) {
+          Group<T>& tg = groups_[node_group_[dc.target]];
+          size_t i = (size_t)dc.port * tg.n + (dc.target - tg.i0);
+          tg.acc[i] += (double)(dc.weight * n);
+        }
+      }
+    }
+  }
+
+  // src/get_spike.cu:142-166: input = (float)((double)input + acc*weight)
+  void fold(Group<T>& g) {
+    if (g.n_port == 0) return;
+    const ModelInfo& mi = info(g.model);
+    int nsv = mi.scal_var.size(), npv = mi.port_var.size();
+    int nsp = mi.scal_param.size(), npp = mi.port_param.size();
+    for (int q = 0; q < g.n_port; ++q) {
+      for (int r = 0; r < g.n; ++r) {
+        double a = g.acc[(size_t)q * g.n + r];
+        int iv = mi.input_in_port_block ? nsv + q * npv + mi.input_var
+                                        : mi.input_var + q;
+        T* in = &g.y[(size_t)r * g.nv + iv];
+        double wgt = 1.0;
+        if (mi.weight_param >= 0) wgt = (double)g.p[(size_t)r * g.np_ + nsp + q * npp + mi.weight_param];
+        *in = (T)((double)*in + a * wgt);
+      }
+    }
+  }
+
+  // src/multimeter.cu:76-110
+  void write_records(double t) {
+    for (auto& r : records_) {
+      Group<T>& g = groups_[r.group];
+      float v;
+      if (r.var == "spike") {
+        // GetSpikeActivity: most recent buffered spike of age 0
+        int s = r.node;
+        v = 0.f;
+        if (sb_size_[s] > 0) {
+          size_t ia = (size_t)sb_idx0_[s] * n_nodes_ + s;
+          if (sb_time_[ia] == 0) v = sb_height_[ia];
+        }
+      } else {
+        int iv = var_index(g, r.var, r.port);
+        if (iv < 0) throw std::runtime_error("Unrecognized variable " + r.var);
+        v = (float)g.y[(size_t)r.rel * g.nv + iv];
+      }
+      r.t.push_back((float)t);
+      r.v.push_back(v);
+    }
+  }
+
+  float dt_;
+  uint64_t seed_;
+  bool calibrated_ = false, first_sim_ = true;
+  double t_min_ = 0.0, neural_time_ = 0.0, neur_t0_ = 0.0;
+  long long time_idx_ = 0;
+  int n_nodes_ = 0;
+  int max_sb_ = 20;
+  std::vector<Group<T>> groups_;
+  std::vector<int> node_group_;
+  // connections (insertion order), then sorted store
+  std::vector<int> c_src_, c_tgt_, c_d_;
+  std::vector<unsigned char> c_port_;
+  std::vector<float> c_w_;
+  std::vector<int> s_tgt_;
+  std::vector<unsigned char> s_port_;
+  std::vector<float> s_w_;
+  std::vector<int> grp_off_;          // per source: first delay group
+  std::vector<long long> grp_start_;  // per delay group: first connection
+  std::vector<int> grp_delay_;
+  // spike buffers
+  std::vector<int> sb_size_, sb_idx0_, sb_time_, sb_conn_;
+  std::vector<float> sb_height_;
+  std::vector<long long> last_spike_idx_;
+  // send list
+  std::vector<int> send_src_, send_grp_;
+  std::vector<float> send_h_;
+};
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C API used by the tests (ctypes).  Handles carry their precision.
+using namespace orc;
+
+struct OrcHandle {
+  int prec;  // 0 float, 1 double
+  Engine<float>* f;
+  Engine<double>* d;
+  std::string err;
+};
+
+template <class F>
+static int guard(OrcHandle* h, F fn) {
+  try {
+    fn();
+    return 0;
+  } catch (std::exception& e) {
+    h->err = e.what();
+    return -1;
+  }
+}
+
+#define DISPATCH(h, call) ((h)->prec == 0 ? (h)->f->call : (h)->d->call)
+
+extern "C" {
+
+void* orc_new(float dt, unsigned long long seed, int prec) {
+  OrcHandle* h = new OrcHandle{prec, nullptr, nullptr, ""};
+  if (prec == 0) h->f = new Engine<float>(dt, seed);
+  else h->d = new Engine<double>(dt, seed);
+  return h;
+}
+
+void orc_free(void* hv) {
+  OrcHandle* h = (OrcHandle*)hv;
+  delete h->f;
+  delete h->d;
+  delete h;
+}
+
+const char* orc_error(void* hv) { return ((OrcHandle*)hv)->err.c_str(); }
+
+int orc_create(void* hv, const char* model, int n, int n_port) {
+  OrcHandle* h = (OrcHandle*)hv;
+  int ret = -1;
+  if (guard(h, [&] { ret = DISPATCH(h, create(model, n, n_port)); }) < 0) return -1;
+  return ret;
+}
+
+int orc_set(void* hv, int node0, int n, const char* name, int port, const float* vals) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return guard(h, [&] { DISPATCH(h, set(node0, n, name, port, vals)); });
+}
+
+double orc_get(void* hv, int node, const char* name, int port) {
+  OrcHandle* h = (OrcHandle*)hv;
+  double v = NAN;
+  guard(h, [&] { v = DISPATCH(h, get(node, name, port)); });
+  return v;
+}
+
+int orc_set_spike_times(void* hv, int node, int n, const float* t, const float* hg) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return guard(h, [&] { DISPATCH(h, set_spike_times(node, n, t, hg)); });
+}
+
+int orc_connect(void* hv, long long n, const int* src, const int* tgt,
+                const unsigned char* port, const float* w, const float* delay) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return guard(h, [&] { DISPATCH(h, connect(n, src, tgt, port, w, delay)); });
+}
+
+int orc_record(void* hv, int node, const char* var, int port) {
+  OrcHandle* h = (OrcHandle*)hv;
+  int ret = -1;
+  if (guard(h, [&] { ret = DISPATCH(h, record(node, var, port)); }) < 0) return -1;
+  return ret;
+}
+
+int orc_simulate(void* hv, float sim_time) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return guard(h, [&] { DISPATCH(h, simulate(sim_time)); });
+}
+
+long long orc_record_rows(void* hv, int rec) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return h->prec == 0 ? h->f->records_.at(rec).t.size() : h->d->records_.at(rec).t.size();
+}
+
+void orc_record_copy(void* hv, int rec, float* t, float* v) {
+  OrcHandle* h = (OrcHandle*)hv;
+  const Record& r = h->prec == 0 ? h->f->records_.at(rec) : h->d->records_.at(rec);
+  std::copy(r.t.begin(), r.t.end(), t);
+  std::copy(r.v.begin(), r.v.end(), v);
+}
+
+long long orc_n_events(void* hv) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return h->prec == 0 ? h->f->ev_t_.size() : h->d->ev_t_.size();
+}
+
+void orc_events(void* hv, long long* t_idx, int* node) {
+  OrcHandle* h = (OrcHandle*)hv;
+  if (h->prec == 0) {
+    std::copy(h->f->ev_t_.begin(), h->f->ev_t_.end(), t_idx);
+    std::copy(h->f->ev_node_.begin(), h->f->ev_node_.end(), node);
+  } else {
+    std::copy(h->d->ev_t_.begin(), h->d->ev_t_.end(), t_idx);
+    std::copy(h->d->ev_node_.begin(), h->d->ev_node_.end(), node);
+  }
+}
+
+// stats[0]=attempts, [1]=rejected, [2]=accepted substeps, [3]=synaptic events
+void orc_stats(void* hv, long long* stats) {
+  OrcHandle* h = (OrcHandle*)hv;
+  const StepStats& s = h->prec == 0 ? h->f->stats_ : h->d->stats_;
+  stats[0] = s.attempts;
+  stats[1] = s.rejected;
+  stats[2] = s.substeps;
+  stats[3] = h->prec == 0 ? h->f->n_syn_events_ : h->d->n_syn_events_;
+}
+
+int orc_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// One RK5 step of y' = lambda*y from (x0, y0) with step h (h_min = h_max =
+// h so the step is forced).  out = {y1, err_estimate_scaled, h_next}.
+void orc_rk5_linear(int prec, double lambda, double y0, double h, double* out) {
+  if (prec == 0) {
+    float y = (float)y0, hh = (float)h, p = (float)lambda;
+    double x = 0.0;
+    rk5_step<float, LinearModel<float>>(0, 1, x, &y, hh, (float)h, (float)h, &p, nullptr);
+    out[0] = y; out[1] = x; out[2] = hh;
+  } else {
+    double y = y0, hh = h, p = lambda;
+    double x = 0.0;
+    rk5_step<double, LinearModel<double>>(0, 1, x, &y, hh, h, h, &p, nullptr);
+    out[0] = y; out[1] = x; out[2] = hh;
+  }
+}
+
+// Adaptive integration of y' = lambda*y over [0, x1] with the reference
+// controller (h_min, h0).  out = {y(x1), x_final, n_attempts, n_rejected}.
+void orc_rk5_linear_adaptive(int prec, double lambda, double y0, double x1,
+                             double h0, double h_min, double* out) {
+  StepStats st;
+  if (prec == 0) {
+    float y = (float)y0, h = (float)h0, p = (float)lambda;
+    double x = 0.0;
+    while (!(x >= x1 - (double)(float)h_min)) {
+      float hmax = (float)(x1 - x);
+      rk5_step<float, LinearModel<float>>(0, 1, x, &y, h, (float)h_min, hmax, &p, &st);
+    }
+    out[0] = y; out[1] = x;
+  } else {
+    double y = y0, h = h0, p = lambda;
+    double x = 0.0;
+    while (!(x >= x1 - h_min)) {
+      double hmax = x1 - x;
+      rk5_step<double, LinearModel<double>>(0, 1, x, &y, h, h_min, hmax, &p, &st);
+    }
+    out[0] = y; out[1] = x;
+  }
+  out[2] = st.attempts;
+  out[3] = st.rejected;
+}
+
+void orc_philox(unsigned int* ctr, unsigned int k0, unsigned int k1, unsigned int* out) {
+  Philox4 c{{ctr[0], ctr[1], ctr[2], ctr[3]}};
+  Philox4 r = philox4x32_10(c, k0, k1);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+int orc_poisson(unsigned long long seed, unsigned int stream, long long t_idx, double lambda) {
+  int m = poisson_chunks(lambda);
+  return poisson_draw(seed, stream, t_idx, m, m > 0 ? std::exp(-lambda / m) : 0.0);
+}
+
+double orc_propagator_32(double tau_syn, double tau, double C, double h) {
+  return propagator_32(tau_syn, tau, C, h);
+}
+
+}  // extern "C"

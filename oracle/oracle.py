@@ -1,0 +1,181 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (liboracle.so).
+
+The oracle is a host C++ restatement of the reference NEST GPU step (see
+oracle.cpp for the file:line map).  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg use it, as the checker; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the Makefile beside this file."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, ip, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+        L.orc_new.restype = vp
+        L.orc_new.argtypes = [ctypes.c_float, ctypes.c_ulonglong, ctypes.c_int]
+        L.orc_free.argtypes = [vp]
+        L.orc_error.restype = ctypes.c_char_p
+        L.orc_error.argtypes = [vp]
+        L.orc_create.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.orc_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, fp]
+        L.orc_get.restype = ctypes.c_double
+        L.orc_get.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.orc_set_spike_times.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, fp]
+        L.orc_connect.argtypes = [vp, ctypes.c_longlong, ip, ip,
+                                  ctypes.POINTER(ctypes.c_ubyte), fp, fp]
+        L.orc_record.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.orc_simulate.argtypes = [vp, ctypes.c_float]
+        L.orc_record_rows.restype = ctypes.c_longlong
+        L.orc_record_rows.argtypes = [vp, ctypes.c_int]
+        L.orc_record_copy.argtypes = [vp, ctypes.c_int, fp, fp]
+        L.orc_n_events.restype = ctypes.c_longlong
+        L.orc_n_events.argtypes = [vp]
+        L.orc_events.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong), ip]
+        L.orc_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        L.orc_rk5_linear.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.orc_rk5_linear_adaptive.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              ctypes.POINTER(ctypes.c_double)]
+        L.orc_philox.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.c_uint, ctypes.c_uint,
+                                 ctypes.POINTER(ctypes.c_uint)]
+        L.orc_poisson.restype = ctypes.c_int
+        L.orc_poisson.argtypes = [ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_longlong, ctypes.c_double]
+        L.orc_propagator_32.restype = ctypes.c_double
+        L.orc_propagator_32.argtypes = [ctypes.c_double] * 4
+        L.orc_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class Oracle:
+    """One oracle network.  prec=0: reference float path, prec=1: FP64."""
+
+    def __init__(self, dt=0.1, seed=54321 + 12345, prec=0):
+        self.L = lib()
+        self.h = self.L.orc_new(dt, seed, prec)
+
+    def __del__(self):
+        try:
+            self.L.orc_free(self.h)
+        except Exception:
+            pass
+
+    def _check(self, ret):
+        if ret < 0:
+            raise ValueError(self.L.orc_error(self.h).decode())
+        return ret
+
+    def create(self, model, n=1, n_port=1):
+        return self._check(self.L.orc_create(self.h, model.encode(), n, n_port))
+
+    def set(self, node0, n, name, vals, port=0):
+        v = _f32(np.broadcast_to(np.asarray(vals, dtype=np.float32), (n,)))
+        self._check(self.L.orc_set(self.h, node0, n, name.encode(), port,
+                                   v.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def get(self, node, name, port=0):
+        return self.L.orc_get(self.h, node, name.encode(), port)
+
+    def set_spike_times(self, node, times, heights=None):
+        t = _f32(times)
+        hp = None
+        if heights is not None:
+            hh = _f32(heights)
+            hp = hh.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        self._check(self.L.orc_set_spike_times(self.h, node, len(t),
+                                               t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), hp))
+
+    def connect(self, src, tgt, port, weight, delay):
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        n = len(src)
+        tgt = np.ascontiguousarray(np.broadcast_to(tgt, (n,)), dtype=np.int32)
+        port = np.ascontiguousarray(np.broadcast_to(port, (n,)), dtype=np.uint8)
+        w = _f32(np.broadcast_to(np.asarray(weight, dtype=np.float32), (n,)))
+        d = _f32(np.broadcast_to(np.asarray(delay, dtype=np.float32), (n,)))
+        self._check(self.L.orc_connect(
+            self.h, n, src.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            tgt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            port.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+            w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def record(self, node, var, port=0):
+        return self._check(self.L.orc_record(self.h, node, var.encode(), port))
+
+    def simulate(self, t):
+        self._check(self.L.orc_simulate(self.h, t))
+
+    def record_data(self, rec):
+        n = self.L.orc_record_rows(self.h, rec)
+        t = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        self.L.orc_record_copy(self.h, rec, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return t, v
+
+    def events(self):
+        """All emitted spikes as (time_index, node), sorted."""
+        n = self.L.orc_n_events(self.h)
+        t = np.zeros(n, np.int64)
+        nd = np.zeros(n, np.int32)
+        self.L.orc_events(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                          nd.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        o = np.lexsort((nd, t))
+        return t[o], nd[o]
+
+    def stats(self):
+        s = (ctypes.c_longlong * 4)()
+        self.L.orc_stats(self.h, s)
+        return {"attempts": s[0], "rejected": s[1], "substeps": s[2], "syn_events": s[3]}
+
+
+def rk5_linear(prec, lam, y0, h):
+    out = (ctypes.c_double * 3)()
+    lib().orc_rk5_linear(prec, lam, y0, h, out)
+    return out[0], out[1], out[2]
+
+
+def rk5_linear_adaptive(prec, lam, y0, x1, h0, h_min):
+    out = (ctypes.c_double * 4)()
+    lib().orc_rk5_linear_adaptive(prec, lam, y0, x1, h0, h_min, out)
+    return out[0], out[1], int(out[2]), int(out[3])
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint * 4)(*ctr)
+    o = (ctypes.c_uint * 4)()
+    lib().orc_philox(c, key[0], key[1], o)
+    return list(o)
+
+
+def poisson(seed, stream, t_idx, lam):
+    return lib().orc_poisson(seed, stream, t_idx, lam)
+
+
+def propagator_32(tau_syn, tau, C, h):
+    return lib().orc_propagator_32(tau_syn, tau, C, h)
+
+
+def threads():
+    return lib().orc_threads()
