@@ -1,0 +1,224 @@
+/*
+ * C ABI of the MI355X-native NEST GPU simulation step.
+ *
+ * Every function keeps the name, signature and error convention of the
+ * reference's C interface, src/nestgpu_C.h:34-353 (implementation
+ * src/nestgpu_C.cpp:37-1194), so existing ctypes/C callers bind unchanged:
+ *   - one process-global simulator, created on the first call;
+ *   - each call clears the error state; on failure NESTGPU_GetErrorCode()
+ *     returns 2 (library error), 1 (allocation error) or 255 (other) and
+ *     NESTGPU_GetErrorMessage() the message (src/propagate_error.h:4-30);
+ *   - with NESTGPU_SetOnException(0) (the default) a failing call prints the
+ *     message and exits with the error code, with 1 it returns normally;
+ *   - arrays returned by getters are owned by the library and stay valid
+ *     until the next call of the same getter.
+ * Functions of subsystems outside this build (MPI spike exchange, remote
+ * connections, synapse groups/STDP, external neurons) exist and report
+ * error 2 with a message saying so.
+ *
+ * Line numbers below cite the reference declaration each entry replaces.
+ */
+#ifndef NESTGPU_AMD_C_H
+#define NESTGPU_AMD_C_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors and kernel settings (src/nestgpu_C.h:34-52) ---- */
+char *NESTGPU_GetErrorMessage();                                   /* :34 */
+unsigned char NESTGPU_GetErrorCode();                              /* :36 */
+void NESTGPU_SetOnException(int on_exception);                     /* :38 */
+int NESTGPU_SetRandomSeed(unsigned long long seed);                /* :40 */
+int NESTGPU_SetTimeResolution(float time_res);                     /* :42 */
+float NESTGPU_GetTimeResolution();                                 /* :44 */
+int NESTGPU_SetMaxSpikeBufferSize(int max_size);                   /* :46 */
+int NESTGPU_GetMaxSpikeBufferSize();                               /* :48 */
+int NESTGPU_SetSimTime(float sim_time);                            /* :50 */
+int NESTGPU_SetVerbosityLevel(int verbosity_level);                /* :52 */
+
+/* ---- node creation and recording (src/nestgpu_C.h:54-68) ---- */
+int NESTGPU_Create(char *model_name, int n_neuron, int n_port);    /* :54 */
+int NESTGPU_CreatePoissonGenerator(int n_node, float rate);        /* :56 */
+int NESTGPU_CreateRecord(char *file_name, char *var_name_arr[],    /* :58 */
+                         int *i_node_arr, int *port_arr, int n_node);
+int NESTGPU_GetRecordDataRows(int i_record);                       /* :62 */
+int NESTGPU_GetRecordDataColumns(int i_record);                    /* :64 */
+float **NESTGPU_GetRecordData(int i_record);                       /* :66 */
+
+/* ---- parameters and variables (src/nestgpu_C.h:68-150) ---- */
+int NESTGPU_SetNeuronScalParam(int i_node, int n_neuron, char *param_name, float val);
+int NESTGPU_SetNeuronArrayParam(int i_node, int n_neuron, char *param_name,
+                                float *param, int array_size);
+int NESTGPU_SetNeuronPtScalParam(int *i_node, int n_neuron, char *param_name, float val);
+int NESTGPU_SetNeuronPtArrayParam(int *i_node, int n_neuron, char *param_name,
+                                  float *param, int array_size);
+int NESTGPU_IsNeuronScalParam(int i_node, char *param_name);
+int NESTGPU_IsNeuronPortParam(int i_node, char *param_name);
+int NESTGPU_IsNeuronArrayParam(int i_node, char *param_name);
+int NESTGPU_SetNeuronIntVar(int i_node, int n_neuron, char *var_name, int val);
+int NESTGPU_SetNeuronScalVar(int i_node, int n_neuron, char *var_name, float val);
+int NESTGPU_SetNeuronArrayVar(int i_node, int n_neuron, char *var_name,
+                              float *var, int array_size);
+int NESTGPU_SetNeuronPtIntVar(int *i_node, int n_neuron, char *var_name, int val);
+int NESTGPU_SetNeuronPtScalVar(int *i_node, int n_neuron, char *var_name, float val);
+int NESTGPU_SetNeuronPtArrayVar(int *i_node, int n_neuron, char *var_name,
+                                float *var, int array_size);
+int NESTGPU_IsNeuronIntVar(int i_node, char *var_name);
+int NESTGPU_IsNeuronScalVar(int i_node, char *var_name);
+int NESTGPU_IsNeuronPortVar(int i_node, char *var_name);
+int NESTGPU_IsNeuronArrayVar(int i_node, char *var_name);
+int NESTGPU_GetNeuronParamSize(int i_node, char *param_name);
+int NESTGPU_GetNeuronVarSize(int i_node, char *var_name);
+float *NESTGPU_GetNeuronParam(int i_node, int n_neuron, char *param_name);
+float *NESTGPU_GetNeuronPtParam(int *i_node, int n_neuron, char *param_name);
+float *NESTGPU_GetArrayParam(int i_node, char *param_name);
+int *NESTGPU_GetNeuronIntVar(int i_node, int n_neuron, char *param_name);
+int *NESTGPU_GetNeuronPtIntVar(int *i_node, int n_neuron, char *param_name);
+float *NESTGPU_GetNeuronVar(int i_node, int n_neuron, char *param_name);
+float *NESTGPU_GetNeuronPtVar(int *i_node, int n_neuron, char *param_name);
+float *NESTGPU_GetArrayVar(int i_node, char *var_name);
+
+/* ---- simulation (src/nestgpu_C.h:152-160) ---- */
+int NESTGPU_Calibrate();
+int NESTGPU_Simulate();
+int NESTGPU_StartSimulation();
+int NESTGPU_SimulationStep();
+int NESTGPU_EndSimulation();
+
+/* ---- MPI (src/nestgpu_C.h:162-170): not available in this build ---- */
+int NESTGPU_ConnectMpiInit(int argc, char *argv[]);
+int NESTGPU_MpiId();
+int NESTGPU_MpiNp();
+int NESTGPU_ProcMaster();
+int NESTGPU_MpiFinalize();
+
+/* ---- host random numbers (src/nestgpu_C.h:172-179) ---- */
+unsigned int *NESTGPU_RandomInt(size_t n);
+float *NESTGPU_RandomUniform(size_t n);
+float *NESTGPU_RandomNormal(size_t n, float mean, float stddev);
+float *NESTGPU_RandomNormalClipped(size_t n, float mean, float stddev,
+                                   float vmin, float vmax, float vstep);
+
+/* ---- connections (src/nestgpu_C.h:181-237) ---- */
+int NESTGPU_Connect(int i_source_node, int i_target_node, unsigned char port,
+                    unsigned char syn_group, float weight, float delay);
+int NESTGPU_ConnSpecInit();
+int NESTGPU_SetConnSpecParam(char *param_name, int value);
+int NESTGPU_ConnSpecIsParam(char *param_name);
+int NESTGPU_SynSpecInit();
+int NESTGPU_SetSynSpecIntParam(char *param_name, int value);
+int NESTGPU_SetSynSpecFloatParam(char *param_name, float value);
+int NESTGPU_SetSynSpecFloatPtParam(char *param_name, float *array_pt);
+int NESTGPU_SynSpecIsIntParam(char *param_name);
+int NESTGPU_SynSpecIsFloatParam(char *param_name);
+int NESTGPU_SynSpecIsFloatPtParam(char *param_name);
+int NESTGPU_ConnectSeqSeq(int i_source, int n_source, int i_target, int n_target);
+int NESTGPU_ConnectSeqGroup(int i_source, int n_source, int *i_target, int n_target);
+int NESTGPU_ConnectGroupSeq(int *i_source, int n_source, int i_target, int n_target);
+int NESTGPU_ConnectGroupGroup(int *i_source, int n_source, int *i_target, int n_target);
+int NESTGPU_RemoteConnectSeqSeq(int i_source_host, int i_source, int n_source,
+                                int i_target_host, int i_target, int n_target);
+int NESTGPU_RemoteConnectSeqGroup(int i_source_host, int i_source, int n_source,
+                                  int i_target_host, int *i_target, int n_target);
+int NESTGPU_RemoteConnectGroupSeq(int i_source_host, int *i_source, int n_source,
+                                  int i_target_host, int i_target, int n_target);
+int NESTGPU_RemoteConnectGroupGroup(int i_source_host, int *i_source, int n_source,
+                                    int i_target_host, int *i_target, int n_target);
+
+/* ---- name lists (src/nestgpu_C.h:239-269) ---- */
+char **NESTGPU_GetIntVarNames(int i_node);
+char **NESTGPU_GetScalVarNames(int i_node);
+int NESTGPU_GetNIntVar(int i_node);
+int NESTGPU_GetNScalVar(int i_node);
+char **NESTGPU_GetPortVarNames(int i_node);
+int NESTGPU_GetNPortVar(int i_node);
+char **NESTGPU_GetScalParamNames(int i_node);
+int NESTGPU_GetNScalParam(int i_node);
+char **NESTGPU_GetPortParamNames(int i_node);
+int NESTGPU_GetNGroupParam(int i_node);
+char **NESTGPU_GetGroupParamNames(int i_node);
+int NESTGPU_GetNPortParam(int i_node);
+char **NESTGPU_GetArrayParamNames(int i_node);
+int NESTGPU_GetNArrayParam(int i_node);
+char **NESTGPU_GetArrayVarNames(int i_node);
+int NESTGPU_GetNArrayVar(int i_node);
+
+/* ---- connection queries (src/nestgpu_C.h:271-290) ---- */
+int *NESTGPU_GetSeqSeqConnections(int i_source, int n_source, int i_target,
+                                  int n_target, int syn_group, int *n_conn);
+int *NESTGPU_GetSeqGroupConnections(int i_source, int n_source, int *i_target,
+                                    int n_target, int syn_group, int *n_conn);
+int *NESTGPU_GetGroupSeqConnections(int *i_source, int n_source, int i_target,
+                                    int n_target, int syn_group, int *n_conn);
+int *NESTGPU_GetGroupGroupConnections(int *i_source, int n_source, int *i_target,
+                                      int n_target, int syn_group, int *n_conn);
+int NESTGPU_GetConnectionStatus(int i_source, int i_group, int i_conn, int *i_target,
+                                unsigned char *port, unsigned char *syn_group,
+                                float *delay, float *weight);
+
+/* ---- synapse groups (src/nestgpu_C.h:292-306): outside this build ---- */
+int NESTGPU_CreateSynGroup(char *model_name);
+int NESTGPU_GetSynGroupNParam(int i_syn_group);
+char **NESTGPU_GetSynGroupParamNames(int i_syn_group);
+int NESTGPU_IsSynGroupParam(int i_syn_group, char *param_name);
+int NESTGPU_GetSynGroupParamIdx(int i_syn_group, char *param_name);
+float NESTGPU_GetSynGroupParam(int i_syn_group, char *param_name);
+int NESTGPU_SetSynGroupParam(int i_syn_group, char *param_name, float val);
+
+/* ---- spike counting / spike-time recording (src/nestgpu_C.h:308-320) ---- */
+int NESTGPU_ActivateSpikeCount(int i_node, int n_node);
+int NESTGPU_ActivateRecSpikeTimes(int i_node, int n_node, int max_n_rec_spike_times);
+int NESTGPU_SetRecSpikeTimesStep(int i_node, int n_node, int rec_spike_times_step);
+int NESTGPU_GetNRecSpikeTimes(int i_node);
+int NESTGPU_GetRecSpikeTimes(int i_node, int n_node, int **n_spike_times_pt,
+                             float ***spike_times_pt);
+
+/* ---- external spikes (src/nestgpu_C.h:322-326): outside this build ---- */
+int NESTGPU_PushSpikesToNodes(int n_spikes, int *node_id);
+int NESTGPU_GetExtNeuronInputSpikes(int *n_spikes, int **node, int **port,
+                                    float **spike_height, int include_zeros);
+
+/* ---- group parameters and kernel parameters (src/nestgpu_C.h:328-350) ---- */
+int NESTGPU_SetNeuronGroupParam(int i_node, int n_node, char *param_name, float val);
+int NESTGPU_IsNeuronGroupParam(int i_node, char *param_name);
+float NESTGPU_GetNeuronGroupParam(int i_node, char *param_name);
+int NESTGPU_GetNBoolParam();
+char **NESTGPU_GetBoolParamNames();
+int NESTGPU_IsBoolParam(char *param_name);
+int NESTGPU_GetBoolParamIdx(char *param_name);
+bool NESTGPU_GetBoolParam(char *param_name);
+int NESTGPU_SetBoolParam(char *param_name, bool val);
+int NESTGPU_GetNFloatParam();
+char **NESTGPU_GetFloatParamNames();
+int NESTGPU_IsFloatParam(char *param_name);
+int NESTGPU_GetFloatParamIdx(char *param_name);
+float NESTGPU_GetFloatParam(char *param_name);
+int NESTGPU_SetFloatParam(char *param_name, float val);
+int NESTGPU_GetNIntParam();
+char **NESTGPU_GetIntParamNames();
+int NESTGPU_IsIntParam(char *param_name);
+int NESTGPU_GetIntParamIdx(char *param_name);
+int NESTGPU_GetIntParam(char *param_name);
+int NESTGPU_SetIntParam(char *param_name, int val);
+int NESTGPU_RemoteCreate(int i_host, char *model_name, int n_neuron, int n_port);
+
+/* ---- instrumentation of this build (no counterpart in the reference) ----
+ * out[0..4] = RK5 attempts, rejected attempts, accepted substeps, delivered
+ * (spike, synapse) events, spikes dropped by a full spike list; all since
+ * calibration.  Returns 0, or -1 before calibration. */
+int NESTGPU_AMD_GetStepStats(unsigned long long *out);
+/* Destroys the simulator (all nodes, connections, records, device memory);
+ * the next call starts a new one with default settings. */
+int NESTGPU_AMD_ResetKernel();
+/* Blocks until all queued device work of the simulator is done. */
+int NESTGPU_AMD_Synchronize();
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

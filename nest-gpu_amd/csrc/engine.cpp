@@ -1,0 +1,1701 @@
+// Host engine of the MI355X simulation step (see engine.h).
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "philox.h"
+
+namespace ngpu {
+
+#define HIPCHK(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw Error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " \
+                  + #call);                                                   \
+  } while (0)
+
+static double wall_time() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+void DBuf<T>::alloc(size_t count) {
+  free();
+  n = count;
+  if (count == 0) return;
+  HIPCHK(hipMalloc((void**)&p, count * sizeof(T)));
+}
+
+template <class T>
+void DBuf<T>::free() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+template <class T>
+static void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+  d.alloc(h.size());
+  if (!h.empty()) {
+    HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Model name tables.  Names and their order are the reference's interface
+// (src/*_kernel.h enums and name arrays) so Set/GetStatus indices match.
+static const std::vector<std::string> kAeifScal = {
+    "V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
+    "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"};
+
+static const std::vector<ModelDesc>& model_table() {
+  static const std::vector<ModelDesc> t = {
+      {"aeif_cond_beta_multisynapse", MODEL_AEIF_COND_BETA_MS, {"V_m", "w"}, {"g", "g1"},
+       kAeifScal, {"E_rev", "tau_rise", "tau_decay", "g0"}, {"h_min_rel", "h0_rel"}, {},
+       -1, true, 1, true, 3},
+      {"aeif_cond_alpha_multisynapse", MODEL_AEIF_COND_ALPHA_MS, {"V_m", "w"}, {"g", "g1"},
+       kAeifScal, {"E_rev", "tau_syn", "g0"}, {"h_min_rel", "h0_rel"}, {},
+       -1, true, 1, true, 2},
+      {"aeif_psc_exp", MODEL_AEIF_PSC_EXP, {"I_syn_ex", "I_syn_in", "V_m", "w"}, {},
+       {"tau_syn_ex", "tau_syn_in", "V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b",
+        "tau_w", "I_e", "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"},
+       {}, {"h_min_rel", "h0_rel"}, {}, 2, true, 0, false, -1},
+      {"izhikevich_cond_beta", MODEL_IZH_COND_BETA, {"V_m", "u"}, {"g", "g1"},
+       {"V_th", "a", "b", "c", "d", "I_e", "t_ref", "refractory_step", "den_delay"},
+       {"E_rev", "tau_rise", "tau_decay", "g0"}, {"h_min_rel", "h0_rel"}, {},
+       -1, true, 1, true, 3},
+      {"iaf_psc_exp", MODEL_IAF_PSC_EXP, {"I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"},
+       {},
+       {"tau_m", "C_m", "E_L", "I_e", "Theta_rel", "V_reset_rel", "tau_ex", "tau_in",
+        "t_ref", "den_delay", "P20", "P11ex", "P11in", "P21ex", "P21in", "P22"},
+       {}, {}, {}, 2, false, 0, false, -1},
+      {"poisson_generator", MODEL_POISSON_GEN, {}, {}, {"rate", "origin", "start", "stop"},
+       {}, {}, {}, 0, false, -1, false, -1},
+      {"spike_generator", MODEL_SPIKE_GEN, {}, {}, {}, {}, {}, {"spike_times", "spike_heights"},
+       0, false, -1, false, -1},
+      {"spike_detector", MODEL_SPIKE_DETECTOR, {"input_spike_height", "spike_height"}, {},
+       {"hold_spike_height"}, {}, {}, {}, 1, false, 0, false, -1},
+  };
+  return t;
+}
+
+// Reference model names outside this build's scope (src/neuron_models.h:60-86).
+static const char* kOtherModels[] = {
+    "iaf_psc_exp_g", "iaf_psc_exp_hc", "iaf_psc_alpha", "ext_neuron", "aeif_cond_alpha",
+    "aeif_cond_beta", "aeif_psc_alpha", "aeif_psc_delta", "aeif_psc_exp_multisynapse",
+    "aeif_psc_alpha_multisynapse", "parrot_neuron", "izhikevich", "izhikevich_psc_exp_5s",
+    "izhikevich_psc_exp_2s", "izhikevich_psc_exp", "user_m1", "user_m2"};
+
+const ModelDesc* find_model(const std::string& name) {
+  for (auto& m : model_table())
+    if (name == m.name) return &m;
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side staging: before calibration a population's state lives in host
+// vectors (no GPU needed); calibration uploads it and from then on the
+// device columns are authoritative.
+Engine::Engine() { start_real_time_ = wall_time(); }
+
+Engine::~Engine() {
+  for (auto* g : groups_) delete g;
+  for (auto& r : records_)
+    if (r.fp) fclose(r.fp);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::check_uncalibrated(const char* msg) const {
+  if (calibrated_) throw Error(msg);
+}
+
+int Engine::SetRandomSeed(unsigned long long seed) {
+  seed_ = seed;
+  kernel_seed_ = seed + 12345ull;   // src/nestgpu.cu:184
+  rng_ctr_ = 0;
+  return 0;
+}
+
+int Engine::SetTimeResolution(float dt) {
+  dt_ = dt;
+  return 0;
+}
+
+// ---- kernel parameters ------------------------------------------------------
+std::vector<std::string> Engine::FloatParamNames() const {
+  return {"time_resolution", "max_spike_num_fact", "max_spike_per_host_fact"};
+}
+std::vector<std::string> Engine::IntParamNames() const {
+  return {"rnd_seed", "verbosity_level", "max_spike_buffer_size", "remote_spike_height_flag"};
+}
+std::vector<std::string> Engine::BoolParamNames() const { return {"print_time"}; }
+
+float Engine::GetFloatParam(const std::string& n) const {
+  if (n == "time_resolution") return dt_;
+  if (n == "max_spike_num_fact") return max_spike_num_fact_;
+  if (n == "max_spike_per_host_fact") return max_spike_per_host_fact_;
+  throw Error("Unrecognized kernel float parameter " + n);
+}
+int Engine::SetFloatParam(const std::string& n, float v) {
+  if (n == "time_resolution") return SetTimeResolution(v);
+  if (n == "max_spike_num_fact") { max_spike_num_fact_ = v; return 0; }
+  if (n == "max_spike_per_host_fact") { max_spike_per_host_fact_ = v; return 0; }
+  throw Error("Unrecognized kernel float parameter " + n);
+}
+int Engine::GetIntParam(const std::string& n) const {
+  if (n == "rnd_seed") return (int)seed_;
+  if (n == "verbosity_level") return verbosity_;
+  if (n == "max_spike_buffer_size") return max_spike_buffer_size_;
+  if (n == "remote_spike_height_flag") return remote_spike_height_;
+  throw Error("Unrecognized kernel int parameter " + n);
+}
+int Engine::SetIntParam(const std::string& n, int v) {
+  if (n == "rnd_seed") return SetRandomSeed((unsigned long long)v);
+  if (n == "verbosity_level") {
+    if (v < 0) throw Error("Verbosity level must be >= 0");
+    verbosity_ = v;
+    return 0;
+  }
+  if (n == "max_spike_buffer_size") { max_spike_buffer_size_ = v; return 0; }
+  if (n == "remote_spike_height_flag") {
+    if (v != 0) throw Error("remote spikes (MPI) are not supported in this build");
+    return 0;
+  }
+  throw Error("Unrecognized kernel int parameter " + n);
+}
+bool Engine::GetBoolParam(const std::string& n) const {
+  if (n == "print_time") return print_time_;
+  throw Error("Unrecognized kernel boolean parameter " + n);
+}
+int Engine::SetBoolParam(const std::string& n, bool v) {
+  if (n == "print_time") { print_time_ = v; return 0; }
+  throw Error("Unrecognized kernel boolean parameter " + n);
+}
+
+// ---- nodes ------------------------------------------------------------------
+int Engine::param_idx(const Group& g, const std::string& name, int port) const {
+  const ModelDesc& m = *g.md;
+  for (size_t i = 0; i < m.scal_param.size(); ++i)
+    if (m.scal_param[i] == name) return (int)i;
+  for (size_t i = 0; i < m.port_param.size(); ++i)
+    if (m.port_param[i] == name) return (int)(m.scal_param.size() + port * m.port_param.size() + i);
+  return -1;
+}
+
+int Engine::var_idx(const Group& g, const std::string& name, int port) const {
+  const ModelDesc& m = *g.md;
+  for (size_t i = 0; i < m.scal_var.size(); ++i)
+    if (m.scal_var[i] == name) return (int)i;
+  for (size_t i = 0; i < m.port_var.size(); ++i)
+    if (m.port_var[i] == name) return (int)(m.scal_var.size() + port * m.port_var.size() + i);
+  return -1;
+}
+
+static int group_index(const std::vector<Group*>& gs, const Group* g) {
+  for (size_t k = 0; k < gs.size(); ++k)
+    if (gs[k] == g) return (int)k;
+  return -1;
+}
+
+int Engine::Create(const std::string& model, int n, int n_port) {
+  check_uncalibrated("Nodes cannot be created after calibration");
+  if (n <= 0) throw Error("Number of nodes must be greater than zero.");
+  if (n_port < 0) throw Error("Number of ports must be >= zero.");
+  const ModelDesc* md = find_model(model);
+  if (!md) {
+    for (const char* o : kOtherModels)
+      if (model == o)
+        throw Error("Neuron model " + model + " is outside the scope of this build");
+    throw Error("Unknown neuron model name: " + model);
+  }
+  if (md->forced_port >= 0) n_port = md->forced_port;
+  if (md->rk5 && md->forced_port < 0 && (n_port < 1 || n_port > kMaxPort))
+    throw Error("Number of ports of " + model + " must be between 1 and " +
+                std::to_string(kMaxPort) + " in this build");
+  if ((long long)n_nodes_ + n > (1ll << 24))
+    throw Error("Maximum number of nodes (16777216) exceeded");
+  Group* g = new Group;
+  g->md = md;
+  g->i0 = n_nodes_;
+  g->n = n;
+  g->n_port = n_port;
+  g->nv = (int)(md->scal_var.size() + md->port_var.size() * n_port);
+  g->np = (int)(md->scal_param.size() + md->port_param.size() * n_port);
+  g->uni.assign(g->np, 1);
+  g->uni_val.assign(g->np, 0.f);
+  std::vector<float> var((size_t)g->nv * n, 0.f);
+  std::vector<float> dflt_var(g->nv, 0.f);
+  auto P = [&](const char* nm, float v, int port = 0) { g->uni_val[param_idx(*g, nm, port)] = v; };
+  auto V = [&](const char* nm, float v, int port = 0) { dflt_var[var_idx(*g, nm, port)] = v; };
+  switch (md->id) {  // defaults of NodeInit / Init in the reference model files
+    case MODEL_AEIF_COND_BETA_MS:
+    case MODEL_AEIF_COND_ALPHA_MS:
+    case MODEL_AEIF_PSC_EXP:
+      P("V_th", -50.4f); P("Delta_T", 2.0f); P("g_L", 30.0f); P("E_L", -70.6f);
+      P("C_m", 281.0f); P("a", 4.0f); P("b", 80.5f); P("tau_w", 144.0f); P("I_e", 0.f);
+      P("V_peak", 0.f); P("V_reset", -60.0f); P("t_ref", 0.f); P("den_delay", 0.f);
+      P("refractory_step", 0.f);
+      V("V_m", -70.6f); V("w", 0.f);
+      if (md->id == MODEL_AEIF_PSC_EXP) {
+        P("tau_syn_ex", 0.2f); P("tau_syn_in", 2.0f);
+      } else {
+        for (int q = 0; q < n_port; ++q) {
+          P("E_rev", 0.f, q);
+          if (md->id == MODEL_AEIF_COND_BETA_MS) { P("tau_decay", 20.f, q); P("tau_rise", 2.f, q); }
+          else P("tau_syn", 2.f, q);
+        }
+      }
+      break;
+    case MODEL_IZH_COND_BETA:
+      P("V_th", 30.f); P("a", 0.02f); P("b", 0.2f); P("c", -65.f); P("d", 8.f); P("I_e", 0.f);
+      P("t_ref", 0.f); P("den_delay", 0.f); P("refractory_step", 0.f);
+      V("V_m", -70.f); V("u", 0.2f * -70.f);
+      for (int q = 0; q < n_port; ++q) { P("E_rev", 0.f, q); P("tau_decay", 20.f, q); P("tau_rise", 2.f, q); }
+      break;
+    case MODEL_IAF_PSC_EXP:
+      P("tau_m", 10.f); P("C_m", 250.f); P("E_L", -70.f); P("I_e", 0.f);
+      P("Theta_rel", -55.0f - (-70.0f)); P("V_reset_rel", 0.f); P("tau_ex", 2.f); P("tau_in", 2.f);
+      P("t_ref", 2.f); P("den_delay", 0.f);
+      break;
+    case MODEL_POISSON_GEN:
+      P("rate", 0.f); P("origin", 0.f); P("start", 0.f); P("stop", 1.0e30f);
+      break;
+    case MODEL_SPIKE_DETECTOR:
+      P("hold_spike_height", 1.f);
+      break;
+    default:
+      break;
+  }
+  for (int j = 0; j < g->nv; ++j)
+    std::fill(var.begin() + (size_t)j * n, var.begin() + (size_t)(j + 1) * n, dflt_var[j]);
+  std::vector<float> par((size_t)g->np * n, 0.f);
+  for (int j = 0; j < g->np; ++j)
+    std::fill(par.begin() + (size_t)j * n, par.begin() + (size_t)(j + 1) * n, g->uni_val[j]);
+  if (md->rk5) {  // refractory_step is per-neuron mutable state
+    int r = param_idx(*g, "refractory_step", 0);
+    g->uni[r] = 0;
+  }
+  if (md->id == MODEL_SPIKE_GEN) {
+    g->sg_times.assign(n, {});
+    g->sg_heights.assign(n, {});
+  }
+  groups_.push_back(g);
+  stage_var_.push_back(std::move(var));
+  stage_param_.push_back(std::move(par));
+  node_group_.insert(node_group_.end(), n, (int)groups_.size() - 1);
+  n_nodes_ += n;
+  return g->i0;
+}
+
+Group& Engine::GroupOf(int node, int n_node, int* rel) {
+  if (node < 0 || n_node < 1 || node + n_node > n_nodes_)
+    throw Error("Unrecognized node in getting node sequence offset");
+  int gi = node_group_[node];
+  if (node_group_[node + n_node - 1] != gi)
+    throw Error("Nodes belong to different node groups in setting parameter");
+  *rel = node - groups_[gi]->i0;
+  return *groups_[gi];
+}
+
+std::vector<int> Engine::NodeListRel(const int* nodes, int n, Group** g) {
+  if (n < 1) throw Error("Empty node list");
+  int in0 = nodes[0];
+  if (in0 < 0 || in0 >= n_nodes_) throw Error("Unrecognized node in setting parameter");
+  int gi = node_group_[in0];
+  std::vector<int> rel(n);
+  for (int k = 0; k < n; ++k) {
+    int in = nodes[k];
+    if (in < 0 || in >= n_nodes_) throw Error("Unrecognized node in setting parameter");
+    if (node_group_[in] != gi) throw Error("Nodes belong to different node groups in setting parameter");
+    rel[k] = in - groups_[gi]->i0;
+  }
+  *g = groups_[gi];
+  return rel;
+}
+
+static Group* group_at(const std::vector<Group*>& gs, const std::vector<int>& ng, int node, int n_nodes) {
+  if (node < 0 || node >= n_nodes) throw Error("Unrecognized node");
+  return gs[ng[node]];
+}
+
+bool Engine::IsScalParam(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->scal_param;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsPortParam(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->port_param;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsArrayParam(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->array_param;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsGroupParam(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->group_param;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsIntVar(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  return std::find(g->int_var.begin(), g->int_var.end(), n) != g->int_var.end();
+}
+bool Engine::IsScalVar(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->scal_var;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsPortVar(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  auto& v = g->md->port_var;
+  return std::find(v.begin(), v.end(), n) != v.end();
+}
+bool Engine::IsArrayVar(int, const std::string&) { return false; }
+
+int Engine::GetParamSize(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  if (IsArrayParam(node, n)) {
+    int rel = node - g->i0;
+    if (n == "spike_times") return (int)g->sg_times[rel].size();
+    return (int)g->sg_heights[rel].size();
+  }
+  if (IsScalParam(node, n)) return 1;
+  if (IsPortParam(node, n)) return g->n_port;
+  throw Error("Unrecognized parameter " + n);
+}
+
+int Engine::GetVarSize(int node, const std::string& n) {
+  Group* g = group_at(groups_, node_group_, node, n_nodes_);
+  if (IsIntVar(node, n) || IsScalVar(node, n)) return 1;
+  if (IsPortVar(node, n)) return g->n_port;
+  throw Error("Unrecognized variable " + n);
+}
+
+// Column j of a population: host staging before calibration, device after.
+static void col_read(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev, int j,
+                     int rel0, int cnt, float* out) {
+  if (!cal) {
+    std::memcpy(out, stage.data() + (size_t)j * g.n + rel0, cnt * sizeof(float));
+  } else {
+    HIPCHK(hipMemcpy(out, dev.p + (size_t)j * g.n + rel0, cnt * sizeof(float), hipMemcpyDeviceToHost));
+  }
+}
+static void col_write(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev, int j,
+                      int rel0, int cnt, const float* in) {
+  if (!cal) {
+    std::memcpy(stage.data() + (size_t)j * g.n + rel0, in, cnt * sizeof(float));
+  } else {
+    HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n + rel0, in, cnt * sizeof(float), hipMemcpyHostToDevice));
+  }
+}
+
+void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const float* vals,
+                             bool whole) {
+  int gi = group_index(groups_, &g);
+  auto& stage = stage_param_[gi];
+  if (whole) {
+    std::vector<float> col(g.n, vals[0]);
+    col_write(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
+    if (!(g.md->rk5 && j == param_idx(g, "refractory_step", 0))) {
+      g.uni[j] = 1;
+      g.uni_val[j] = vals[0];
+    }
+    return;
+  }
+  std::vector<float> col(g.n);
+  col_read(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
+  for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
+  col_write(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
+  bool same = std::all_of(col.begin(), col.end(), [&](float v) { return v == col[0]; });
+  if (same && !(g.md->rk5 && j == param_idx(g, "refractory_step", 0))) {
+    g.uni[j] = 1;
+    g.uni_val[j] = col[0];
+  } else {
+    g.uni[j] = 0;
+  }
+}
+
+void Engine::set_var_index(Group& g, int j, const std::vector<int>& rel, const float* vals) {
+  int gi = group_index(groups_, &g);
+  auto& stage = stage_var_[gi];
+  std::vector<float> col(g.n);
+  col_read(calibrated_, g, stage, g.var, j, 0, g.n, col.data());
+  for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
+  col_write(calibrated_, g, stage, g.var, j, 0, g.n, col.data());
+}
+
+static std::vector<int> iota_rel(int rel0, int n) {
+  std::vector<int> r(n);
+  std::iota(r.begin(), r.end(), rel0);
+  return r;
+}
+
+void Engine::SetParam(int i_node, int n, const std::string& name, float val) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  auto& sp = g.md->scal_param;
+  if (std::find(sp.begin(), sp.end(), name) == sp.end())
+    throw Error("Unrecognized scalar parameter " + name);
+  int j = param_idx(g, name, 0);
+  std::vector<float> v(n, val);
+  set_param_index(g, j, iota_rel(rel0, n), v.data(), rel0 == 0 && n == g.n);
+}
+
+void Engine::SetParam(const int* nodes, int n, const std::string& name, float val) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  auto& sp = g->md->scal_param;
+  if (std::find(sp.begin(), sp.end(), name) == sp.end())
+    throw Error("Unrecognized scalar parameter " + name);
+  std::vector<float> v(n, val);
+  set_param_index(*g, param_idx(*g, name, 0), rel, v.data(), false);
+}
+
+void Engine::SetParamArr(int i_node, int n, const std::string& name, const float* v, int size) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  auto& pp = g.md->port_param;
+  if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
+    if (size != g.n_port) throw Error("Parameter array size must be equal to the number of ports.");
+    for (int q = 0; q < size; ++q) {
+      std::vector<float> vv(n, v[q]);
+      set_param_index(g, param_idx(g, name, q), iota_rel(rel0, n), vv.data(), rel0 == 0 && n == g.n);
+    }
+    return;
+  }
+  auto& ap = g.md->array_param;
+  if (std::find(ap.begin(), ap.end(), name) != ap.end()) {
+    check_uncalibrated("Spike generator arrays must be set before calibration");
+    for (int k = rel0; k < rel0 + n; ++k) {
+      if (name == "spike_times") g.sg_times[k].assign(v, v + size);
+      else g.sg_heights[k].assign(v, v + size);
+    }
+    return;
+  }
+  throw Error("Unrecognized array parameter " + name);
+}
+
+void Engine::SetParamArr(const int* nodes, int n, const std::string& name, const float* v, int size) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  auto& pp = g->md->port_param;
+  if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
+    if (size != g->n_port) throw Error("Parameter array size must be equal to the number of ports.");
+    for (int q = 0; q < size; ++q) {
+      std::vector<float> vv(n, v[q]);
+      set_param_index(*g, param_idx(*g, name, q), rel, vv.data(), false);
+    }
+    return;
+  }
+  auto& ap = g->md->array_param;
+  if (std::find(ap.begin(), ap.end(), name) != ap.end()) {
+    check_uncalibrated("Spike generator arrays must be set before calibration");
+    for (int k : rel) {
+      if (name == "spike_times") g->sg_times[k].assign(v, v + size);
+      else g->sg_heights[k].assign(v, v + size);
+    }
+    return;
+  }
+  throw Error("Unrecognized array parameter " + name);
+}
+
+void Engine::SetVar(int i_node, int n, const std::string& name, float val) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  auto& sv = g.md->scal_var;
+  if (std::find(sv.begin(), sv.end(), name) == sv.end())
+    throw Error("Unrecognized scalar variable " + name);
+  std::vector<float> v(n, val);
+  set_var_index(g, var_idx(g, name, 0), iota_rel(rel0, n), v.data());
+}
+
+void Engine::SetVar(const int* nodes, int n, const std::string& name, float val) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  auto& sv = g->md->scal_var;
+  if (std::find(sv.begin(), sv.end(), name) == sv.end())
+    throw Error("Unrecognized scalar variable " + name);
+  std::vector<float> v(n, val);
+  set_var_index(*g, var_idx(*g, name, 0), rel, v.data());
+}
+
+void Engine::SetVarArr(int i_node, int n, const std::string& name, const float* v, int size) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  auto& pv = g.md->port_var;
+  if (std::find(pv.begin(), pv.end(), name) == pv.end())
+    throw Error("Unrecognized port variable " + name);
+  if (size != g.n_port) throw Error("Variable array size must be equal to the number of ports.");
+  for (int q = 0; q < size; ++q) {
+    std::vector<float> vv(n, v[q]);
+    set_var_index(g, var_idx(g, name, q), iota_rel(rel0, n), vv.data());
+  }
+}
+
+void Engine::SetVarArr(const int* nodes, int n, const std::string& name, const float* v, int size) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  auto& pv = g->md->port_var;
+  if (std::find(pv.begin(), pv.end(), name) == pv.end())
+    throw Error("Unrecognized port variable " + name);
+  if (size != g->n_port) throw Error("Variable array size must be equal to the number of ports.");
+  for (int q = 0; q < size; ++q) {
+    std::vector<float> vv(n, v[q]);
+    set_var_index(*g, var_idx(*g, name, q), rel, vv.data());
+  }
+}
+
+void Engine::SetIntVar(int i_node, int n, const std::string& name, int val) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (std::find(g.int_var.begin(), g.int_var.end(), name) == g.int_var.end())
+    throw Error("Unrecognized integer variable " + name);
+  if (!calibrated_) return;  // counters are created zeroed at calibration
+  DBuf<int>& d = (name == "spike_count") ? g.spike_count : g.n_rec;
+  std::vector<int> v(n, val);
+  HIPCHK(hipMemcpy(d.p + rel0, v.data(), n * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void Engine::SetIntVar(const int* nodes, int n, const std::string& name, int val) {
+  for (int k = 0; k < n; ++k) SetIntVar(nodes[k], 1, name, val);
+}
+
+// Returned arrays stay valid until the next call of the same getter.
+static std::vector<float>& ret_float(int which) {
+  static std::vector<float> b[4];
+  return b[which];
+}
+static std::vector<int>& ret_int() {
+  static std::vector<int> b;
+  return b;
+}
+
+float* Engine::GetParam(int i_node, int n, const std::string& name) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  int gi = group_index(groups_, &g);
+  auto& sp = g.md->scal_param;
+  auto& pp = g.md->port_param;
+  std::vector<float>& out = ret_float(0);
+  if (std::find(sp.begin(), sp.end(), name) != sp.end()) {
+    out.resize(n);
+    col_read(calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, 0), rel0, n, out.data());
+    return out.data();
+  }
+  if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
+    out.assign((size_t)n * g.n_port, 0.f);
+    std::vector<float> col(n);
+    for (int q = 0; q < g.n_port; ++q) {
+      col_read(calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, q), rel0, n, col.data());
+      for (int k = 0; k < n; ++k) out[(size_t)k * g.n_port + q] = col[k];
+    }
+    return out.data();
+  }
+  auto& ap = g.md->array_param;
+  if (std::find(ap.begin(), ap.end(), name) != ap.end()) {
+    if (n != 1) throw Error("Cannot get array parameters for more than one node at a time");
+    return GetArrayParam(i_node, name);
+  }
+  throw Error("Unrecognized parameter " + name);
+}
+
+float* Engine::GetParam(const int* nodes, int n, const std::string& name) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  auto& sp = g->md->scal_param;
+  auto& pp = g->md->port_param;
+  int gi = group_index(groups_, g);
+  std::vector<float> col(g->n);
+  std::vector<float>& out = ret_float(0);
+  if (std::find(sp.begin(), sp.end(), name) != sp.end()) {
+    col_read(calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, 0), 0, g->n, col.data());
+    out.resize(n);
+    for (int k = 0; k < n; ++k) out[k] = col[rel[k]];
+    return out.data();
+  }
+  if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
+    out.assign((size_t)n * g->n_port, 0.f);
+    for (int q = 0; q < g->n_port; ++q) {
+      col_read(calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, q), 0, g->n, col.data());
+      for (int k = 0; k < n; ++k) out[(size_t)k * g->n_port + q] = col[rel[k]];
+    }
+    return out.data();
+  }
+  auto& ap = g->md->array_param;
+  if (std::find(ap.begin(), ap.end(), name) != ap.end()) {
+    if (n != 1) throw Error("Cannot get array parameters for more than one node at a time");
+    return GetArrayParam(nodes[0], name);
+  }
+  throw Error("Unrecognized parameter " + name);
+}
+
+float* Engine::GetArrayParam(int node, const std::string& name) {
+  int rel;
+  Group& g = GroupOf(node, 1, &rel);
+  if (name == "spike_times" && g.md->id == MODEL_SPIKE_GEN) return g.sg_times[rel].data();
+  if (name == "spike_heights" && g.md->id == MODEL_SPIKE_GEN) return g.sg_heights[rel].data();
+  throw Error("Unrecognized parameter " + name);
+}
+
+float* Engine::GetVar(int i_node, int n, const std::string& name) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  int gi = group_index(groups_, &g);
+  auto& sv = g.md->scal_var;
+  auto& pv = g.md->port_var;
+  std::vector<float>& out = ret_float(1);
+  if (std::find(sv.begin(), sv.end(), name) != sv.end()) {
+    out.resize(n);
+    col_read(calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, 0), rel0, n, out.data());
+    return out.data();
+  }
+  if (std::find(pv.begin(), pv.end(), name) != pv.end()) {
+    out.assign((size_t)n * g.n_port, 0.f);
+    std::vector<float> col(n);
+    for (int q = 0; q < g.n_port; ++q) {
+      col_read(calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, q), rel0, n, col.data());
+      for (int k = 0; k < n; ++k) out[(size_t)k * g.n_port + q] = col[k];
+    }
+    return out.data();
+  }
+  throw Error("Unrecognized variable " + name);
+}
+
+float* Engine::GetVar(const int* nodes, int n, const std::string& name) {
+  Group* g;
+  auto rel = NodeListRel(nodes, n, &g);
+  int gi = group_index(groups_, g);
+  auto& sv = g->md->scal_var;
+  auto& pv = g->md->port_var;
+  std::vector<float> col(g->n);
+  std::vector<float>& out = ret_float(1);
+  if (std::find(sv.begin(), sv.end(), name) != sv.end()) {
+    col_read(calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, 0), 0, g->n, col.data());
+    out.resize(n);
+    for (int k = 0; k < n; ++k) out[k] = col[rel[k]];
+    return out.data();
+  }
+  if (std::find(pv.begin(), pv.end(), name) != pv.end()) {
+    out.assign((size_t)n * g->n_port, 0.f);
+    for (int q = 0; q < g->n_port; ++q) {
+      col_read(calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, q), 0, g->n, col.data());
+      for (int k = 0; k < n; ++k) out[(size_t)k * g->n_port + q] = col[rel[k]];
+    }
+    return out.data();
+  }
+  throw Error("Unrecognized variable " + name);
+}
+
+float* Engine::GetArrayVar(int, const std::string& name) {
+  throw Error("Unrecognized array variable " + name);
+}
+
+int* Engine::GetIntVar(int i_node, int n, const std::string& name) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (std::find(g.int_var.begin(), g.int_var.end(), name) == g.int_var.end())
+    throw Error("Unrecognized integer variable " + name);
+  std::vector<int>& out = ret_int();
+  out.assign(n, 0);
+  if (calibrated_) {
+    DBuf<int>& d = (name == "spike_count") ? g.spike_count : g.n_rec;
+    HIPCHK(hipStreamSynchronize(stream_));
+    HIPCHK(hipMemcpy(out.data(), d.p + rel0, n * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  return out.data();
+}
+
+int* Engine::GetIntVar(const int* nodes, int n, const std::string& name) {
+  std::vector<int> tmp(n);
+  for (int k = 0; k < n; ++k) tmp[k] = GetIntVar(nodes[k], 1, name)[0];
+  std::vector<int>& out = ret_int();
+  out = tmp;
+  return out.data();
+}
+
+int Engine::SetGroupParam(int i_node, int n, const std::string& name, float val) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (rel0 != 0 || n != g.n)
+    throw Error("Group parameter must be set for all and only the nodes of the same group");
+  if (name == "h_min_rel") g.h_min_rel = val;
+  else if (name == "h0_rel") g.h0_rel = val;
+  else throw Error("Unrecognized neuron group parameter " + name);
+  if (calibrated_ && g.md->rk5) {
+    g.h_min = g.h_min_rel * dt_;
+    g.h0 = g.h0_rel * dt_;
+  }
+  return 0;
+}
+
+float Engine::GetGroupParam(int node, const std::string& name) {
+  int rel;
+  Group& g = GroupOf(node, 1, &rel);
+  if (!g.md->rk5) throw Error("Unrecognized neuron group parameter " + name);
+  if (name == "h_min_rel") return g.h_min_rel;
+  if (name == "h0_rel") return g.h0_rel;
+  throw Error("Unrecognized neuron group parameter " + name);
+}
+
+std::vector<std::string> Engine::ScalVarNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->scal_var; }
+std::vector<std::string> Engine::PortVarNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->port_var; }
+std::vector<std::string> Engine::ScalParamNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->scal_param; }
+std::vector<std::string> Engine::PortParamNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->port_param; }
+std::vector<std::string> Engine::GroupParamNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->group_param; }
+std::vector<std::string> Engine::ArrayParamNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->md->array_param; }
+std::vector<std::string> Engine::ArrayVarNames(int) { return {}; }
+std::vector<std::string> Engine::IntVarNames(int node) { return group_at(groups_, node_group_, node, n_nodes_)->int_var; }
+
+// ---- host random numbers ----------------------------------------------------
+void Engine::random_block(uint32_t out[4]) {
+  U32x4 c{(uint32_t)rng_ctr_, (uint32_t)(rng_ctr_ >> 32), 0x686f7374u, 0u};
+  ++rng_ctr_;
+  U32x4 r = philox4x32_10(c, (uint32_t)seed_, (uint32_t)(seed_ >> 32));
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+unsigned int* Engine::RandomInt(size_t n) {
+  unsigned int* a = new unsigned int[n];
+  uint32_t b[4];
+  for (size_t i = 0; i < n; i += 4) {
+    random_block(b);
+    for (size_t k = 0; k < 4 && i + k < n; ++k) a[i + k] = b[k];
+  }
+  return a;
+}
+
+// uniform on (0, 1]: ((u >> 8) + 1) * 2^-24, exact in float
+float* Engine::RandomUniform(size_t n) {
+  float* a = new float[n];
+  uint32_t b[4];
+  for (size_t i = 0; i < n; i += 4) {
+    random_block(b);
+    for (size_t k = 0; k < 4 && i + k < n; ++k) a[i + k] = ((b[k] >> 8) + 1) * (1.0f / 16777216.0f);
+  }
+  return a;
+}
+
+// Box-Muller on pairs of uniforms (Box & Muller 1958)
+float* Engine::RandomNormal(size_t n, float mean, float sd) {
+  float* a = new float[n];
+  uint32_t b[4];
+  for (size_t i = 0; i < n; i += 4) {
+    random_block(b);
+    for (int k = 0; k < 4; k += 2) {
+      double u1 = ((b[k] >> 8) + 1) * (1.0 / 16777216.0);
+      double u2 = (b[k + 1] >> 8) * (1.0 / 16777216.0);
+      double rr = std::sqrt(-2.0 * std::log(u1));
+      double z0 = rr * std::cos(2.0 * M_PI * u2), z1 = rr * std::sin(2.0 * M_PI * u2);
+      if (i + k < n) a[i + k] = (float)(mean + sd * z0);
+      if (i + k + 1 < n) a[i + k + 1] = (float)(mean + sd * z1);
+    }
+  }
+  return a;
+}
+
+// Normal draws redrawn until inside [vmin, vmax], then optionally rounded to
+// the grid vmin + k*vstep (the reference's rule, src/nestgpu.cu:1150-1188).
+float* Engine::RandomNormalClipped(size_t n, float mean, float sd, float vmin, float vmax, float vstep) {
+  float* a = RandomNormal(n, mean, sd);
+  for (size_t i = 0; i < n; ++i) {
+    int tries = 0;
+    while (a[i] < vmin || a[i] > vmax) {
+      float* e = RandomNormal(2, mean, sd);
+      a[i] = e[0];
+      delete[] e;
+      if (++tries > 1000000) throw Error("normal_clipped: interval has negligible probability");
+    }
+  }
+  if (vstep > sd * 1.0e-6f)
+    for (size_t i = 0; i < n; ++i) a[i] = vmin + vstep * std::round((a[i] - vmin) / vstep);
+  return a;
+}
+
+// ---- connections ------------------------------------------------------------
+int Engine::Connect(int src, int tgt, int port, int syn_group, float w, float delay) {
+  check_uncalibrated("Connections cannot be created after calibration");
+  if (src < 0 || src >= n_nodes_ || tgt < 0 || tgt >= n_nodes_)
+    throw Error("Unrecognized node in connection");
+  if (delay < dt_) throw Error("Delay must be >= time resolution");
+  if (syn_group != 0)
+    throw Error("Synapse groups (STDP) are outside the scope of this build");
+  if (port < 0 || port > 255) throw Error("Port index must be >=0 and < 256");
+  int d_int = (int)std::round(delay / dt_) - 1;   // src/connect.cu:45
+  if (d_int > 65535) throw Error("Delay too large (max 65536 steps)");
+  c_src_.push_back(src);
+  c_tgt_.push_back(tgt);
+  c_port_.push_back((unsigned char)port);
+  c_syn_.push_back((unsigned char)syn_group);
+  c_w_.push_back(w);
+  c_d_.push_back(d_int);
+  sorted_ = false;
+  return 0;
+}
+
+int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* tgt_list,
+                        int i_tgt0, int n_tgt, const ConnSpecS& cs, const SynSpecS& ss) {
+  check_uncalibrated("Connections cannot be created after calibration");
+  if (ss.syn_group != 0) throw Error("Synapse groups (STDP) are outside the scope of this build");
+  auto S = [&](int k) { return src_list ? src_list[k] : i_src0 + k; };
+  auto T = [&](int k) { return tgt_list ? tgt_list[k] : i_tgt0 + k; };
+  for (int k = 0; k < n_src; ++k)
+    if (S(k) < 0 || S(k) >= n_nodes_) throw Error("Unrecognized source node");
+  for (int k = 0; k < n_tgt; ++k)
+    if (T(k) < 0 || T(k) >= n_nodes_) throw Error("Unrecognized target node");
+  // connections are generated as (isn, itn, i_array) triples in the
+  // reference's array order, then appended in that order
+  std::vector<int> ps, pt;
+  std::vector<long long> pa;
+  switch (cs.rule) {
+    case 0:  // one_to_one
+      if (n_src != n_tgt)
+        throw Error("Number of source and target nodes must be equal for the one-to-one connection rule");
+      for (int k = 0; k < n_src; ++k) { ps.push_back(k); pt.push_back(k); pa.push_back(k); }
+      break;
+    case 1:  // all_to_all: target-major, src/connect_rules.h:173-185
+      ps.reserve((size_t)n_src * n_tgt); pt.reserve((size_t)n_src * n_tgt); pa.reserve((size_t)n_src * n_tgt);
+      for (int it = 0; it < n_tgt; ++it)
+        for (int is = 0; is < n_src; ++is) {
+          ps.push_back(is); pt.push_back(it); pa.push_back((long long)it * n_src + is);
+        }
+      break;
+    case 2: {  // fixed_total_number
+      unsigned int* r = RandomInt(2 * (size_t)cs.total_num);
+      for (int c = 0; c < cs.total_num; ++c) {
+        ps.push_back(r[2 * c] % n_src); pt.push_back(r[2 * c + 1] % n_tgt); pa.push_back(c);
+      }
+      delete[] r;
+      break;
+    }
+    case 3:    // fixed_indegree: distinct sources per target, drawn per target
+    case 4: {  // fixed_outdegree: distinct targets per source, drawn per source
+      const bool in = cs.rule == 3;
+      const int deg = in ? cs.indegree : cs.outdegree;
+      const int n_pool = in ? n_src : n_tgt;
+      const int n_own = in ? n_tgt : n_src;
+      if (deg > n_pool)
+        throw Error(in ? "Indegree larger than number of source nodes"
+                       : "Outdegree larger than number of target nodes");
+      const uint64_t key = seed_ ^ ((uint64_t)(++rng_ctr_) << 40);
+      std::vector<int> chosen((size_t)n_own * deg);
+#pragma omp parallel
+      {
+        std::vector<char> used(n_pool, 0);
+        std::vector<int> perm;
+#pragma omp for schedule(dynamic, 64)
+        for (int o = 0; o < n_own; ++o) {
+          uint32_t blk = 0;
+          U32x4 r{};
+          int avail = 0;
+          auto next = [&]() -> uint32_t {
+            if (avail == 0) {
+              r = philox4x32_10(U32x4{blk++, (uint32_t)o, 0x72756c65u, 0u}, (uint32_t)key,
+                                (uint32_t)(key >> 32));
+              avail = 4;
+            }
+            return pick(r, 4 - avail--);
+          };
+          int* out = chosen.data() + (size_t)o * deg;
+          if (n_pool < 5 * deg) {  // partial Fisher-Yates over the pool
+            perm.resize(n_pool);
+            std::iota(perm.begin(), perm.end(), 0);
+            for (int i = 0; i < deg; ++i) {
+              int j = i + (int)(next() % (uint32_t)(n_pool - i));
+              std::swap(perm[i], perm[j]);
+              out[i] = perm[i];
+            }
+          } else {  // rejection with a marker array
+            for (int i = 0; i < deg; ++i) {
+              int j;
+              do { j = (int)(next() % (uint32_t)n_pool); } while (used[j]);
+              used[j] = 1;
+              out[i] = j;
+            }
+            for (int i = 0; i < deg; ++i) used[out[i]] = 0;
+          }
+        }
+      }
+      size_t tot = (size_t)n_own * deg;
+      ps.resize(tot); pt.resize(tot); pa.resize(tot);
+      for (int o = 0; o < n_own; ++o)
+        for (int i = 0; i < deg; ++i) {
+          size_t k = (size_t)o * deg + i;
+          if (in) { ps[k] = chosen[k]; pt[k] = o; } else { ps[k] = o; pt[k] = chosen[k]; }
+          pa[k] = (long long)k;
+        }
+      break;
+    }
+    default:
+      throw Error("Unknown connection rule");
+  }
+  size_t n = ps.size();
+  size_t base = c_src_.size();
+  // validate delays first so a bad call leaves the store unchanged
+  for (size_t k = 0; k < n; ++k) {
+    float d = ss.delay_array ? ss.delay_array[pa[k]] : ss.delay;
+    if (d < dt_) throw Error("Delay must be >= time resolution");
+  }
+  c_src_.resize(base + n); c_tgt_.resize(base + n); c_port_.resize(base + n);
+  c_syn_.resize(base + n); c_w_.resize(base + n); c_d_.resize(base + n);
+#pragma omp parallel for schedule(static)
+  for (long long k = 0; k < (long long)n; ++k) {
+    float w = ss.weight_array ? ss.weight_array[pa[k]] : ss.weight;
+    float d = ss.delay_array ? ss.delay_array[pa[k]] : ss.delay;
+    c_src_[base + k] = S(ps[k]);
+    c_tgt_[base + k] = T(pt[k]);
+    c_port_[base + k] = (unsigned char)ss.port;
+    c_syn_[base + k] = (unsigned char)ss.syn_group;
+    c_w_[base + k] = w;
+    c_d_[base + k] = (int)std::round(d / dt_) - 1;
+  }
+  for (size_t k = 0; k < n; ++k)
+    if (c_d_[base + k] > 65535) throw Error("Delay too large (max 65536 steps)");
+  if (n) sorted_ = false;
+  return 0;
+}
+
+// Stable sort of the store by (source, d_int): counting sort by source, then
+// a stable sort by delay inside each source segment.
+void Engine::sort_connections() {
+  const size_t nc = c_src_.size();
+  if (sorted_ && row_ptr_.size() == (size_t)n_nodes_ + 1) return;
+  std::vector<long long> cnt(n_nodes_ + 1, 0);
+  for (size_t k = 0; k < nc; ++k) cnt[c_src_[k] + 1]++;
+  for (int s = 0; s < n_nodes_; ++s) cnt[s + 1] += cnt[s];
+  row_ptr_ = cnt;
+  std::vector<long long> dest(nc);
+  {
+    std::vector<long long> pos(cnt.begin(), cnt.end() - 1);
+    for (size_t k = 0; k < nc; ++k) dest[k] = pos[c_src_[k]]++;
+  }
+  auto permute = [&](auto& v) {
+    using T = typename std::decay<decltype(v)>::type::value_type;
+    std::vector<T> o(nc);
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)nc; ++k) o[dest[k]] = v[k];
+    v.swap(o);
+  };
+  permute(c_src_); permute(c_tgt_); permute(c_port_); permute(c_syn_); permute(c_w_); permute(c_d_);
+  dest.clear();
+  dest.shrink_to_fit();
+  // stable sort by delay within each source segment
+#pragma omp parallel
+  {
+    std::vector<long long> idx;
+    std::vector<int> t_tgt, t_d;
+    std::vector<unsigned char> t_port, t_syn;
+    std::vector<float> t_w;
+#pragma omp for schedule(dynamic, 256)
+    for (int s = 0; s < n_nodes_; ++s) {
+      long long b = row_ptr_[s], e = row_ptr_[s + 1];
+      bool ok = true;
+      for (long long k = b + 1; k < e && ok; ++k) ok = c_d_[k - 1] <= c_d_[k];
+      if (ok) continue;
+      idx.resize(e - b);
+      std::iota(idx.begin(), idx.end(), b);
+      std::stable_sort(idx.begin(), idx.end(), [&](long long x, long long y) { return c_d_[x] < c_d_[y]; });
+      t_tgt.resize(e - b); t_d.resize(e - b); t_port.resize(e - b); t_syn.resize(e - b); t_w.resize(e - b);
+      for (long long k = 0; k < e - b; ++k) {
+        t_tgt[k] = c_tgt_[idx[k]]; t_d[k] = c_d_[idx[k]]; t_port[k] = c_port_[idx[k]];
+        t_syn[k] = c_syn_[idx[k]]; t_w[k] = c_w_[idx[k]];
+      }
+      for (long long k = 0; k < e - b; ++k) {
+        c_tgt_[b + k] = t_tgt[k]; c_d_[b + k] = t_d[k]; c_port_[b + k] = t_port[k];
+        c_syn_[b + k] = t_syn[k]; c_w_[b + k] = t_w[k];
+      }
+    }
+  }
+  // delay groups
+  grp_ptr_.assign(n_nodes_ + 1, 0);
+  grp_start_.clear();
+  for (int s = 0; s < n_nodes_; ++s) {
+    grp_ptr_[s] = (long long)grp_start_.size();
+    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k)
+      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) grp_start_.push_back(k);
+  }
+  grp_ptr_[n_nodes_] = (long long)grp_start_.size();
+  grp_start_.push_back((long long)nc);
+  sorted_ = true;
+}
+
+std::vector<long long> Engine::GetConnections(const int* src_list, int i_src0, int n_src,
+                                              const int* tgt_list, int i_tgt0, int n_tgt) {
+  sort_connections();
+  if (!src_list && n_src <= 0) { i_src0 = 0; n_src = n_nodes_; }
+  if (!tgt_list && n_tgt <= 0) { i_tgt0 = 0; n_tgt = n_nodes_; }
+  std::vector<int> tv;
+  if (tgt_list) { tv.assign(tgt_list, tgt_list + n_tgt); std::sort(tv.begin(), tv.end()); }
+  std::vector<long long> out;
+  for (int k = 0; k < n_src; ++k) {
+    int s = src_list ? src_list[k] : i_src0 + k;
+    if (s < 0 || s >= n_nodes_) throw Error("Unrecognized source node");
+    for (long long gi = grp_ptr_[s]; gi < grp_ptr_[s + 1]; ++gi) {
+      for (long long c = grp_start_[gi]; c < grp_start_[gi + 1]; ++c) {
+        int t = c_tgt_[c];
+        bool hit = tgt_list ? std::binary_search(tv.begin(), tv.end(), t)
+                            : (t >= i_tgt0 && t < i_tgt0 + n_tgt);
+        if (hit) {
+          out.push_back(s);
+          out.push_back(gi - grp_ptr_[s]);
+          out.push_back(c - grp_start_[gi]);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+void Engine::GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned char* port,
+                                 unsigned char* syn, float* delay, float* w) {
+  sort_connections();
+  if (src < 0 || src >= n_nodes_) throw Error("Unrecognized source node");
+  long long gi = grp_ptr_[src] + grp;
+  if (grp < 0 || gi >= grp_ptr_[src + 1]) throw Error("Unrecognized connection group");
+  long long c = grp_start_[gi] + conn;
+  if (conn < 0 || c >= grp_start_[gi + 1]) throw Error("Unrecognized connection index");
+  *tgt = c_tgt_[c];
+  *port = c_port_[c];
+  *syn = c_syn_[c];
+  *delay = dt_ * (c_d_[c] + 1);   // src/connect.cu:168
+  *w = c_w_[c];
+}
+
+// ---- recording ----------------------------------------------------------------
+int Engine::CreateRecord(const std::string& file, char** var_names, const int* nodes,
+                         const int* ports, int n) {
+  RecordDef r;
+  r.file = file;
+  for (int k = 0; k < n; ++k) {
+    int rel;
+    Group& g = GroupOf(nodes[k], 1, &rel);
+    std::string v = var_names[k];
+    if (v != "spike" && var_idx(g, v, 0) < 0) throw Error("Unrecognized variable " + v);
+    if (ports[k] < 0 || (ports[k] != 0 && ports[k] >= g.n_port)) throw Error("Port index out of range");
+    r.group.push_back(group_index(groups_, &g));
+    r.rel.push_back(rel);
+    r.port.push_back(ports[k]);
+    r.node.push_back(nodes[k]);
+    r.var.push_back(v);
+  }
+  if (!file.empty()) {
+    r.fp = fopen(file.c_str(), "w");
+    if (!r.fp) throw Error("Cannot open record file " + file);
+  }
+  records_.push_back(std::move(r));
+  return (int)records_.size() - 1;
+}
+
+RecordDef& Engine::Record(int i) {
+  if (i < 0 || i >= (int)records_.size()) throw Error("Record does not exist.");
+  RecordDef& r = records_[i];
+  r.rows.resize(r.data.size());
+  for (size_t k = 0; k < r.data.size(); ++k) r.rows[k] = r.data[k].data();
+  return r;
+}
+
+int Engine::ActivateSpikeCount(int i_node, int n) {
+  check_uncalibrated("Spike count must be activated before calibration");
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (rel0 != 0 || n != g.n)
+    throw Error("Spike count must be activated for all and only  the nodes of the same group");
+  if (std::find(g.int_var.begin(), g.int_var.end(), "spike_count") != g.int_var.end())
+    throw Error("Spike count already activated");
+  g.int_var.push_back("spike_count");
+  return 0;
+}
+
+int Engine::ActivateRecSpikeTimes(int i_node, int n, int max_n) {
+  check_uncalibrated("Spike time recording must be activated before calibration");
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (rel0 != 0 || n != g.n)
+    throw Error("Spike count must be activated for all and only  the nodes of the same group");
+  if (max_n <= 0) throw Error("Maximum number of recorded spike times must be greater than 0");
+  if (g.max_rec > 0) throw Error("Spike times recording already activated");
+  g.int_var.push_back("n_rec_spike_times");
+  g.max_rec = max_n;
+  g.rec_host.assign(g.n, {});
+  return 0;
+}
+
+int Engine::SetRecSpikeTimesStep(int i_node, int n, int step) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (rel0 != 0 || n != g.n)
+    throw Error("Time step for buffering spike time recording must be set for all and only the nodes of the same group");
+  g.rec_step = step;
+  return 0;
+}
+
+int Engine::GetNRecSpikeTimes(int node) {
+  int rel;
+  Group& g = GroupOf(node, 1, &rel);
+  if (g.max_rec <= 0) throw Error("Spike times recording was not activated");
+  if (!calibrated_) return 0;
+  int v;
+  HIPCHK(hipStreamSynchronize(stream_));
+  HIPCHK(hipMemcpy(&v, g.n_rec.p + rel, sizeof(int), hipMemcpyDeviceToHost));
+  return v;
+}
+
+// Move the device spike-time buffers to the host and reset the counters
+// (src/base_neuron.cu:1488-1526).
+void Engine::buffer_rec_spike_times(Group& g) {
+  if (!calibrated_ || g.max_rec <= 0) return;
+  HIPCHK(hipStreamSynchronize(stream_));
+  std::vector<int> cnt(g.n);
+  HIPCHK(hipMemcpy(cnt.data(), g.n_rec.p, g.n * sizeof(int), hipMemcpyDeviceToHost));
+  long long tot = 0;
+  for (int c : cnt) tot += c;
+  if (tot == 0) return;
+  std::vector<float> t((size_t)g.n * g.max_rec);
+  HIPCHK(hipMemcpy(t.data(), g.rec_times.p, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < g.n; ++i)
+    g.rec_host[i].insert(g.rec_host[i].end(), t.begin() + (size_t)i * g.max_rec,
+                         t.begin() + (size_t)i * g.max_rec + cnt[i]);
+  HIPCHK(hipMemset(g.n_rec.p, 0, g.n * sizeof(int)));
+}
+
+int Engine::GetRecSpikeTimes(int i_node, int n, int** n_pt, float*** t_pt) {
+  int rel0;
+  Group& g = GroupOf(i_node, n, &rel0);
+  if (rel0 != 0 || n != g.n)
+    throw Error("Spike times must be extracted for all and only  the nodes of the same group");
+  if (g.max_rec <= 0) throw Error("Spike times recording was not activated");
+  buffer_rec_spike_times(g);
+  g.rec_n_host.resize(g.n);
+  g.rec_pt_host.resize(g.n);
+  for (int i = 0; i < g.n; ++i) {
+    g.rec_n_host[i] = (int)g.rec_host[i].size();
+    g.rec_pt_host[i] = g.rec_host[i].data();
+  }
+  *n_pt = g.rec_n_host.data();
+  *t_pt = g.rec_pt_host.data();
+  return 0;
+}
+
+// ---- calibration --------------------------------------------------------------
+void Engine::calibrate_group(Group& g) {
+  int gi = group_index(groups_, &g);
+  std::vector<float>& par = stage_param_[gi];
+  const int n = g.n;
+  auto col = [&](int j) { return par.data() + (size_t)j * n; };
+  const ModelDesc& m = *g.md;
+  const int ns = (int)m.scal_param.size(), npp = (int)m.port_param.size();
+  if (m.rk5) {
+    g.h_min = g.h_min_rel * dt_;
+    g.h0 = g.h0_rel * dt_;
+    float* ref = col(param_idx(g, "refractory_step", 0));
+    std::fill(ref, ref + n, 0.f);
+    if (m.id == MODEL_AEIF_COND_BETA_MS || m.id == MODEL_IZH_COND_BETA) {
+      // beta normalisation, src/aeif_cond_beta_multisynapse.cu:72-99
+      for (int q = 0; q < g.n_port; ++q) {
+        int jr = ns + q * npp + 1, jd = ns + q * npp + 2, jg = ns + q * npp + 3;
+        auto g0_of = [](float tr, float td) -> float {
+          float denom1 = td - tr, denom2 = 0;
+          if (denom1 != 0) {
+            float t_p = td * tr * std::log(td / tr) / denom1;
+            denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
+          }
+          if (denom2 == 0) return (float)(M_E / td);
+          return (float)((1. / tr - 1. / td) / denom2);
+        };
+        float* gc = col(jg);
+        const float* trc = col(jr);
+        const float* tdc = col(jd);
+        for (int i = 0; i < n; ++i) gc[i] = g0_of(trc[i], tdc[i]);
+        g.uni[jg] = g.uni[jr] && g.uni[jd];
+        g.uni_val[jg] = gc[0];
+      }
+    } else if (m.id == MODEL_AEIF_COND_ALPHA_MS) {
+      for (int q = 0; q < g.n_port; ++q) {
+        int js = ns + q * npp + 1, jg = ns + q * npp + 2;
+        float* gc = col(jg);
+        const float* ts = col(js);
+        for (int i = 0; i < n; ++i) gc[i] = (float)(M_E / ts[i]);
+        g.uni[jg] = g.uni[js];
+        g.uni_val[jg] = gc[0];
+      }
+    } else if (m.id == MODEL_AEIF_PSC_EXP) {
+      int jdt = param_idx(g, "Delta_T", 0), jth = param_idx(g, "V_th", 0), jpk = param_idx(g, "V_peak", 0);
+      bool changed = false;
+      for (int i = 0; i < n; ++i)
+        if (col(jdt)[i] <= 0.0f) { col(jpk)[i] = col(jth)[i]; changed = true; }
+      if (changed) {
+        bool same = std::all_of(col(jpk), col(jpk) + n, [&](float v) { return v == col(jpk)[0]; });
+        g.uni[jpk] = same;
+        g.uni_val[jpk] = col(jpk)[0];
+      }
+    }
+  } else if (m.id == MODEL_IAF_PSC_EXP) {
+    // propagators, src/iaf_psc_exp.cu:68-82 (FP64 propagator_32)
+    auto p32 = [](double tau_syn, double tau, double C, double h) {
+      const double lin = 1.0 / (2.0 * C * tau * tau) * h * h * (tau_syn - tau) * std::exp(-h / tau);
+      const double sing = h / C * std::exp(-h / tau);
+      const double p = -tau / (C * (1.0 - tau / tau_syn)) * std::exp(-h / tau_syn) *
+                       std::expm1(h * (1.0 / tau_syn - 1.0 / tau));
+      const double dev = std::fabs(p - sing);
+      if (tau == tau_syn || (std::fabs(tau - tau_syn) < 0.1 && dev > 2.0 * std::fabs(lin))) return sing;
+      return p;
+    };
+    enum { tau_m, C_m, E_L, I_e, Theta, V_reset, tau_ex, tau_in, t_ref, den, P20, P11ex, P11in,
+           P21ex, P21in, P22 };
+    const float h = dt_;
+    for (int i = 0; i < n; ++i) {
+      float tm = col(tau_m)[i], cm = col(C_m)[i], te = col(tau_ex)[i], ti = col(tau_in)[i];
+      col(P11ex)[i] = std::exp(-h / te);
+      col(P11in)[i] = std::exp(-h / ti);
+      float p22 = std::exp(-h / tm);
+      col(P22)[i] = p22;
+      col(P21ex)[i] = (float)p32(te, tm, cm, h);
+      col(P21in)[i] = (float)p32(ti, tm, cm, h);
+      col(P20)[i] = (float)(tm / cm * (1.0 - p22));
+    }
+    for (int j : {P20, P11ex, P11in, P21ex, P21in, P22}) {
+      bool same = std::all_of(col(j), col(j) + n, [&](float v) { return v == col(j)[0]; });
+      g.uni[j] = same;
+      g.uni_val[j] = col(j)[0];
+    }
+  } else if (m.id == MODEL_SPIKE_GEN) {
+    // src/spike_generator.cu:230-269
+    std::vector<int> off(n + 1, 0), tidx;
+    std::vector<float> hh;
+    for (int i = 0; i < n; ++i) {
+      auto& t = g.sg_times[i];
+      auto& hg = g.sg_heights[i];
+      if (!t.empty()) {
+        if (hg.empty()) hg.assign(t.size(), 1.0f);
+        else if (hg.size() != t.size())
+          throw Error("spike time array and spike height array must have the same size in spike generator");
+        for (size_t k = 0; k < t.size(); ++k) {
+          int ti = (int)std::round((t[k] - (float)t_min_) / dt_);
+          if (k > 0 && ti <= tidx.back())
+            throw Error("Spike times must be ordered, and the difference between\nconsecutive spikes must be >= the time resolution");
+          tidx.push_back(ti);
+          hh.push_back(hg[k]);
+        }
+      }
+      off[i + 1] = (int)tidx.size();
+    }
+    upload(g.sg_off, off, stream_);
+    upload(g.sg_tidx, tidx, stream_);
+    upload(g.sg_h, hh, stream_);
+    std::vector<int> zero(n, 0);
+    upload(g.sg_next, zero, stream_);
+  }
+}
+
+int Engine::Calibrate() {
+  check_uncalibrated("Calibration can be made only once");
+  if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  sort_connections();
+  const size_t nc = c_src_.size();
+  // channels and port checks
+  n_ch_ = 0;
+  for (auto* g : groups_) {
+    g->ch_base = n_ch_;
+    n_ch_ += (long long)g->n * g->n_port;
+  }
+  if (n_ch_ >= (1ll << 32)) throw Error("Too many input channels");
+  max_d_int_ = 0;
+  has_out_.assign(n_nodes_, 0);
+  std::vector<unsigned int> ch(nc);
+  std::vector<unsigned short> dd(nc);
+  bool bad_port = false;
+#pragma omp parallel for schedule(static) reduction(max : max_d_int_) reduction(|| : bad_port)
+  for (long long k = 0; k < (long long)nc; ++k) {
+    const Group* g = groups_[node_group_[c_tgt_[k]]];
+    int port = c_port_[k];
+    if (port >= g->n_port) bad_port = true;
+    ch[k] = (unsigned int)(g->ch_base + (long long)port * g->n + (c_tgt_[k] - g->i0));
+    dd[k] = (unsigned short)c_d_[k];
+    max_d_int_ = std::max(max_d_int_, c_d_[k]);
+  }
+  if (bad_port) throw Error("Connection port index larger than the number of ports of the target");
+  for (int s = 0; s < n_nodes_; ++s) has_out_[s] = row_ptr_[s + 1] > row_ptr_[s];
+  n_slot_ = max_d_int_ + 2;
+  // device synapse table
+  upload(d_row_ptr_, row_ptr_, stream_);
+  upload(d_syn_ch_, ch, stream_);
+  upload(d_syn_w_, c_w_, stream_);
+  upload(d_syn_d_, dd, stream_);
+  HIPCHK(hipStreamSynchronize(stream_));
+  ch.clear(); ch.shrink_to_fit(); dd.clear(); dd.shrink_to_fit();
+  d_ring_.alloc((size_t)n_slot_ * std::max<long long>(n_ch_, 1));
+  HIPCHK(hipMemsetAsync(d_ring_.p, 0, d_ring_.n * sizeof(double), stream_));
+  // spike list: every node may emit in a step; headroom for repeated spikes
+  spk_cap_ = std::max(2 * n_nodes_, 1024);
+  d_spk_count_.alloc(2);
+  d_overflow_.alloc(1);
+  d_spk_src_.alloc((size_t)2 * spk_cap_);
+  d_spk_h_.alloc((size_t)2 * spk_cap_);
+  HIPCHK(hipMemsetAsync(d_spk_count_.p, 0, 2 * sizeof(int), stream_));
+  HIPCHK(hipMemsetAsync(d_overflow_.p, 0, sizeof(int), stream_));
+  std::vector<long long> guard(n_nodes_, 0x70000000LL);   // src/spike_buffer.cu:37
+  upload(d_last_idx_, guard, stream_);
+  d_last_h_.alloc(std::max(n_nodes_, 1));
+  HIPCHK(hipMemsetAsync(d_last_h_.p, 0, d_last_h_.n * sizeof(float), stream_));
+  d_ctx_.alloc(1);
+  d_stats_.alloc(4);
+  HIPCHK(hipMemsetAsync(d_stats_.p, 0, 4 * sizeof(unsigned long long), stream_));
+  // poisson generator tables and direct edges, src/nestgpu.cu:1190-1226
+  n_gen_ = 0;
+  for (auto* g : groups_)
+    if (g->md->id == MODEL_POISSON_GEN) { g->gen_base = n_gen_; n_gen_ += g->n; }
+  struct Edge { long long ch; int gen; float w, delay; unsigned int stream; };
+  std::vector<Edge> edges;
+  for (auto* g : groups_) {
+    if (g->md->id != MODEL_POISSON_GEN) continue;
+    for (int s = g->i0; s < g->i0 + g->n; ++s)
+      for (long long gi = grp_ptr_[s]; gi < grp_ptr_[s + 1]; ++gi)
+        for (long long c = grp_start_[gi]; c < grp_start_[gi + 1]; ++c) {
+          const Group* tg = groups_[node_group_[c_tgt_[c]]];
+          Edge e;
+          e.ch = tg->ch_base + (long long)c_port_[c] * tg->n + (c_tgt_[c] - tg->i0);
+          e.gen = g->gen_base + (s - g->i0);
+          e.w = c_w_[c];
+          e.delay = dt_ * (c_d_[c] + 1);
+          e.stream = (unsigned int)c;
+          edges.push_back(e);
+        }
+  }
+  std::stable_sort(edges.begin(), edges.end(), [](const Edge& a, const Edge& b) { return a.ch < b.ch; });
+  for (auto* g : groups_) {
+    if (g->n_port == 0) continue;
+    long long lo = g->ch_base, hi = g->ch_base + (long long)g->n * g->n_port;
+    auto b = std::lower_bound(edges.begin(), edges.end(), lo, [](const Edge& e, long long v) { return e.ch < v; });
+    auto e = std::lower_bound(edges.begin(), edges.end(), hi, [](const Edge& e, long long v) { return e.ch < v; });
+    if (b == e) { g->has_pe = false; continue; }
+    g->has_pe = true;
+    size_t ne = e - b;
+    std::vector<int> off((size_t)g->n * g->n_port + 1, 0), gen(ne);
+    std::vector<float> w(ne), dl(ne);
+    std::vector<unsigned int> st(ne);
+    for (size_t k = 0; k < ne; ++k) {
+      const Edge& x = *(b + k);
+      off[x.ch - lo + 1]++;
+      gen[k] = x.gen; w[k] = x.w; dl[k] = x.delay; st[k] = x.stream;
+    }
+    for (size_t k = 1; k < off.size(); ++k) off[k] += off[k - 1];
+    upload(g->pe_off, off, stream_);
+    upload(g->pe_gen, gen, stream_);
+    upload(g->pe_w, w, stream_);
+    upload(g->pe_delay, dl, stream_);
+    upload(g->pe_stream, st, stream_);
+  }
+  d_gen_chunks_.alloc(std::max(n_gen_, 1));
+  d_gen_exp_.alloc(std::max(n_gen_, 1));
+  d_gen_origin_.alloc(std::max(n_gen_, 1));
+  d_gen_start_.alloc(std::max(n_gen_, 1));
+  d_gen_stop_.alloc(std::max(n_gen_, 1));
+  // populations: calibrate on the host staging, then upload
+  for (auto* g : groups_) {
+    calibrate_group(*g);
+    int gi = group_index(groups_, g);
+    upload(g->var, stage_var_[gi], stream_);
+    upload(g->param, stage_param_[gi], stream_);
+    if (g->md->rk5) {
+      std::vector<double> x(g->n, t_min_);
+      std::vector<float> h(g->n, g->h0);
+      upload(g->x, x, stream_);
+      upload(g->h, h, stream_);
+    }
+    for (auto& iv : g->int_var) {
+      std::vector<int> z(g->n, 0);
+      if (iv == "spike_count") upload(g->spike_count, z, stream_);
+      else upload(g->n_rec, z, stream_);
+    }
+    if (g->max_rec > 0) g->rec_times.alloc((size_t)g->n * g->max_rec);
+  }
+  HIPCHK(hipStreamSynchronize(stream_));
+  for (size_t k = 0; k < groups_.size(); ++k) {
+    stage_var_[k].clear(); stage_var_[k].shrink_to_fit();
+    stage_param_[k].clear(); stage_param_[k].shrink_to_fit();
+  }
+  calibrated_ = true;
+  neural_time_ = t_min_;
+  return 0;
+}
+
+void Engine::upload_poisson_tables() {
+  if (n_gen_ == 0) return;
+  // lambda = time_step*rate in float, src/poiss_gen.cu:74, src/nestgpu.cu:560
+  const float time_step = (float)(dt_ / 1000.0);
+  std::vector<int> chunks(n_gen_);
+  std::vector<double> ex(n_gen_);
+  std::vector<float> org(n_gen_), sta(n_gen_), sto(n_gen_);
+  for (auto* g : groups_) {
+    if (g->md->id != MODEL_POISSON_GEN) continue;
+    std::vector<float> p((size_t)g->np * g->n);
+    HIPCHK(hipMemcpy(p.data(), g->param.p, p.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g->n; ++i) {
+      float lambda = time_step * p[(size_t)0 * g->n + i];
+      int m = poisson_chunks_host(lambda);
+      chunks[g->gen_base + i] = m;
+      ex[g->gen_base + i] = m > 0 ? std::exp(-(double)lambda / m) : 0.0;
+      org[g->gen_base + i] = p[(size_t)1 * g->n + i];
+      sta[g->gen_base + i] = p[(size_t)2 * g->n + i];
+      sto[g->gen_base + i] = p[(size_t)3 * g->n + i];
+    }
+  }
+  HIPCHK(hipMemcpyAsync(d_gen_chunks_.p, chunks.data(), n_gen_ * sizeof(int), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipMemcpyAsync(d_gen_exp_.p, ex.data(), n_gen_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipMemcpyAsync(d_gen_origin_.p, org.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipMemcpyAsync(d_gen_start_.p, sta.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipMemcpyAsync(d_gen_stop_.p, sto.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
+  HIPCHK(hipStreamSynchronize(stream_));
+}
+
+static ParamTable make_table(const Group& g) {
+  ParamTable t;
+  std::memset(&t, 0, sizeof(t));
+  for (int j = 0; j < g.np && j < kMaxParam; ++j) {
+    t.val[j] = g.uni_val[j];
+    if (!g.uni[j]) t.per_neuron[j >> 6] |= 1ull << (j & 63);
+  }
+  return t;
+}
+
+void Engine::launch_step() {
+  SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
+  for (auto* g : groups_) {
+    SpikeBook sb{d_last_idx_.p, d_last_h_.p, g->spike_count.p, g->rec_times.p, g->n_rec.p,
+                 g->max_rec};
+    switch (g->md->id) {
+      case MODEL_AEIF_COND_BETA_MS:
+      case MODEL_AEIF_COND_ALPHA_MS:
+      case MODEL_AEIF_PSC_EXP:
+      case MODEL_IZH_COND_BETA: {
+        RK5Args a;
+        a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
+        a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        a.stats = d_stats_.p;
+        launch_rk5(g->md->id, g->n_port, a, stream_);
+        break;
+      }
+      case MODEL_IAF_PSC_EXP: {
+        IafArgs a;
+        a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.tab = make_table(*g);
+        a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        launch_iaf(a, stream_);
+        break;
+      }
+      case MODEL_SPIKE_GEN: {
+        SpikeGenArgs a;
+        a.n = g->n; a.i0 = g->i0; a.off = g->sg_off.p; a.t_idx = g->sg_tidx.p; a.height = g->sg_h.p;
+        a.next = g->sg_next.p; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        launch_spike_gen(a, stream_);
+        break;
+      }
+      case MODEL_SPIKE_DETECTOR: {
+        DetectorArgs a{g->n, g->var.p, g->param.p};
+        launch_detector(a, stream_);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  if (rec_ncol_ > 0) {
+    RecordArgs ra{d_rec_cols_.p, rec_ncol_, d_rec_buf_.p, d_ctx_.p, d_last_idx_.p, d_last_h_.p,
+                  rec_row_base_, rec_buf_rows_};
+    launch_record(ra, stream_);
+  }
+  DeliverArgs da;
+  da.ctx = d_ctx_.p; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
+  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_; da.events = d_stats_.p + 3;
+  launch_deliver(da, 1024, stream_);
+  for (auto* g : groups_) {
+    if (g->n_port == 0) continue;
+    FoldArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.n = g->n; fa.n_port = g->n_port; fa.ch_base = g->ch_base; fa.var = g->var.p;
+    const ModelDesc& m = *g->md;
+    if (m.in_port_block) {
+      fa.in_var = (int)m.scal_var.size() + m.in_var;
+      fa.in_port_stride = (int)m.port_var.size();
+    } else {
+      fa.in_var = m.in_var;
+      fa.in_port_stride = 1;
+    }
+    fa.param = g->param.p; fa.tab = make_table(*g);
+    fa.w_param = m.w_param >= 0 ? (int)m.scal_param.size() + m.w_param : -1;
+    fa.w_port_stride = (int)m.port_param.size();
+    fa.ring = d_ring_.p; fa.n_ch = n_ch_; fa.ctx = d_ctx_.p;
+    if (g->has_pe) {
+      fa.pe_off = g->pe_off.p; fa.pe_gen = g->pe_gen.p; fa.pe_w = g->pe_w.p; fa.pe_delay = g->pe_delay.p;
+      fa.pe_stream = g->pe_stream.p; fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
+      fa.gen_origin = d_gen_origin_.p; fa.gen_start = d_gen_start_.p; fa.gen_stop = d_gen_stop_.p;
+      fa.seed = kernel_seed_;
+    }
+    launch_fold(fa, stream_);
+  }
+  launch_end_step(d_ctx_.p, dt_, n_slot_, stream_);
+}
+
+int Engine::StartSimulation() {
+  if (!calibrated_) Calibrate();
+  if (first_sim_) {
+    // record row at the initial time (src/nestgpu.cu:383-388)
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (auto& r : records_) {
+      std::vector<float> row{(float)neural_time_};
+      for (size_t k = 0; k < r.var.size(); ++k) {
+        Group* g = groups_[r.group[k]];
+        float v = 0.f;
+        if (r.var[k] != "spike") {
+          int j = var_idx(*g, r.var[k], r.port[k]);
+          HIPCHK(hipMemcpy(&v, g->var.p + (size_t)j * g->n + r.rel[k], sizeof(float), hipMemcpyDeviceToHost));
+        }
+        row.push_back(v);
+      }
+      if (r.fp) {
+        fprintf(r.fp, "%f", row[0]);
+        for (size_t k = 1; k < row.size(); ++k) fprintf(r.fp, "\t%f", row[k]);
+        fprintf(r.fp, "\n");
+      }
+      r.data.push_back(std::move(row));
+    }
+    build_real_time_ = wall_time();
+    first_sim_ = false;
+  }
+  neur_t0_ = neural_time_;
+  it_ = 0;
+  nt_ = (long long)std::round(sim_time_ / dt_);
+  upload_poisson_tables();
+  StepCtx c;
+  c.it = 0;
+  c.t_idx = (long long)(int)std::round(neur_t0_ / dt_) + 1;
+  c.t0 = neur_t0_;
+  c.t1 = neur_t0_ + (double)dt_ * 1.0;
+  c.slot = (int)(c.t_idx % n_slot_);
+  c.parity = (int)(c.t_idx & 1);
+  HIPCHK(hipMemcpyAsync(d_ctx_.p, &c, sizeof(c), hipMemcpyHostToDevice, stream_));
+  // record columns and buffer for this call
+  std::vector<RecCol> cols;
+  for (auto& r : records_)
+    for (size_t k = 0; k < r.var.size(); ++k) {
+      Group* g = groups_[r.group[k]];
+      RecCol rc;
+      rc.node = r.node[k];
+      rc.has_out = has_out_[r.node[k]];
+      rc.ptr = nullptr;
+      if (r.var[k] != "spike") {
+        int j = var_idx(*g, r.var[k], r.port[k]);
+        rc.ptr = g->var.p + (size_t)j * g->n + r.rel[k];
+      }
+      cols.push_back(rc);
+    }
+  rec_ncol_ = (int)cols.size();
+  if (rec_ncol_ > 0) {
+    upload(d_rec_cols_, cols, stream_);
+    rec_buf_rows_ = std::max<long long>(std::min<long long>(nt_, 1 << 20), 1);
+    d_rec_buf_.alloc((size_t)rec_buf_rows_ * (rec_ncol_ + 1));
+  }
+  rec_rows_pending_ = 0;
+  rec_row_base_ = 0;
+  return 0;
+}
+
+int Engine::SimulationStep() {
+  if (first_sim_ || !calibrated_) StartSimulation();
+  if (rec_ncol_ > 0 && rec_rows_pending_ >= rec_buf_rows_) {
+    // record buffer full: move its rows to the host and restart it
+    HIPCHK(hipStreamSynchronize(stream_));
+    fetch_records(rec_rows_pending_);
+    rec_rows_pending_ = 0;
+    rec_row_base_ = it_;
+  }
+  launch_step();
+  ++rec_rows_pending_;
+  long long t_idx = (long long)(int)std::round(neur_t0_ / dt_) + it_ + 1;
+  ++it_;
+  neural_time_ = neur_t0_ + (double)dt_ * it_;
+  for (auto* g : groups_)
+    if (g->max_rec > 0 && g->rec_step > 0 && (t_idx % g->rec_step == g->rec_step - 1))
+      buffer_rec_spike_times(*g);
+  return 0;
+}
+
+void Engine::fetch_records(long long n_rows) {
+  if (rec_ncol_ == 0 || n_rows == 0) return;
+  std::vector<float> buf((size_t)n_rows * (rec_ncol_ + 1));
+  HIPCHK(hipMemcpy(buf.data(), d_rec_buf_.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+  int col0 = 1;
+  for (auto& r : records_) {
+    int nc = (int)r.var.size();
+    for (long long row = 0; row < n_rows; ++row) {
+      const float* src = buf.data() + (size_t)row * (rec_ncol_ + 1);
+      std::vector<float> v;
+      v.reserve(nc + 1);
+      v.push_back(src[0]);
+      for (int k = 0; k < nc; ++k) v.push_back(src[col0 + k]);
+      if (r.fp) {
+        fprintf(r.fp, "%f", v[0]);
+        for (int k = 1; k <= nc; ++k) fprintf(r.fp, "\t%f", v[k]);
+        fprintf(r.fp, "\n");
+      }
+      r.data.push_back(std::move(v));
+    }
+    col0 += nc;
+  }
+}
+
+int Engine::EndSimulation() {
+  HIPCHK(hipStreamSynchronize(stream_));
+  fetch_records(rec_rows_pending_);
+  rec_rows_pending_ = 0;
+  double end = wall_time();
+  if (verbosity_ >= 1) {
+    printf("Building time: %g\n", build_real_time_ - start_real_time_);
+    printf("Simulation time: %g\n", end - build_real_time_);
+  }
+  return 0;
+}
+
+int Engine::Simulate() {
+  StartSimulation();
+  if (verbosity_ >= 1) printf("Neural activity simulation time: %.3lf ms\n", (double)sim_time_);
+  for (long long it = 0; it < nt_; ++it) SimulationStep();
+  EndSimulation();
+  return 0;
+}
+
+void Engine::Stats(unsigned long long* out) {
+  for (int k = 0; k < 5; ++k) out[k] = 0;
+  if (!calibrated_) return;
+  HIPCHK(hipStreamSynchronize(stream_));
+  HIPCHK(hipMemcpy(out, d_stats_.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  int ov = 0;
+  HIPCHK(hipMemcpy(&ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
+  out[4] = (unsigned long long)ov;
+}
+
+void Engine::Synchronize() {
+  if (stream_) HIPCHK(hipStreamSynchronize(stream_));
+}
+
+template struct DBuf<float>;
+template struct DBuf<double>;
+template struct DBuf<int>;
+template struct DBuf<long long>;
+template struct DBuf<unsigned int>;
+template struct DBuf<unsigned short>;
+template struct DBuf<unsigned long long>;
+template struct DBuf<StepCtx>;
+template struct DBuf<RecCol>;
+
+}  // namespace ngpu

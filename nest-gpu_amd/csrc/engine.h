@@ -1,0 +1,290 @@
+// Host engine of the MI355X simulation step: node populations, connection
+// store, calibration (CSR build, channel map, coefficient tables), and the
+// per-step launch sequence.  It plays the role of the reference's NESTGPU
+// class (src/nestgpu.h:106-718) behind the same C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace ngpu {
+
+// Library error, mapped to error code 2 by the C ABI (src/propagate_error.h).
+class Error : public std::exception {
+ public:
+  explicit Error(std::string m) : msg_(std::move(m)) {}
+  const char* what() const noexcept override { return msg_.c_str(); }
+
+ private:
+  std::string msg_;
+};
+
+struct ModelDesc {
+  const char* name;
+  int id;
+  std::vector<std::string> scal_var, port_var, scal_param, port_param,
+      group_param, array_param;
+  int forced_port;        // -1: chosen by the caller
+  bool rk5;
+  int in_var;             // input variable (port-block index if in_port_block)
+  bool in_port_block;
+  int w_param;            // port-param index of the input weight, -1 => 1
+};
+
+const ModelDesc* find_model(const std::string& name);
+
+// Device buffer owned by the engine.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count);
+  void free();
+  ~DBuf() { free(); }
+};
+
+struct Group {
+  const ModelDesc* md = nullptr;
+  int i0 = 0, n = 0, n_port = 0, nv = 0, np = 0;
+  DBuf<float> var, param;
+  DBuf<double> x;
+  DBuf<float> h;
+  std::vector<float> uni_val;   // value of each parameter if uniform
+  std::vector<char> uni;        // parameter is equal over the population
+  float h_min_rel = 1.0e-3f, h0_rel = 1.0e-2f, h_min = 0.f, h0 = 0.f;
+  long long ch_base = 0;        // first input channel
+  // spike generator
+  std::vector<std::vector<float>> sg_times, sg_heights;
+  DBuf<int> sg_off, sg_tidx, sg_next;
+  DBuf<float> sg_h;
+  // spike count / spike-time recording
+  std::vector<std::string> int_var;
+  DBuf<int> spike_count, n_rec;
+  DBuf<float> rec_times;
+  int max_rec = 0, rec_step = 0;
+  std::vector<std::vector<float>> rec_host;   // buffered spike times
+  std::vector<int> rec_n_host;
+  std::vector<float*> rec_pt_host;
+  // poisson input edges into this population, CSR by local channel
+  DBuf<int> pe_off, pe_gen;
+  DBuf<float> pe_w, pe_delay;
+  DBuf<unsigned int> pe_stream;
+  bool has_pe = false;
+  // poisson generator: index of the first generator in the global tables
+  int gen_base = 0;
+};
+
+struct RecordDef {
+  std::string file;
+  std::vector<int> group, rel, port, node;
+  std::vector<std::string> var;
+  std::vector<std::vector<float>> data;
+  std::vector<float*> rows;     // returned by GetRecordData
+  FILE* fp = nullptr;
+};
+
+struct ConnSpecS {
+  int rule = 1;   // all_to_all
+  int total_num = 0, indegree = 0, outdegree = 0;
+};
+
+struct SynSpecS {
+  int syn_group = 0, port = 0;
+  float weight = 0.f, delay = 0.f;
+  float* weight_array = nullptr;
+  float* delay_array = nullptr;
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+
+  // ---- kernel parameters (src/nestgpu.cu:64-99,1705-1947)
+  int SetRandomSeed(unsigned long long seed);
+  int SetTimeResolution(float dt);
+  float GetTimeResolution() const { return dt_; }
+  int SetMaxSpikeBufferSize(int m) { max_spike_buffer_size_ = m; return 0; }
+  int GetMaxSpikeBufferSize() const { return max_spike_buffer_size_; }
+  int SetSimTime(float t) { sim_time_ = t; return 0; }
+  int SetVerbosityLevel(int v) { verbosity_ = v; return 0; }
+  std::vector<std::string> FloatParamNames() const;
+  std::vector<std::string> IntParamNames() const;
+  std::vector<std::string> BoolParamNames() const;
+  float GetFloatParam(const std::string& n) const;
+  int SetFloatParam(const std::string& n, float v);
+  int GetIntParam(const std::string& n) const;
+  int SetIntParam(const std::string& n, int v);
+  bool GetBoolParam(const std::string& n) const;
+  int SetBoolParam(const std::string& n, bool v);
+
+  // ---- nodes
+  int Create(const std::string& model, int n, int n_port);
+  int NNodes() const { return n_nodes_; }
+  Group& GroupOf(int node, int n_node, int* rel);
+  std::vector<int> NodeListRel(const int* nodes, int n, Group** g);
+
+  // parameters / variables (src/base_neuron.cu:219-1203)
+  bool IsScalParam(int node, const std::string& n);
+  bool IsPortParam(int node, const std::string& n);
+  bool IsArrayParam(int node, const std::string& n);
+  bool IsGroupParam(int node, const std::string& n);
+  bool IsIntVar(int node, const std::string& n);
+  bool IsScalVar(int node, const std::string& n);
+  bool IsPortVar(int node, const std::string& n);
+  bool IsArrayVar(int node, const std::string& n);
+  int GetParamSize(int node, const std::string& n);
+  int GetVarSize(int node, const std::string& n);
+
+  void SetParam(int i_node, int n, const std::string& name, float val);
+  void SetParam(const int* nodes, int n, const std::string& name, float val);
+  void SetParamArr(int i_node, int n, const std::string& name, const float* v, int size);
+  void SetParamArr(const int* nodes, int n, const std::string& name, const float* v, int size);
+  void SetVar(int i_node, int n, const std::string& name, float val);
+  void SetVar(const int* nodes, int n, const std::string& name, float val);
+  void SetVarArr(int i_node, int n, const std::string& name, const float* v, int size);
+  void SetVarArr(const int* nodes, int n, const std::string& name, const float* v, int size);
+  void SetIntVar(int i_node, int n, const std::string& name, int val);
+  void SetIntVar(const int* nodes, int n, const std::string& name, int val);
+  float* GetParam(int i_node, int n, const std::string& name);
+  float* GetParam(const int* nodes, int n, const std::string& name);
+  float* GetArrayParam(int node, const std::string& name);
+  float* GetVar(int i_node, int n, const std::string& name);
+  float* GetVar(const int* nodes, int n, const std::string& name);
+  float* GetArrayVar(int node, const std::string& name);
+  int* GetIntVar(int i_node, int n, const std::string& name);
+  int* GetIntVar(const int* nodes, int n, const std::string& name);
+  int SetGroupParam(int i_node, int n, const std::string& name, float val);
+  float GetGroupParam(int node, const std::string& name);
+
+  std::vector<std::string> ScalVarNames(int node);
+  std::vector<std::string> PortVarNames(int node);
+  std::vector<std::string> ScalParamNames(int node);
+  std::vector<std::string> PortParamNames(int node);
+  std::vector<std::string> GroupParamNames(int node);
+  std::vector<std::string> ArrayParamNames(int node);
+  std::vector<std::string> ArrayVarNames(int node);
+  std::vector<std::string> IntVarNames(int node);
+
+  // ---- connections (src/connect.cu:38-74, src/connect_rules.h:54-338)
+  int Connect(int src, int tgt, int port, int syn_group, float w, float delay);
+  int ConnectRule(const int* src_list, int i_src0, int n_src, const int* tgt_list,
+                  int i_tgt0, int n_tgt, const ConnSpecS& cs, const SynSpecS& ss);
+  std::vector<long long> GetConnections(const int* src_list, int i_src0, int n_src,
+                                        const int* tgt_list, int i_tgt0, int n_tgt);
+  void GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned char* port,
+                           unsigned char* syn, float* delay, float* w);
+  long long NConnections() const { return (long long)c_src_.size(); }
+
+  // ---- host random numbers (own Philox stream; src/random.cu counterpart)
+  unsigned int* RandomInt(size_t n);
+  float* RandomUniform(size_t n);
+  float* RandomNormal(size_t n, float mean, float sd);
+  float* RandomNormalClipped(size_t n, float mean, float sd, float vmin, float vmax, float vstep);
+
+  // ---- recording
+  int CreateRecord(const std::string& file, char** var_names, const int* nodes,
+                   const int* ports, int n);
+  RecordDef& Record(int i);
+  int ActivateSpikeCount(int i_node, int n);
+  int ActivateRecSpikeTimes(int i_node, int n, int max_n);
+  int SetRecSpikeTimesStep(int i_node, int n, int step);
+  int GetNRecSpikeTimes(int node);
+  int GetRecSpikeTimes(int i_node, int n, int** n_pt, float*** t_pt);
+
+  // ---- simulation (src/nestgpu.cu:294-635)
+  int Calibrate();
+  int Simulate();
+  int StartSimulation();
+  int SimulationStep();
+  int EndSimulation();
+
+  // ---- instrumentation (not part of the reference API)
+  void Stats(unsigned long long* out);   // attempts, rejected, substeps, events, overflow
+  void Synchronize();
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void check_uncalibrated(const char* msg) const;
+  void upload_param_column(Group& g, int j);
+  void set_param_index(Group& g, int j, const std::vector<int>& rel, const float* vals, bool whole);
+  void set_var_index(Group& g, int j, const std::vector<int>& rel, const float* vals);
+  void sort_connections();
+  void build_device_tables();
+  void calibrate_group(Group& g);
+  void upload_poisson_tables();
+  void launch_step();
+  void fetch_records(long long n_rows);
+  void buffer_rec_spike_times(Group& g);
+  int param_idx(const Group& g, const std::string& name, int port) const;
+  int var_idx(const Group& g, const std::string& name, int port) const;
+  void random_block(uint32_t out[4]);
+
+  // kernel params
+  float dt_ = 0.1f;
+  float sim_time_ = 1000.f;
+  float max_spike_num_fact_ = 1.f, max_spike_per_host_fact_ = 1.f;
+  unsigned long long seed_ = 54321ull, kernel_seed_ = 54321ull + 12345ull;
+  int verbosity_ = 4, max_spike_buffer_size_ = 20, remote_spike_height_ = 0;
+  bool print_time_ = false;
+  uint64_t rng_ctr_ = 0;
+
+  std::vector<Group*> groups_;
+  std::vector<std::vector<float>> stage_var_, stage_param_;   // pre-calibration state
+  std::vector<int> node_group_;
+  int n_nodes_ = 0;
+
+  // connection store (insertion order until sorted; sorting is stable by
+  // (source, d_int), the order NetConnection::Insert maintains)
+  std::vector<int> c_src_, c_tgt_, c_d_;
+  std::vector<unsigned char> c_port_, c_syn_;
+  std::vector<float> c_w_;
+  bool sorted_ = true;
+  std::vector<long long> row_ptr_;      // [n_nodes+1]
+  std::vector<long long> grp_ptr_;      // per source: first delay group index
+  std::vector<long long> grp_start_;    // per delay group: first connection
+  std::vector<int> has_out_;
+
+  // device tables
+  DBuf<long long> d_row_ptr_;
+  DBuf<unsigned int> d_syn_ch_;
+  DBuf<float> d_syn_w_;
+  DBuf<unsigned short> d_syn_d_;
+  DBuf<double> d_ring_;
+  long long n_ch_ = 0;
+  int n_slot_ = 2;
+  int max_d_int_ = 0;
+  DBuf<int> d_spk_count_, d_spk_src_, d_overflow_;
+  DBuf<float> d_spk_h_;
+  int spk_cap_ = 0;
+  DBuf<long long> d_last_idx_;
+  DBuf<float> d_last_h_;
+  DBuf<StepCtx> d_ctx_;
+  DBuf<unsigned long long> d_stats_;   // attempts, rejected, substeps, events
+  // poisson generator tables
+  int n_gen_ = 0;
+  DBuf<int> d_gen_chunks_;
+  DBuf<double> d_gen_exp_;
+  DBuf<float> d_gen_origin_, d_gen_start_, d_gen_stop_;
+  // records
+  std::vector<RecordDef> records_;
+  DBuf<RecCol> d_rec_cols_;
+  DBuf<float> d_rec_buf_;
+  int rec_ncol_ = 0;
+
+  hipStream_t stream_ = nullptr;
+  bool calibrated_ = false, first_sim_ = true;
+  double t_min_ = 0.0, neural_time_ = 0.0, neur_t0_ = 0.0;
+  long long it_ = 0, nt_ = 0;
+  long long rec_rows_pending_ = 0;
+  long long rec_row_base_ = 0, rec_buf_rows_ = 0;
+  double build_real_time_ = 0.0, start_real_time_ = 0.0;
+};
+
+}  // namespace ngpu

@@ -1,0 +1,133 @@
+// Host-side launchers of the HIP kernels (kernels.hip).  All launches are
+// asynchronous on the given stream; none synchronises with the host.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace ngpu {
+
+enum ModelId {
+  MODEL_AEIF_COND_BETA_MS = 0,
+  MODEL_AEIF_COND_ALPHA_MS,
+  MODEL_AEIF_PSC_EXP,
+  MODEL_IZH_COND_BETA,
+  MODEL_IAF_PSC_EXP,
+  MODEL_POISSON_GEN,
+  MODEL_SPIKE_GEN,
+  MODEL_SPIKE_DETECTOR,
+  N_MODEL_ID
+};
+
+struct RK5Args {
+  int n;                 // neurons in the population
+  int i0;                // global id of the first neuron
+  float* var;            // SoA [n_var][n]
+  float* param;          // SoA [n_param][n]
+  double* x;             // [n] integration time (FP64, src/rk5.h:180)
+  float* h;              // [n] proposed step
+  ParamTable tab;        // uniform parameters
+  float h_min;
+  float dt;
+  const StepCtx* ctx;
+  SpikeList sl;
+  SpikeBook sb;
+  unsigned long long* stats;  // [3] attempts, rejected, substeps (or null)
+};
+
+struct IafArgs {
+  int n, i0;
+  float* var;
+  float* param;
+  ParamTable tab;
+  float dt;
+  const StepCtx* ctx;
+  SpikeList sl;
+  SpikeBook sb;
+};
+
+struct SpikeGenArgs {
+  int n, i0;
+  const int* off;        // [n+1] spikes per generator (CSR)
+  const int* t_idx;      // spike step indexes (relative to Simulate start)
+  const float* height;
+  int* next;             // [n] next spike to emit
+  const StepCtx* ctx;
+  SpikeList sl;
+  SpikeBook sb;
+};
+
+struct DetectorArgs {
+  int n;
+  float* var;
+  float* param;
+};
+
+struct DeliverArgs {
+  const StepCtx* ctx;
+  SpikeList sl;
+  const long long* row_ptr;    // [n_nodes+1] synapse segment per source
+  const unsigned int* syn_ch;  // [n_syn] target channel
+  const float* syn_w;          // [n_syn] weight
+  const unsigned short* syn_d; // [n_syn] delay slot d_int
+  double* ring;                // [n_slot][n_ch]
+  long long n_ch;
+  int n_slot;
+  unsigned long long* events;  // delivered (spike, synapse) pairs, or null
+};
+
+struct FoldArgs {
+  int n, n_port;
+  long long ch_base;
+  float* var;
+  int in_var;            // var index of port 0's input
+  int in_port_stride;    // var index stride between ports
+  const float* param;
+  ParamTable tab;
+  int w_param;           // param index of port 0's weight, -1 => 1.0
+  int w_port_stride;
+  double* ring;
+  long long n_ch;
+  const StepCtx* ctx;
+  // poisson direct connections into this population, CSR by local channel
+  const int* pe_off;     // [n*n_port+1] or null
+  const int* pe_gen;     // generator index
+  const float* pe_w;
+  const float* pe_delay;
+  const unsigned int* pe_stream;
+  const int* gen_chunks;
+  const double* gen_exp;
+  const float* gen_origin;
+  const float* gen_start;
+  const float* gen_stop;
+  unsigned long long seed;
+};
+
+struct RecCol {
+  const float* ptr;      // value to copy, or null for the "spike" pseudo-variable
+  int node;              // global node (spike activity)
+  int has_out;           // node has outgoing connections
+};
+
+struct RecordArgs {
+  const RecCol* cols;
+  int ncol;
+  float* buf;            // [rows][ncol+1]
+  const StepCtx* ctx;
+  const long long* last_idx;
+  const float* last_height;
+  long long row_base;    // step index (ctx->it) stored in buffer row 0
+  long long max_rows;    // rows in buf
+};
+
+int rk5_supported(int model, int n_port);
+void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);
+void launch_iaf(const IafArgs& a, hipStream_t s);
+void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
+void launch_detector(const DetectorArgs& a, hipStream_t s);
+void launch_deliver(const DeliverArgs& a, int grid, hipStream_t s);
+void launch_fold(const FoldArgs& a, hipStream_t s);
+void launch_record(const RecordArgs& a, hipStream_t s);
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s);
+
+}  // namespace ngpu

@@ -1,0 +1,551 @@
+// HIP kernels of the simulation step for gfx950 (MI355X).
+//
+// One lane per neuron; state and parameters SoA (coalesced 64-lane loads);
+// uniform population parameters come from the kernel-argument table; the
+// adaptive RK5 loop, the exact iaf_psc_exp update, spike emission into a
+// per-step spike list, delivery of the spike list over the source-major
+// delay-sorted synapse table into a FP64 input ring, and the end-of-step
+// fold of ring + Poisson input into the receiving state variables.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "common.h"
+#include "kernels.h"
+#include "philox.h"
+#include "stepfactor.h"
+
+namespace ngpu {
+
+// ---------------------------------------------------------------------------
+// Cash-Karp RK5(4) coefficients (published tableau, same float rounding as
+// the reference's __constant__ float table, src/rk5.cu:35-72).
+namespace ck {
+constexpr float a21 = (float)0.2;
+constexpr float a31 = (float)(3.0 / 40.0), a32 = (float)(9.0 / 40.0);
+constexpr float a41 = (float)0.3, a42 = (float)-0.9, a43 = (float)1.2;
+constexpr float a51 = (float)(-11.0 / 54.0), a52 = (float)2.5,
+                a53 = (float)(-70.0 / 27.0), a54 = (float)(35.0 / 27.0);
+constexpr float a61 = (float)(1631.0 / 55296.0), a62 = (float)(175.0 / 512.0),
+                a63 = (float)(575.0 / 13824.0), a64 = (float)(44275.0 / 110592.0),
+                a65 = (float)(253.0 / 4096.0);
+constexpr float b1 = (float)(37.0 / 378.0), b3 = (float)(250.0 / 621.0),
+                b4 = (float)(125.0 / 594.0), b6 = (float)(512.0 / 1771.0);
+constexpr float e1 = (float)(37.0 / 378.0 - 2825.0 / 27648.0),
+                e3 = (float)(250.0 / 621.0 - 18575.0 / 48384.0),
+                e4 = (float)(125.0 / 594.0 - 13525.0 / 55296.0),
+                e5 = (float)(-277.00 / 14336.0), e6 = (float)(512.0 / 1771.0 - 0.25);
+// step factors err^-0.25 (reject) and err^-0.2 (accept): stepfactor.h
+constexpr float eps = 1.0e-6f, safety = 0.9f, err_min = 1.889568e-4f, scal_min = 0.1f;
+}  // namespace ck
+
+// ---------------------------------------------------------------------------
+// Neuron models (right-hand sides and post-step updates).
+// Parameter and variable indices follow the reference's name tables.
+
+// aeif_cond_beta_multisynapse, src/aeif_cond_beta_multisynapse_kernel.h:165-226
+template <int NP>
+struct AeifCondBeta {
+  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 4 * NP;
+  enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+         t_ref, ref, den };
+  static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
+                       i_jump = b, i_tref = t_ref, i_ref = ref;
+  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+    float I_syn = 0.0f;
+    const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
+    const float V_spike = p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? 0.0f
+                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e]) / p[C_m];
+    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 4 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 4 * i + 2];
+    }
+  }
+};
+
+// aeif_cond_alpha_multisynapse, src/aeif_cond_alpha_multisynapse_kernel.h:163-226
+template <int NP>
+struct AeifCondAlpha {
+  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 3 * NP;
+  enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+         t_ref, ref, den };
+  static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
+                       i_jump = b, i_tref = t_ref, i_ref = ref;
+  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+    float I_syn = 0.0f;
+    const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
+    const float V_spike = p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? 0.0f
+                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e]) / p[C_m];
+    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 3 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 3 * i + 1];
+    }
+  }
+};
+
+// aeif_psc_exp, src/aeif_psc_exp_kernel.h:145-201 (2 fixed ports, I_ex - I_in)
+struct AeifPscExp {
+  static constexpr int NV = 4, NPAR = 16;
+  enum { I_ex, I_in, V_m, w };
+  enum { tau_ex, tau_in, V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak,
+         V_reset, t_ref, ref, den };
+  static constexpr int iV = V_m, iw = w, i_reset = V_reset, i_thr = V_peak,
+                       i_jump = b, i_tref = t_ref, i_ref = ref;
+  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+    float I_syn_tot = 0.0f;
+    I_syn_tot += y[I_ex] - y[I_in];
+    const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[V_m], p[V_peak]);
+    const float V_spike = p[Delta_T] == 0.0f ? 0.0f
+                                             : p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
+    dy[V_m] = (p[ref] > 0) ? 0.0f
+                           : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot - y[w] + p[I_e]) / p[C_m];
+    dy[w] = (p[a] * (V - p[E_L]) - y[w]) / p[tau_w];
+    dy[I_ex] = -y[I_ex] / p[tau_ex];
+    dy[I_in] = -y[I_in] / p[tau_in];
+  }
+};
+
+// izhikevich_cond_beta, src/izhikevich_cond_beta_kernel.h:152-213
+template <int NP>
+struct IzhCondBeta {
+  static constexpr int NV = 2 + 2 * NP, NPAR = 9 + 4 * NP;
+  enum { V_th, a, b, c, d, I_e, t_ref, ref, den };
+  static constexpr int iV = 0, iw = 1, i_reset = c, i_thr = V_th, i_jump = d,
+                       i_tref = t_ref, i_ref = ref;
+  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+    float I_syn = 0.0f;
+    const float V = (p[ref] > 0) ? p[c] : y[0];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[9 + 4 * i] - V);
+    // evaluated in double, as the reference's double literals make it
+    dy[0] = (p[ref] > 0) ? 0.0f
+                         : (float)(0.04 * V * V + 5.0 * V + 140.0 - y[1] + I_syn + p[I_e]);
+    dy[1] = p[a] * (p[b] * V - y[1]);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      dy[3 + 2 * i] = -y[3 + 2 * i] / p[9 + 4 * i + 1];
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[9 + 4 * i + 2];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Spike emission: append to this step's spike list and do the per-node
+// bookkeeping of src/spike_buffer.cu:162-217.
+__device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
+                                  int parity, int node, int rel, float height,
+                                  long long t_idx, double t1) {
+  int pos = atomicAdd(&sl.count[parity], 1);
+  if (pos < sl.cap) {
+    sl.src[(size_t)parity * sl.cap + pos] = node;
+    sl.height[(size_t)parity * sl.cap + pos] = height;
+  } else {
+    atomicAdd(sl.overflow, 1);
+  }
+  sb.last_idx[node] = t_idx;
+  sb.last_height[node] = height;
+  if (sb.spike_count) sb.spike_count[rel]++;
+  if (sb.max_rec > 0) {
+    int k = sb.n_rec[rel];
+    if (k < sb.max_rec - 1) {
+      sb.rec_times[(size_t)rel * sb.max_rec + k] = (float)t1;
+      sb.n_rec[rel] = k + 1;
+    }
+  }
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive RK5 update of one population, src/rk5.h:69-207.
+template <class M>
+__global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < a.n;
+  unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
+  if (active) {
+    const int n = a.n;
+    float y[M::NV], p[M::NPAR];
+#pragma unroll
+    for (int j = 0; j < M::NV; ++j) y[j] = a.var[(size_t)j * n + i];
+#pragma unroll
+    for (int j = 0; j < M::NPAR; ++j)
+      p[j] = a.tab.is_per_neuron(j) ? a.param[(size_t)j * n + i] : a.tab.val[j];
+    double x = a.x[i];
+    float h = a.h[i];
+    const double x1 = a.ctx->t1;
+    const long long t_idx = a.ctx->t_idx;
+    const int parity = a.ctx->parity;
+    const float h_min = a.h_min;
+    const float ref_in = p[M::i_ref];
+
+    bool end_step = false;
+    while (!end_step) {
+      const float h_max = (float)(x1 - x);
+      // ---- RK5Step ----
+      float k1[M::NV], k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV];
+      float yn[M::NV], ys[M::NV];
+      M::deriv(y, k1, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
+      float err;
+      for (;;) {
+        if (h > h_max) h = h_max;
+        if (h < h_min) h = h_min;
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
+        M::deriv(yn, k2, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
+        M::deriv(yn, k3, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j)
+          yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
+        M::deriv(yn, k4, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j)
+          yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
+        M::deriv(yn, k5, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j)
+          yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
+                              ck::a64 * k4[j] + ck::a65 * k5[j]);
+        M::deriv(yn, k6, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j)
+          yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
+        err = 0.0f;
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j) {
+          float v = h * (ck::e1 * k1[j] + ck::e3 * k3[j] + ck::e4 * k4[j] + ck::e5 * k5[j] +
+                         ck::e6 * k6[j]);
+          v /= ys[j];
+          err = fmaxf(err, fabsf(v));
+        }
+        err /= ck::eps;
+        ++n_att;
+        if (err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5)) break;
+        ++n_rej;
+        const float h_new = h * ck::safety * pow_m025(err);
+        h = (float)fmax((double)h_new, 0.1 * (double)h);
+      }
+      ++n_sub;
+      x += h;
+      h = (err > ck::err_min) ? h * ck::safety * pow_m02(err) : 5.0f * h;
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j) y[j] = yn[j];
+      // ---- ExternalUpdate ----
+      end_step = (x >= x1 - (double)h_min);
+      if (y[M::iV] < -1.0e3) {
+        y[M::iV] = p[M::i_reset];
+        y[M::iw] = 0.0f;
+        continue;
+      }
+      if (y[M::iw] < -1.0e6 || y[M::iw] > 1.0e6) {
+        y[M::iV] = p[M::i_reset];
+        y[M::iw] = 0.0f;
+        continue;
+      }
+      if (p[M::i_ref] > 0.0f) {
+        y[M::iV] = p[M::i_reset];
+        if (end_step) p[M::i_ref] -= 1.0f;
+      } else if (y[M::iV] >= p[M::i_thr]) {
+        emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, t_idx, x1);
+        y[M::iV] = p[M::i_reset];
+        y[M::iw] += p[M::i_jump];
+        float r = (float)(int)roundf(p[M::i_tref] / a.dt);
+        p[M::i_ref] = r < 0.0f ? 0.0f : r;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < M::NV; ++j) a.var[(size_t)j * n + i] = y[j];
+    a.x[i] = x;
+    a.h[i] = h;
+    if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
+  }
+  if (a.stats) {
+    unsigned long long s0 = wave_sum(n_att), s1 = wave_sum(n_rej), s2 = wave_sum(n_sub);
+    if ((threadIdx.x & (kWave - 1)) == 0 && s0) {
+      atomicAdd(&a.stats[0], s0);
+      atomicAdd(&a.stats[1], s1);
+      atomicAdd(&a.stats[2], s2);
+    }
+  }
+}
+
+template <class M>
+static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
+  const int block = 256;
+  const int grid = (a.n + block - 1) / block;
+  hipLaunchKernelGGL(rk5_kernel<M>, dim3(grid), dim3(block), 0, s, a);
+}
+
+template <template <int> class M>
+static void dispatch_ports(int n_port, const RK5Args& a, hipStream_t s) {
+  switch (n_port) {
+    case 1: launch_rk5_t<M<1>>(a, s); break;
+    case 2: launch_rk5_t<M<2>>(a, s); break;
+    case 3: launch_rk5_t<M<3>>(a, s); break;
+    case 4: launch_rk5_t<M<4>>(a, s); break;
+    case 5: launch_rk5_t<M<5>>(a, s); break;
+    case 6: launch_rk5_t<M<6>>(a, s); break;
+    case 7: launch_rk5_t<M<7>>(a, s); break;
+    case 8: launch_rk5_t<M<8>>(a, s); break;
+    default: break;
+  }
+}
+
+int rk5_supported(int model, int n_port) {
+  switch (model) {
+    case MODEL_AEIF_COND_BETA_MS:
+    case MODEL_AEIF_COND_ALPHA_MS:
+    case MODEL_IZH_COND_BETA:
+      return n_port >= 1 && n_port <= kMaxPort;
+    case MODEL_AEIF_PSC_EXP:
+      return n_port == 2;
+    default:
+      return 0;
+  }
+}
+
+void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  switch (model) {
+    case MODEL_AEIF_COND_BETA_MS: dispatch_ports<AeifCondBeta>(n_port, a, s); break;
+    case MODEL_AEIF_COND_ALPHA_MS: dispatch_ports<AeifCondAlpha>(n_port, a, s); break;
+    case MODEL_IZH_COND_BETA: dispatch_ports<IzhCondBeta>(n_port, a, s); break;
+    case MODEL_AEIF_PSC_EXP: launch_rk5_t<AeifPscExp>(a, s); break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// iaf_psc_exp exact update, src/iaf_psc_exp.cu:85-110.  HBM-bound: 16 B of
+// state read and written per neuron-step, coefficients from the uniform
+// table.  Contraction is off so the float arithmetic matches the reference
+// operation by operation.
+__global__ void __launch_bounds__(256) iaf_kernel(IafArgs a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  enum { I_ex, I_in, V, REF };
+  enum { tau_m, C_m, E_L, I_e, Theta, V_reset, tau_ex, tau_in, t_ref, den,
+         P20, P11ex, P11in, P21ex, P21in, P22 };
+  auto P = [&](int j) {
+    return a.tab.is_per_neuron(j) ? a.param[(size_t)j * n + i] : a.tab.val[j];
+  };
+  float iex = a.var[(size_t)I_ex * n + i];
+  float iin = a.var[(size_t)I_in * n + i];
+  float v = a.var[(size_t)V * n + i];
+  float ref = a.var[(size_t)REF * n + i];
+  if (ref > 0.0f) {
+    ref -= 1.0f;
+  } else {
+    v = v * P(P22) + iex * P(P21ex) + iin * P(P21in) + P(I_e) * P(P20);
+  }
+  iex *= P(P11ex);
+  iin *= P(P11in);
+  if (v >= P(Theta)) {
+    emit_spike(a.sl, a.sb, a.ctx->parity, a.i0 + i, i, 1.0f, a.ctx->t_idx, a.ctx->t1);
+    v = P(V_reset);
+    ref = (float)(int)roundf(P(t_ref) / a.dt);
+  }
+  a.var[(size_t)I_ex * n + i] = iex;
+  a.var[(size_t)I_in * n + i] = iin;
+  a.var[(size_t)V * n + i] = v;
+  a.var[(size_t)REF * n + i] = ref;
+}
+
+void launch_iaf(const IafArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(iaf_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// spike_generator, src/spike_generator.cu:49-67 (step index relative to the
+// start of the current Simulate call, as the reference passes it_).
+__global__ void spike_gen_kernel(SpikeGenArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int b = a.off[i], e = a.off[i + 1];
+  if (b == e) return;
+  const int is = b + a.next[i];
+  if (is < e && (long long)a.t_idx[is] == a.ctx->it) {
+    emit_spike(a.sl, a.sb, a.ctx->parity, a.i0 + i, i, a.height[is], a.ctx->t_idx, a.ctx->t1);
+    a.next[i]++;
+  }
+}
+
+void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(spike_gen_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+// spike_detector, src/spike_detector.cu:58-79
+__global__ void detector_kernel(DetectorArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float sh = a.var[i];                 // input_spike_height
+  if (sh != 0.0f) {
+    if (a.param[i] == 0.0f) sh = 1.0f;  // hold_spike_height
+    a.var[i] = 0.0f;
+  }
+  a.var[(size_t)a.n + i] = sh;         // spike_height
+}
+
+void launch_detector(const DetectorArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(detector_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Delivery: every spike of this step is expanded over its source's synapse
+// segment (source-major, delay groups ascending) by one wavefront; each
+// synapse adds height*weight into ring slot (t_idx+1+d_int) mod n_slot at its
+// channel.  Replaces the reference's per-node spike ring + send list +
+// per-(spike, group) blocks (src/spike_buffer.cu:247-317,
+// src/get_spike.cu:60-108): the arrival step t_idx+1+d_int is the step whose
+// SpikeBufferUpdate would send the spike in the reference.  FP64 hardware
+// atomics accumulate (double)(height*weight), the reference's accumulator
+// type, so sums of float products are exact and order-independent.
+__global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
+  const int parity = a.ctx->parity;
+  int n_spk = a.sl.count[parity];
+  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int n_waves = gridDim.x * blockDim.x / kWave;
+  int base = (int)((a.ctx->t_idx + 1) % a.n_slot);
+  const int* srcs = a.sl.src + (size_t)parity * a.sl.cap;
+  const float* hts = a.sl.height + (size_t)parity * a.sl.cap;
+  unsigned long long n_ev = 0;
+  for (int s = wave; s < n_spk; s += n_waves) {
+    const int src = srcs[s];
+    const float hgt = hts[s];
+    const long long b = a.row_ptr[src], e = a.row_ptr[src + 1];
+    n_ev += (unsigned long long)(e - b);
+    for (long long k = b + lane; k < e; k += kWave) {
+      const unsigned int ch = a.syn_ch[k];
+      const float w = a.syn_w[k];
+      int slot = base + (int)a.syn_d[k];
+      if (slot >= a.n_slot) slot -= a.n_slot;
+      unsafeAtomicAdd(&a.ring[(size_t)slot * a.n_ch + ch], (double)(hgt * w));
+    }
+  }
+  if (a.events && lane == 0 && n_ev) atomicAdd(a.events, n_ev);
+  // the next step writes the other list: clear its counter now (it was
+  // consumed by the previous step's delivery)
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.sl.count[parity ^ 1] = 0;
+}
+
+void launch_deliver(const DeliverArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(deliver_kernel, dim3(grid), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Fold, src/get_spike.cu:142-166 plus poisson_generator input
+// (src/poiss_gen.cu:55-86): for each (port, neuron) channel,
+//   acc   = ring[slot][ch] + sum_e (double)(w_e * Poisson(lambda_e))
+//   input = (float)((double)input + acc * port_weight)
+// and the ring slot is cleared for reuse.  Poisson edges are sorted by target
+// channel, so each lane sums its own edges (no atomics).  Contraction is off
+// so the double expression rounds exactly as the reference's.
+__global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
+#pragma clang fp contract(off)
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)a.n * a.n_port;
+  if (idx >= total) return;
+  const int port = (int)(idx / a.n);
+  const int i = (int)(idx - (long long)port * a.n);
+  const size_t r = (size_t)a.ctx->slot * a.n_ch + (size_t)(a.ch_base + idx);
+  double acc = a.ring[r];
+  a.ring[r] = 0.0;
+  if (a.pe_off) {
+    const int b = a.pe_off[idx], e = a.pe_off[idx + 1];
+    const double t = a.ctx->t1;
+    const long long t_idx = a.ctx->t_idx;
+    for (int k = b; k < e; ++k) {
+      const int g = a.pe_gen[k];
+      const double t_rel = t - a.gen_origin[g] - a.pe_delay[k];
+      if (t_rel >= a.gen_start[g] && t_rel <= a.gen_stop[g]) {
+        const int cnt = poisson_draw(a.seed, a.pe_stream[k], t_idx, a.gen_chunks[g], a.gen_exp[g]);
+        if (cnt > 0) acc += (double)(a.pe_w[k] * (float)cnt);
+      }
+    }
+  }
+  if (acc == 0.0) return;   // input + 0*w == input exactly
+  float w = 1.0f;
+  if (a.w_param >= 0) {
+    const int j = a.w_param + port * a.w_port_stride;
+    w = a.tab.is_per_neuron(j) ? a.param[(size_t)j * a.n + i] : a.tab.val[j];
+  }
+  float* in = a.var + (size_t)(a.in_var + port * a.in_port_stride) * a.n + i;
+  *in = (float)((double)(*in) + acc * (double)w);
+}
+
+void launch_fold(const FoldArgs& a, hipStream_t s) {
+  const long long total = (long long)a.n * a.n_port;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Multimeter row, src/multimeter.cu:76-110: time column then one value per
+// column; "spike" reads the height emitted in this step (nodes without
+// outgoing connections keep no spike buffer in the reference and read 0,
+// src/base_neuron.cu:1205-1234).
+__global__ void record_kernel(RecordArgs a) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > a.ncol) return;
+  const long long row = a.ctx->it - a.row_base;
+  if (row < 0 || row >= a.max_rows) return;
+  float v;
+  if (c == 0) {
+    v = (float)a.ctx->t1;
+  } else {
+    const RecCol rc = a.cols[c - 1];
+    if (rc.ptr) {
+      v = *rc.ptr;
+    } else {
+      v = (rc.has_out && a.last_idx[rc.node] == a.ctx->t_idx) ? a.last_height[rc.node] : 0.0f;
+    }
+  }
+  a.buf[(size_t)row * (a.ncol + 1) + c] = v;
+}
+
+void launch_record(const RecordArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(record_kernel, dim3((a.ncol + 1 + 255) / 256), dim3(256), 0, s, a);
+}
+
+// Advance the step context (src/nestgpu.cu:486-490 semantics).
+__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot) {
+  StepCtx c = *ctx;
+  c.it += 1;
+  c.t_idx += 1;
+  c.t1 = c.t0 + (double)dt * (double)(c.it + 1);
+  c.slot = (int)(c.t_idx % n_slot);
+  c.parity = (int)(c.t_idx & 1);
+  *ctx = c;
+}
+
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s) {
+  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot);
+}
+
+}  // namespace ngpu

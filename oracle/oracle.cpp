@@ -148,6 +148,9 @@ struct Tab {
                      scal_min = T(1.0e-1);
 };
 
+inline float exp_model(float x);
+inline double exp_model(double x);
+
 // ---------------------------------------------------------------------------
 // Model right-hand sides.  `y` and `p` follow the reference index layout
 // (scalar block, then `port_var`/`port_param` interleaved per port).
@@ -159,7 +162,7 @@ struct AeifCondBeta {  // src/aeif_cond_beta_multisynapse_kernel.h:165-188
     T I_syn = 0;
     T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
     for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
-    T V_spike = p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+    T V_spike = p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[0] = (p[ref] > 0) ? T(0)
                          : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
                             p[I_e]) / p[C_m];
@@ -179,7 +182,7 @@ struct AeifCondAlpha {  // src/aeif_cond_alpha_multisynapse_kernel.h:163-186
     T I_syn = 0;
     T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
     for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
-    T V_spike = p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+    T V_spike = p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[0] = (p[ref] > 0) ? T(0)
                          : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
                             p[I_e]) / p[C_m];
@@ -201,7 +204,7 @@ struct AeifPscExp {  // src/aeif_psc_exp_kernel.h:145-162
     I_syn_tot += y[I_ex] - y[I_in];
     T V = (p[ref] > 0) ? p[V_reset] : std::min(y[V_m], p[V_peak]);
     T V_spike = p[Delta_T] == 0. ? T(0.)
-                                 : p[Delta_T] * std::exp((V - p[V_th]) / p[Delta_T]);
+                                 : p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[V_m] = (p[ref] > 0) ? T(0)
                            : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot -
                               y[w] + p[I_e]) / p[C_m];
@@ -244,6 +247,47 @@ struct StepStats {
   long long rejected = 0;
   long long substeps = 0;   // accepted attempts
 };
+
+// Step factors err^(-1/4) and err^(-1/5) of src/rk5.h:135-153.  The float
+// path evaluates them in double from IEEE basic operations and rounds to
+// float (the correctly rounded power except for double-rounding ties); the
+// FP64 path uses the library pow.  Pinned against pow in tests/test_oracle.py.
+inline double fifth_root_ieee(double x) {
+  int e;
+  double m = std::frexp(x, &e);
+  int q = e / 5, r = e % 5;
+  if (r < 0) { r += 5; q -= 1; }
+  double t = std::ldexp(m, r);
+  double y = 0.75 + 0.12 * t;
+  for (int k = 0; k < 6; ++k) {
+    double y2 = y * y, y4 = y2 * y2;
+    y = (4.0 * y + t / y4) / 5.0;
+  }
+  return std::ldexp(y, q);
+}
+// e^x of the AdEx exponential term: the float path evaluates it in double
+// (Cody-Waite reduction + degree-11 Taylor polynomial) and rounds to float,
+// i.e. the correctly rounded expf up to double-rounding ties; pinned against
+// exp in tests/test_oracle.py.  The FP64 path uses the library exp.
+inline float exp_model(float x) {
+  if (!(x == x)) return x;
+  if (x > 89.0f) return (float)INFINITY;
+  if (x < -104.0f) return 0.0f;
+  double xd = x;
+  double k = std::rint(xd * 1.4426950408889634);
+  double r = xd - k * 0.6931471805599453;
+  const double c[12] = {1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0,
+                        1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
+  double p = c[0];
+  for (int i = 1; i < 12; ++i) p = p * r + c[i];
+  return (float)std::ldexp(p, (int)k);
+}
+inline double exp_model(double x) { return std::exp(x); }
+
+inline float step_factor_dec(float err) { return (float)(1.0 / std::sqrt(std::sqrt((double)err))); }
+inline float step_factor_inc(float err) { return (float)(1.0 / fifth_root_ieee((double)err)); }
+inline double step_factor_dec(double err) { return std::pow(err, -0.25); }
+inline double step_factor_inc(double err) { return std::pow(err, -0.2); }
 
 // src/rk5.h:69-159.  `nv` state variables, k1 outside the retry loop,
 // y_scal computed once from the pre-loop h.
@@ -291,13 +335,13 @@ void rk5_step(int np, int nv, double& x, T* y, T& h, T h_min, T h_max,
     // double comparison as in the reference (1.0e-5 is a double literal)
     if (err <= 1.0 || (double)h <= (double)h_min * (1.0 + 1.0e-5)) break;
     if (st) st->rejected++;
-    T h_new = h * K::coeff * std::pow(err, K::exp_dec);
+    T h_new = h * K::coeff * step_factor_dec(err);
     h = (T)std::max((double)h_new, 0.1 * (double)h);
   }
   if (st) st->substeps++;
   x += h;
   if (err > K::err_min)
-    h = h * K::coeff * std::pow(err, K::exp_inc);
+    h = h * K::coeff * step_factor_inc(err);
   else
     h = (T)(5.0 * (double)h);
   for (int i = 0; i < nv; ++i) y[i] = y_new[i];
@@ -1179,6 +1223,11 @@ int orc_poisson(unsigned long long seed, unsigned int stream, long long t_idx, d
   int m = poisson_chunks(lambda);
   return poisson_draw(seed, stream, t_idx, m, m > 0 ? std::exp(-lambda / m) : 0.0);
 }
+
+// test hooks for the step factors
+float orc_step_factor_dec(float e) { return step_factor_dec(e); }
+float orc_exp_model(float x) { return exp_model(x); }
+float orc_step_factor_inc(float e) { return step_factor_inc(e); }
 
 double orc_propagator_32(double tau_syn, double tau, double C, double h) {
   return propagator_32(tau_syn, tau, C, h);

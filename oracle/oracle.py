@@ -60,6 +60,12 @@ def lib():
         L.orc_propagator_32.restype = ctypes.c_double
         L.orc_propagator_32.argtypes = [ctypes.c_double] * 4
         L.orc_threads.restype = ctypes.c_int
+        L.orc_step_factor_dec.restype = ctypes.c_float
+        L.orc_step_factor_dec.argtypes = [ctypes.c_float]
+        L.orc_step_factor_inc.restype = ctypes.c_float
+        L.orc_step_factor_inc.argtypes = [ctypes.c_float]
+        L.orc_exp_model.restype = ctypes.c_float
+        L.orc_exp_model.argtypes = [ctypes.c_float]
         _lib = L
     return _lib
 
@@ -179,3 +185,15 @@ def propagator_32(tau_syn, tau, C, h):
 
 def threads():
     return lib().orc_threads()
+
+
+def step_factor_dec(e):
+    return lib().orc_step_factor_dec(e)
+
+
+def step_factor_inc(e):
+    return lib().orc_step_factor_inc(e)
+
+
+def exp_model(x):
+    return lib().orc_exp_model(x)

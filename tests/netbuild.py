@@ -1,0 +1,62 @@
+"""Seeded random networks built identically in this build (through the
+nestgpu API) and in the CPU oracle, for parity tests and the benchmark."""
+import numpy as np
+
+
+def random_edges(rng, n_src, n_tgt, k, src0, tgt0, w_lo, w_hi, d_lo, d_hi, dt=0.1):
+    """Fixed indegree k per target, distinct sources, uniform weights and
+    integer delays U{d_lo..d_hi} * dt."""
+    src = np.empty(n_tgt * k, np.int32)
+    for t in range(n_tgt):
+        src[t * k:(t + 1) * k] = rng.choice(n_src, size=k, replace=False)
+    tgt = np.repeat(np.arange(n_tgt, dtype=np.int32), k)
+    w = rng.uniform(w_lo, w_hi, size=n_tgt * k).astype(np.float32)
+    d = (rng.integers(d_lo, d_hi + 1, size=n_tgt * k) * dt).astype(np.float32)
+    return src + src0, tgt + tgt0, w, d
+
+
+def connect_edges_ngpu(ngpu, src, tgt, port, w, d):
+    ngpu.Connect([int(x) for x in src], [int(x) for x in tgt], {"rule": "one_to_one"},
+                 {"weight_array": [float(x) for x in w], "delay_array": [float(x) for x in d],
+                  "receptor": int(port)})
+
+
+def connect_edges_fast(ngpu, src, tgt, port, w, d):
+    """Same as connect_edges_ngpu through the C ABI with numpy buffers."""
+    import ctypes
+    lib = ngpu._nestgpu
+    ngpu.ConnSpecInit()
+    ngpu.SynSpecInit()
+    ngpu.SetConnSpecParam("rule", 0)
+    ngpu.SetSynSpecIntParam("receptor", int(port))
+    w = np.ascontiguousarray(w, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    s = np.ascontiguousarray(src, np.int32)
+    t = np.ascontiguousarray(tgt, np.int32)
+    lib.NESTGPU_SetSynSpecFloatPtParam(b"weight_array", w.ctypes.data_as(ctypes.c_void_p))
+    lib.NESTGPU_SetSynSpecFloatPtParam(b"delay_array", d.ctypes.data_as(ctypes.c_void_p))
+    lib.NESTGPU_ConnectGroupGroup(s.ctypes.data_as(ctypes.c_void_p), len(s),
+                                  t.ctypes.data_as(ctypes.c_void_p), len(t))
+    ngpu._check()
+
+
+# Receptor settings and Poisson weight of the config-1 network: E_rev 0/-85 mV
+# and 1 ms beta time constants as in python/test/test_brunel_list.py:33-37;
+# the Poisson weight (not fixed by the config) gives ~12 Hz at 20 kHz drive.
+C1_RECEPTORS = {"E_rev": [0.0, -85.0], "tau_decay": [1.0, 1.0], "tau_rise": [1.0, 1.0]}
+C1_POISSON_W = 0.2
+
+
+def c1_network(seed=1, n=1000, k=100, t_rate=20000.0, poiss_w=None, iaf=False):
+    """Config-1 shaped network description: n neurons (80% exc), fixed
+    indegree k (80% from exc to port 0, 20% from inh to port 1), weights
+    U(0.5,1.5) exc / U(2,6) inh (positive weight on the inhibitory reversal
+    port), delays U{1..20}*0.1 ms, one Poisson generator per neuron."""
+    rng = np.random.default_rng(seed)
+    ne = int(0.8 * n)
+    ke = int(0.8 * k)
+    ki = k - ke
+    se, te, we, de = random_edges(rng, ne, n, ke, 0, 0, 0.5, 1.5, 1, 20)
+    si, ti, wi, di = random_edges(rng, n - ne, n, ki, ne, 0, 2.0, 6.0, 1, 20)
+    return dict(n=n, ne=ne, exc=(se, te, we, de), inh=(si, ti, wi, di), rate=t_rate,
+                poiss_w=poiss_w)

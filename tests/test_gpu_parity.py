@@ -1,0 +1,270 @@
+"""Parity of the HIP simulation step with the CPU oracle (MI355X only).
+
+All cases run in this one process; nestgpu.ResetKernel() separates the
+networks.  Tolerances: integer/routing work bit-exact; single-neuron RK5
+trajectories within 1e-5 relative of the oracle's float path; iaf_psc_exp and
+Poisson input bit-exact (same float operation sequence, contraction off).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from scenarios import SCENARIOS, NestGpuBackend, OracleBackend, nest_trace, trace_rmse
+import netbuild
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ngpu():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import nestgpu
+    nestgpu.SetVerbosityLevel(0)
+    return nestgpu
+
+
+@pytest.fixture(autouse=True)
+def fresh(ngpu):
+    print("case start", flush=True)
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    yield
+
+
+@pytest.mark.parametrize("name", sorted(SCENARIOS))
+def test_single_neuron_trace(ngpu, name):
+    sc = SCENARIOS[name]
+    t, v = NestGpuBackend(ngpu).run_single(sc)
+    _, vo = OracleBackend(0).run_single(sc)
+    _, v64 = OracleBackend(1).run_single(sc)
+    _, vn = nest_trace(sc["golden"])
+    assert len(v) == len(vo) == 8001
+    # reference acceptance (python/test/test_*.py): rmse vs NEST trace
+    assert trace_rmse(v, vn) < sc["tol"]
+    rel = np.max(np.abs(v - vo) / np.maximum(np.abs(vo), 1e-6))
+    print(name, "max rel diff vs oracle float", rel,
+          "vs FP64", np.max(np.abs(v - v64) / np.abs(v64)))
+    if sc["model"] == "iaf_psc_exp":
+        np.testing.assert_array_equal(v, vo)
+    else:
+        assert rel < 1e-5
+
+
+def test_izhikevich_cond_beta_single(ngpu):
+    from oracle.oracle import Oracle
+    status = {"I_e": 5.0}
+    res = []
+    for be in ("gpu", "orc"):
+        if be == "gpu":
+            n = ngpu.Create("izhikevich_cond_beta", 1, 2)
+            ngpu.SetStatus(n, {"I_e": 5.0, "E_rev": [0.0, -85.0]})
+            sg = ngpu.Create("spike_generator")
+            ngpu.SetStatus(sg, {"spike_times": [20.0, 50.0, 51.0]})
+            ngpu.Connect(sg, n, {"rule": "all_to_all"}, {"receptor": 0, "weight": 2.0, "delay": 1.0})
+            rec = ngpu.CreateRecord("", ["V_m", "u"], [n[0], n[0]], [0, 0])
+            ngpu.Simulate(200.0)
+            d = np.array(ngpu.GetRecordData(rec))
+            res.append((d[:, 1], d[:, 2]))
+        else:
+            o = Oracle(prec=0)
+            n = o.create("izhikevich_cond_beta", 1, 2)
+            o.set(n, 1, "I_e", [5.0])
+            o.set(n, 1, "E_rev", [0.0], 0)
+            o.set(n, 1, "E_rev", [-85.0], 1)
+            sg = o.create("spike_generator", 1, 0)
+            o.set_spike_times(sg, [20.0, 50.0, 51.0])
+            o.connect([sg], [n], [0], [2.0], [1.0])
+            r1, r2 = o.record(n, "V_m"), o.record(n, "u")
+            o.simulate(200.0)
+            res.append((o.record_data(r1)[1], o.record_data(r2)[1]))
+    (vg, ug), (vo, uo) = res
+    assert np.max(np.abs(vg - vo) / np.maximum(np.abs(vo), 1.0)) < 1e-5
+    assert np.max(np.abs(ug - uo) / np.maximum(np.abs(uo), 1.0)) < 1e-5
+
+
+def test_spike_detector_known_answer(ngpu):
+    # python/test/test_spike_detector.py:5-51
+    neuron = ngpu.Create("aeif_cond_beta_multisynapse", 3)
+    ngpu.SetStatus(neuron, {"I_e": 1000.0})
+    sd = ngpu.Create("spike_detector")
+    for k, (w, d) in enumerate([(1.0, 10.0), (2.0, 20.0), (3.0, 30.0)]):
+        ngpu.Connect([neuron[k]], sd, {"rule": "one_to_one"}, {"weight": w, "delay": d, "receptor": 0})
+    rn = ngpu.CreateRecord("", ["spike"], [neuron[0]], [0])
+    rs = ngpu.CreateRecord("", ["spike_height"], [sd[0]], [0])
+    ngpu.Simulate()
+    sn = [r[1] for r in ngpu.GetRecordData(rn)]
+    ss = [r[1] for r in ngpu.GetRecordData(rs)]
+    hits = 0
+    for i in range(len(sn) - 400):
+        if sn[i] > 0.5:
+            hits += 1
+            assert abs(ss[i + 101] - 1.0) < 1e-6
+            assert abs(ss[i + 201] - 2.0) < 1e-6
+            assert abs(ss[i + 301] - 3.0) < 1e-6
+    assert hits > 5
+
+
+def test_routing_brute_force(ngpu):
+    # every (spike, synapse): arrival one delay after emission, on its port,
+    # with weight*height; multiple ports and delay groups per source
+    times = [[1.0, 2.5, 7.0], [3.0, 3.1]]
+    heights = [[1.0, 1.0, 1.0], [0.5, 2.0]]
+    sg = ngpu.Create("spike_generator", 2)
+    det = ngpu.Create("spike_detector", 6)
+    for k in range(2):
+        ngpu.SetStatus([sg[k]], {"spike_times": times[k], "spike_heights": heights[k]})
+    conns = []
+    for k in range(2):
+        for j, d in enumerate([0.1, 0.4, 1.3]):
+            w = 0.25 * (j + 1)
+            conns.append((k, j, w, d))
+            ngpu.Connect([sg[k]], [det[3 * k + j]], {"rule": "one_to_one"}, {"weight": w, "delay": d})
+    rec = ngpu.CreateRecord("", ["spike_height"] * 6, list(range(det[0], det[0] + 6)), [0] * 6)
+    ngpu.Simulate(12.0)
+    data = np.array(ngpu.GetRecordData(rec), dtype=np.float32)
+    for (k, j, w, d) in conns:
+        expect = np.zeros(data.shape[0], np.float32)
+        for ts, hg in zip(times[k], heights[k]):
+            row = int(round(ts / 0.1)) + 1 + (int(round(d / 0.1)) - 1) + 1 + 1
+            expect[row] += np.float32(w) * np.float32(hg)
+        np.testing.assert_array_equal(data[:, 1 + 3 * k + j], expect)
+
+
+def _oracle_network(net, model, sim_ms, prec=0, poiss_w=1.0, status=None):
+    from oracle.oracle import Oracle
+    o = Oracle(prec=prec)
+    n0 = o.create(model, net["n"], 2)
+    for k, v in (status or {}).items():
+        if isinstance(v, list):
+            for p, vv in enumerate(v):
+                o.set(n0, net["n"], k, [vv] * net["n"], port=p)
+        else:
+            o.set(n0, net["n"], k, [v] * net["n"])
+    pg = o.create("poisson_generator", 1, 0)
+    o.set(pg, 1, "rate", [net["rate"]])
+    o.connect(np.full(net["n"], pg, np.int32), np.arange(net["n"], dtype=np.int32) + n0, 0,
+              poiss_w, 0.1)
+    se, te, we, de = net["exc"]
+    si, ti, wi, di = net["inh"]
+    o.connect(se + n0, te + n0, 0, we, de)
+    o.connect(si + n0, ti + n0, 1, wi, di)
+    o.simulate(sim_ms)
+    return o.events()
+
+
+def _gpu_network(ngpu, net, model, sim_ms, poiss_w=1.0, status=None):
+    nrn = ngpu.Create(model, net["n"], 2)
+    if status:
+        ngpu.SetStatus(nrn, status)
+    pg = ngpu.Create("poisson_generator")
+    ngpu.SetStatus(pg, "rate", net["rate"])
+    # oracle and GPU must give each connection the same stream id: same
+    # connection order per source (poisson first, one source)
+    ngpu.Connect(pg, nrn, {"rule": "all_to_all"}, {"weight": poiss_w, "delay": 0.1, "receptor": 0})
+    se, te, we, de = net["exc"]
+    si, ti, wi, di = net["inh"]
+    netbuild.connect_edges_fast(ngpu, se + nrn[0], te + nrn[0], 0, we, de)
+    netbuild.connect_edges_fast(ngpu, si + nrn[0], ti + nrn[0], 1, wi, di)
+    ngpu.ActivateRecSpikeTimes(nrn, 5000)
+    ngpu.Simulate(sim_ms)
+    st = ngpu.GetRecSpikeTimes(nrn)
+    t_idx, node = [], []
+    for i, ts in enumerate(st):
+        for t in ts:
+            t_idx.append(int(round(t / 0.1)))
+            node.append(i + nrn[0])
+    t_idx = np.array(t_idx, np.int64)
+    node = np.array(node, np.int32)
+    o = np.lexsort((node, t_idx))
+    return t_idx[o], node[o]
+
+
+def test_iaf_psc_exp_network_bitexact(ngpu):
+    # iaf_psc_exp + Poisson + delivery: the same float operations in the same
+    # order on both sides, so spike rasters must be identical
+    net = netbuild.c1_network(seed=3, n=600, k=60, t_rate=8000.0)
+    net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2] * 60.0, net["exc"][3])
+    net["inh"] = (net["inh"][0], net["inh"][1], net["inh"][2] * -60.0, net["inh"][3])
+    tg, ng = _gpu_network(ngpu, net, "iaf_psc_exp", 300.0, poiss_w=30.0)
+    to, no = _oracle_network(net, "iaf_psc_exp", 300.0, poiss_w=30.0)
+    print("iaf spikes gpu", len(tg), "oracle", len(to))
+    assert len(to) > 500
+    np.testing.assert_array_equal(tg, to)
+    np.testing.assert_array_equal(ng, no)
+
+
+def test_aeif_network_config1(ngpu):
+    # config-1 network (1000 AdEx, 2 ports, K=100, 20 kHz Poisson), 100 ms:
+    # spike times must match the oracle over the first 100 ms
+    net = netbuild.c1_network(seed=1)
+    status = netbuild.C1_RECEPTORS
+    tg, ng = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", 100.0, poiss_w=netbuild.C1_POISSON_W,
+                          status=status)
+    to, no = _oracle_network(net, "aeif_cond_beta_multisynapse", 100.0, poiss_w=netbuild.C1_POISSON_W,
+                             status=status)
+    print("aeif spikes gpu", len(tg), "oracle", len(to))
+    assert len(to) > 100
+    a = set(zip(tg.tolist(), ng.tolist()))
+    b = set(zip(to.tolist(), no.tolist()))
+    print("aeif mismatched spikes", len(a ^ b))
+    assert a == b
+
+
+def test_spike_count_and_times_consistency(ngpu):
+    # python/test/test_spike_times.py:8-124: spike_count, detector rows and
+    # recorded spike times agree, across two Simulate calls with buffering
+    n_neurons = 30
+    pg = ngpu.Create("poisson_generator")
+    ngpu.SetStatus(pg, "rate", 500.0)
+    neuron = ngpu.Create("aeif_cond_beta_multisynapse", n_neurons)
+    ngpu.ActivateSpikeCount(neuron)
+    ngpu.ActivateRecSpikeTimes(neuron, 500)
+    ngpu.SetRecSpikeTimesStep(neuron, 5)
+    sd = ngpu.Create("spike_detector", n_neurons)
+    ngpu.Connect(pg, neuron, {"rule": "all_to_all"}, {"weight": 4.0, "delay": 0.2})
+    ngpu.Connect(neuron, sd, {"rule": "one_to_one"}, {"weight": 1.0, "delay": 0.1})
+    record = ngpu.CreateRecord("", ["spike_height"] * n_neurons, sd.ToList(), [0] * n_neurons)
+    ngpu.Simulate(490)
+    ngpu.SetStatus(pg, "rate", 0.0)
+    ngpu.Simulate(10)
+    data = ngpu.GetRecordData(record)
+    spike = np.sum(np.array(data)[:, 1:], axis=0)
+    spike_times = [[] for _ in range(n_neurons)]
+    for row in data:
+        for i in range(1, len(row)):
+            if row[i] > 0.5:
+                spike_times[i - 1].append(round(row[0] - 0.2, 4))
+    spike_count = ngpu.GetStatus(neuron, "spike_count")
+    assert sum(c[0] for c in spike_count) > 0
+    for i in range(n_neurons):
+        assert abs(spike[i] - spike_count[i][0]) < 1e-6
+    rec_times = ngpu.GetRecSpikeTimes(neuron)
+    for j in range(n_neurons):
+        assert len(rec_times[j]) == spike_count[j][0]
+        for i, t in enumerate(rec_times[j]):
+            assert abs(round(t, 4) - spike_times[j][i]) < 1e-6
+
+
+def test_two_simulate_calls_continue(ngpu):
+    sc = SCENARIOS["aeif_cond_beta_multisynapse"]
+    from oracle.oracle import Oracle
+    n = ngpu.Create("aeif_cond_beta_multisynapse", 1, 1)
+    ngpu.SetStatus(n, {"I_e": 600.0})
+    rec = ngpu.CreateRecord("", ["V_m"], [n[0]], [0])
+    ngpu.Simulate(30.0)
+    ngpu.Simulate(45.0)
+    v = np.array([r[1] for r in ngpu.GetRecordData(rec)])
+    o = Oracle(prec=0)
+    m = o.create("aeif_cond_beta_multisynapse", 1, 1)
+    o.set(m, 1, "I_e", [600.0])
+    r = o.record(m, "V_m")
+    o.simulate(30.0)
+    o.simulate(45.0)
+    vo = o.record_data(r)[1]
+    assert len(v) == len(vo) == 751
+    assert np.max(np.abs(v - vo) / np.abs(vo)) < 1e-5

@@ -201,3 +201,31 @@ def test_routing_brute_force():
             row = emit_row + 1 + d_int + 1 + 1
             expect[row] += w * hgt
         np.testing.assert_array_equal(v, expect.astype(np.float32))
+
+
+def test_step_factors_are_correctly_rounded_powers():
+    # err^-0.25 and err^-0.2 of the controller (src/rk5.cu:68-70) rounded from
+    # the exact power; compare with mpmath-free high precision: double pow,
+    # whose error (< 1 ulp of double) cannot move the float rounding except
+    # at ties, which the random sample does not hit
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([10.0 ** rng.uniform(-4, 6, 20000), [1.0, 1.889568e-4, 2.0, 1e6, 3.5]])
+    xs = xs.astype(np.float32)
+    for x in xs:
+        d = np.float32(np.float64(x) ** -0.25)
+        i = np.float32(np.float64(x) ** -0.2)
+        assert O.step_factor_dec(float(x)) == d
+        assert O.step_factor_inc(float(x)) == i
+
+
+def test_exp_model_is_correctly_rounded_exp():
+    # the AdEx exponential term (src/aeif_cond_beta_multisynapse_kernel.h:177)
+    # evaluated in double and rounded: equals float32(exp(float64 x)) except
+    # at double-rounding ties, which a random sample does not hit
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([rng.uniform(-30, 10, 30000), rng.uniform(-103, 88, 5000),
+                         [0.0, -0.0, 1.0, -1.0, 88.5, -103.5]]).astype(np.float32)
+    for x in xs:
+        assert O.exp_model(float(x)) == np.float32(np.exp(np.float64(x))), x
+    assert O.exp_model(100.0) == float("inf")
+    assert O.exp_model(-200.0) == 0.0
