@@ -207,10 +207,16 @@ int NESTGPU_SetIntParam(char *param_name, int val);
 int NESTGPU_RemoteCreate(int i_host, char *model_name, int n_neuron, int n_port);
 
 /* ---- instrumentation of this build (no counterpart in the reference) ----
- * out[0..4] = RK5 attempts, rejected attempts, accepted substeps, delivered
- * (spike, synapse) events, spikes dropped by a full spike list; all since
- * calibration.  Returns 0, or -1 before calibration. */
+ * out[0..5] = RK5 attempts, rejected attempts, accepted substeps, delivered
+ * (spike, synapse) events, emitted spikes, spikes dropped by a full spike
+ * list; all since calibration (zeros before). */
 int NESTGPU_AMD_GetStepStats(unsigned long long *out);
+/* Kernel timing with HIP events on the simulator's stream: on != 0 starts
+ * (and zeroes) accumulation.  ms[5]/launches[5] receive the summed device
+ * time and launch count of: RK5 neuron update, iaf_psc_exp update, spike
+ * delivery, fold + Poisson input, other (generators, records, step end). */
+int NESTGPU_AMD_SetKernelTiming(int on);
+int NESTGPU_AMD_GetKernelTimes(double *ms, long long *launches);
 /* Destroys the simulator (all nodes, connections, records, device memory);
  * the next call starts a new one with default settings. */
 int NESTGPU_AMD_ResetKernel();

@@ -112,6 +112,8 @@ Engine::~Engine() {
   for (auto* g : groups_) delete g;
   for (auto& r : records_)
     if (r.fp) fclose(r.fp);
+  for (auto& p : ev_pending_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : ev_free_) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -854,42 +856,41 @@ int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* t
     if (S(k) < 0 || S(k) >= n_nodes_) throw Error("Unrecognized source node");
   for (int k = 0; k < n_tgt; ++k)
     if (T(k) < 0 || T(k) >= n_nodes_) throw Error("Unrecognized target node");
-  // connections are generated as (isn, itn, i_array) triples in the
-  // reference's array order, then appended in that order
-  std::vector<int> ps, pt;
-  std::vector<long long> pa;
+  // Connections are appended in the reference's order: for rule connection
+  // k the source index, target index and synapse-array index i_array are
+  //   one_to_one: (k, k, k); all_to_all: (k % n_src, k / n_src, k)
+  //   (target-major, src/connect_rules.h:173-185); fixed_total_number:
+  //   (r[2k] % n_src, r[2k+1] % n_tgt, k); fixed_indegree: (pick[k], k / deg,
+  //   k); fixed_outdegree: (k / deg, pick[k], k).
+  long long n = 0;
+  int deg = 0;
+  std::vector<int> chosen;
+  unsigned int* rnd = nullptr;
   switch (cs.rule) {
-    case 0:  // one_to_one
+    case 0:
       if (n_src != n_tgt)
         throw Error("Number of source and target nodes must be equal for the one-to-one connection rule");
-      for (int k = 0; k < n_src; ++k) { ps.push_back(k); pt.push_back(k); pa.push_back(k); }
+      n = n_src;
       break;
-    case 1:  // all_to_all: target-major, src/connect_rules.h:173-185
-      ps.reserve((size_t)n_src * n_tgt); pt.reserve((size_t)n_src * n_tgt); pa.reserve((size_t)n_src * n_tgt);
-      for (int it = 0; it < n_tgt; ++it)
-        for (int is = 0; is < n_src; ++is) {
-          ps.push_back(is); pt.push_back(it); pa.push_back((long long)it * n_src + is);
-        }
+    case 1:
+      n = (long long)n_src * n_tgt;
       break;
-    case 2: {  // fixed_total_number
-      unsigned int* r = RandomInt(2 * (size_t)cs.total_num);
-      for (int c = 0; c < cs.total_num; ++c) {
-        ps.push_back(r[2 * c] % n_src); pt.push_back(r[2 * c + 1] % n_tgt); pa.push_back(c);
-      }
-      delete[] r;
+    case 2:
+      n = cs.total_num;
+      rnd = RandomInt(2 * (size_t)cs.total_num);
       break;
-    }
     case 3:    // fixed_indegree: distinct sources per target, drawn per target
     case 4: {  // fixed_outdegree: distinct targets per source, drawn per source
       const bool in = cs.rule == 3;
-      const int deg = in ? cs.indegree : cs.outdegree;
+      deg = in ? cs.indegree : cs.outdegree;
       const int n_pool = in ? n_src : n_tgt;
       const int n_own = in ? n_tgt : n_src;
       if (deg > n_pool)
         throw Error(in ? "Indegree larger than number of source nodes"
                        : "Outdegree larger than number of target nodes");
+      n = (long long)n_own * deg;
       const uint64_t key = seed_ ^ ((uint64_t)(++rng_ctr_) << 40);
-      std::vector<int> chosen((size_t)n_own * deg);
+      chosen.resize(n);
 #pragma omp parallel
       {
         std::vector<char> used(n_pool, 0);
@@ -927,41 +928,48 @@ int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* t
           }
         }
       }
-      size_t tot = (size_t)n_own * deg;
-      ps.resize(tot); pt.resize(tot); pa.resize(tot);
-      for (int o = 0; o < n_own; ++o)
-        for (int i = 0; i < deg; ++i) {
-          size_t k = (size_t)o * deg + i;
-          if (in) { ps[k] = chosen[k]; pt[k] = o; } else { ps[k] = o; pt[k] = chosen[k]; }
-          pa[k] = (long long)k;
-        }
       break;
     }
     default:
       throw Error("Unknown connection rule");
   }
-  size_t n = ps.size();
-  size_t base = c_src_.size();
-  // validate delays first so a bad call leaves the store unchanged
-  for (size_t k = 0; k < n; ++k) {
-    float d = ss.delay_array ? ss.delay_array[pa[k]] : ss.delay;
-    if (d < dt_) throw Error("Delay must be >= time resolution");
-  }
+  const int rule = cs.rule;
+  auto pair_of = [&](long long k, int* is, int* it) {
+    switch (rule) {
+      case 0: *is = (int)k; *it = (int)k; break;
+      case 1: *is = (int)(k % n_src); *it = (int)(k / n_src); break;
+      case 2: *is = (int)(rnd[2 * k] % (unsigned)n_src); *it = (int)(rnd[2 * k + 1] % (unsigned)n_tgt); break;
+      case 3: *is = chosen[k]; *it = (int)(k / deg); break;
+      default: *is = (int)(k / deg); *it = chosen[k]; break;
+    }
+  };
+  const size_t base = c_src_.size();
   c_src_.resize(base + n); c_tgt_.resize(base + n); c_port_.resize(base + n);
   c_syn_.resize(base + n); c_w_.resize(base + n); c_d_.resize(base + n);
-#pragma omp parallel for schedule(static)
-  for (long long k = 0; k < (long long)n; ++k) {
-    float w = ss.weight_array ? ss.weight_array[pa[k]] : ss.weight;
-    float d = ss.delay_array ? ss.delay_array[pa[k]] : ss.delay;
-    c_src_[base + k] = S(ps[k]);
-    c_tgt_[base + k] = T(pt[k]);
+  int bad_delay = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_delay)
+  for (long long k = 0; k < n; ++k) {
+    int is, it;
+    pair_of(k, &is, &it);
+    const float w = ss.weight_array ? ss.weight_array[k] : ss.weight;
+    const float d = ss.delay_array ? ss.delay_array[k] : ss.delay;
+    if (d < dt_) bad_delay |= 1;
+    const int d_int = (int)std::round(d / dt_) - 1;
+    if (d_int > 65535) bad_delay |= 2;
+    c_src_[base + k] = S(is);
+    c_tgt_[base + k] = T(it);
     c_port_[base + k] = (unsigned char)ss.port;
     c_syn_[base + k] = (unsigned char)ss.syn_group;
     c_w_[base + k] = w;
-    c_d_[base + k] = (int)std::round(d / dt_) - 1;
+    c_d_[base + k] = d_int;
   }
-  for (size_t k = 0; k < n; ++k)
-    if (c_d_[base + k] > 65535) throw Error("Delay too large (max 65536 steps)");
+  delete[] rnd;
+  if (bad_delay) {  // leave the store as it was
+    c_src_.resize(base); c_tgt_.resize(base); c_port_.resize(base);
+    c_syn_.resize(base); c_w_.resize(base); c_d_.resize(base);
+    if (bad_delay & 1) throw Error("Delay must be >= time resolution");
+    throw Error("Delay too large (max 65536 steps)");
+  }
   if (n) sorted_ = false;
   return 0;
 }
@@ -1349,8 +1357,8 @@ int Engine::Calibrate() {
   d_last_h_.alloc(std::max(n_nodes_, 1));
   HIPCHK(hipMemsetAsync(d_last_h_.p, 0, d_last_h_.n * sizeof(float), stream_));
   d_ctx_.alloc(1);
-  d_stats_.alloc(4);
-  HIPCHK(hipMemsetAsync(d_stats_.p, 0, 4 * sizeof(unsigned long long), stream_));
+  d_stats_.alloc(5);
+  HIPCHK(hipMemsetAsync(d_stats_.p, 0, 5 * sizeof(unsigned long long), stream_));
   // poisson generator tables and direct edges, src/nestgpu.cu:1190-1226
   n_gen_ = 0;
   for (auto* g : groups_)
@@ -1483,26 +1491,26 @@ void Engine::launch_step() {
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
         a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
         a.stats = d_stats_.p;
-        launch_rk5(g->md->id, g->n_port, a, stream_);
+        timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
       }
       case MODEL_IAF_PSC_EXP: {
         IafArgs a;
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.tab = make_table(*g);
         a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
-        launch_iaf(a, stream_);
+        timed(KT_IAF, [&] { launch_iaf(a, stream_); });
         break;
       }
       case MODEL_SPIKE_GEN: {
         SpikeGenArgs a;
         a.n = g->n; a.i0 = g->i0; a.off = g->sg_off.p; a.t_idx = g->sg_tidx.p; a.height = g->sg_h.p;
         a.next = g->sg_next.p; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
-        launch_spike_gen(a, stream_);
+        timed(KT_OTHER, [&] { launch_spike_gen(a, stream_); });
         break;
       }
       case MODEL_SPIKE_DETECTOR: {
         DetectorArgs a{g->n, g->var.p, g->param.p};
-        launch_detector(a, stream_);
+        timed(KT_OTHER, [&] { launch_detector(a, stream_); });
         break;
       }
       default:
@@ -1512,12 +1520,12 @@ void Engine::launch_step() {
   if (rec_ncol_ > 0) {
     RecordArgs ra{d_rec_cols_.p, rec_ncol_, d_rec_buf_.p, d_ctx_.p, d_last_idx_.p, d_last_h_.p,
                   rec_row_base_, rec_buf_rows_};
-    launch_record(ra, stream_);
+    timed(KT_OTHER, [&] { launch_record(ra, stream_); });
   }
   DeliverArgs da;
   da.ctx = d_ctx_.p; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
   da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_; da.events = d_stats_.p + 3;
-  launch_deliver(da, 1024, stream_);
+  timed(KT_DELIVER, [&] { launch_deliver(da, 1024, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
     FoldArgs fa;
@@ -1541,9 +1549,57 @@ void Engine::launch_step() {
       fa.gen_origin = d_gen_origin_.p; fa.gen_start = d_gen_start_.p; fa.gen_stop = d_gen_stop_.p;
       fa.seed = kernel_seed_;
     }
-    launch_fold(fa, stream_);
+    timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
-  launch_end_step(d_ctx_.p, dt_, n_slot_, stream_);
+  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, stream_); });
+  if (timing_ && ev_pending_.size() >= 4096) flush_timing();
+}
+
+void Engine::timed(int cat, const std::function<void()>& launch) {
+  if (!timing_) {
+    launch();
+    return;
+  }
+  auto get = [&]() {
+    if (!ev_free_.empty()) {
+      hipEvent_t e = ev_free_.back();
+      ev_free_.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+  };
+  EvPair p{get(), get(), cat};
+  HIPCHK(hipEventRecord(p.a, stream_));
+  launch();
+  HIPCHK(hipEventRecord(p.b, stream_));
+  ev_pending_.push_back(p);
+}
+
+void Engine::flush_timing() {
+  if (ev_pending_.empty()) return;
+  HIPCHK(hipEventSynchronize(ev_pending_.back().b));
+  for (auto& p : ev_pending_) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+    kt_ms_[p.cat] += ms;
+    kt_n_[p.cat] += 1;
+    ev_free_.push_back(p.a);
+    ev_free_.push_back(p.b);
+  }
+  ev_pending_.clear();
+}
+
+void Engine::SetKernelTiming(bool on) {
+  if (timing_ && !on) flush_timing();
+  timing_ = on;
+  for (int k = 0; k < KT_N; ++k) { kt_ms_[k] = 0.0; kt_n_[k] = 0; }
+}
+
+void Engine::KernelTimes(double* ms, long long* launches) {
+  flush_timing();
+  for (int k = 0; k < KT_N; ++k) { ms[k] = kt_ms_[k]; launches[k] = kt_n_[k]; }
 }
 
 int Engine::StartSimulation() {
@@ -1656,6 +1712,7 @@ void Engine::fetch_records(long long n_rows) {
 
 int Engine::EndSimulation() {
   HIPCHK(hipStreamSynchronize(stream_));
+  if (timing_) flush_timing();
   fetch_records(rec_rows_pending_);
   rec_rows_pending_ = 0;
   double end = wall_time();
@@ -1675,13 +1732,13 @@ int Engine::Simulate() {
 }
 
 void Engine::Stats(unsigned long long* out) {
-  for (int k = 0; k < 5; ++k) out[k] = 0;
+  for (int k = 0; k < 6; ++k) out[k] = 0;
   if (!calibrated_) return;
   HIPCHK(hipStreamSynchronize(stream_));
-  HIPCHK(hipMemcpy(out, d_stats_.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out, d_stats_.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   int ov = 0;
   HIPCHK(hipMemcpy(&ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
-  out[4] = (unsigned long long)ov;
+  out[5] = (unsigned long long)ov;
 }
 
 void Engine::Synchronize() {

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <exception>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -206,7 +207,12 @@ class Engine {
   int EndSimulation();
 
   // ---- instrumentation (not part of the reference API)
-  void Stats(unsigned long long* out);   // attempts, rejected, substeps, events, overflow
+  void Stats(unsigned long long* out);   // attempts, rejected, substeps, events, spikes, dropped
+  // Kernel timing with HIP events on the simulator stream, per category:
+  // 0 RK5 update, 1 iaf_psc_exp update, 2 delivery, 3 fold+Poisson, 4 other.
+  enum { KT_RK5 = 0, KT_IAF, KT_DELIVER, KT_FOLD, KT_OTHER, KT_N };
+  void SetKernelTiming(bool on);
+  void KernelTimes(double* ms, long long* launches);
   void Synchronize();
   hipStream_t stream() const { return stream_; }
 
@@ -225,6 +231,8 @@ class Engine {
   int param_idx(const Group& g, const std::string& name, int port) const;
   int var_idx(const Group& g, const std::string& name, int port) const;
   void random_block(uint32_t out[4]);
+  void timed(int cat, const std::function<void()>& launch);
+  void flush_timing();
 
   // kernel params
   float dt_ = 0.1f;
@@ -266,7 +274,7 @@ class Engine {
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
   DBuf<StepCtx> d_ctx_;
-  DBuf<unsigned long long> d_stats_;   // attempts, rejected, substeps, events
+  DBuf<unsigned long long> d_stats_;   // attempts, rejected, substeps, events, spikes
   // poisson generator tables
   int n_gen_ = 0;
   DBuf<int> d_gen_chunks_;
@@ -279,6 +287,12 @@ class Engine {
   int rec_ncol_ = 0;
 
   hipStream_t stream_ = nullptr;
+  bool timing_ = false;
+  std::vector<hipEvent_t> ev_free_;
+  struct EvPair { hipEvent_t a, b; int cat; };
+  std::vector<EvPair> ev_pending_;
+  double kt_ms_[KT_N] = {0, 0, 0, 0, 0};
+  long long kt_n_[KT_N] = {0, 0, 0, 0, 0};
   bool calibrated_ = false, first_sim_ = true;
   double t_min_ = 0.0, neural_time_ = 0.0, neur_t0_ = 0.0;
   long long it_ = 0, nt_ = 0;

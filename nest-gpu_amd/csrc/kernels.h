@@ -73,7 +73,7 @@ struct DeliverArgs {
   double* ring;                // [n_slot][n_ch]
   long long n_ch;
   int n_slot;
-  unsigned long long* events;  // delivered (spike, synapse) pairs, or null
+  unsigned long long* events;  // [2]: delivered (spike, synapse) pairs, spikes; or null
 };
 
 struct FoldArgs {

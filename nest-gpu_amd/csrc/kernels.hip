@@ -449,6 +449,8 @@ __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
     }
   }
   if (a.events && lane == 0 && n_ev) atomicAdd(a.events, n_ev);
+  if (a.events && blockIdx.x == 0 && threadIdx.x == 0 && n_spk)
+    atomicAdd(a.events + 1, (unsigned long long)n_spk);
   // the next step writes the other list: clear its counter now (it was
   // consumed by the previous step's delivery)
   if (blockIdx.x == 0 && threadIdx.x == 0) a.sl.count[parity ^ 1] = 0;

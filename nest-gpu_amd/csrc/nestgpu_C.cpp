@@ -603,6 +603,18 @@ int NESTGPU_AMD_GetStepStats(unsigned long long* out) {
     return 0;
   });
 }
+int NESTGPU_AMD_SetKernelTiming(int on) {
+  return guarded(0, [&] {
+    E().SetKernelTiming(on != 0);
+    return 0;
+  });
+}
+int NESTGPU_AMD_GetKernelTimes(double* ms, long long* launches) {
+  return guarded(0, [&] {
+    E().KernelTimes(ms, launches);
+    return 0;
+  });
+}
 int NESTGPU_AMD_ResetKernel() {
   return guarded(0, [&] {
     delete g_engine;
