@@ -130,11 +130,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--cpu-ms", type=float, default=100.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 1 GiB fill and a 1 GiB elementwise multiply (torch) whose "
+                         "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (multi-GPU spike exchange is out of "
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(0)
+
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # fill: writes 1 GiB
+        y = torch.empty_like(x)
+        torch.mul(x, 2.0, out=y)                                       # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        log("copy calibration: fill 1 GiB, then elementwise 1 GiB read + 1 GiB written")
+        del x, y
 
     import nestgpu as ngpu
     ngpu.SetVerbosityLevel(0)

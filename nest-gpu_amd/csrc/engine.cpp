@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #ifdef _OPENMP
@@ -1357,6 +1358,16 @@ int Engine::Calibrate() {
   d_last_h_.alloc(std::max(n_nodes_, 1));
   HIPCHK(hipMemsetAsync(d_last_h_.p, 0, d_last_h_.n * sizeof(float), stream_));
   d_ctx_.alloc(1);
+  d_work_.alloc(std::max<size_t>(groups_.size(), 1));
+  HIPCHK(hipMemsetAsync(d_work_.p, 0, d_work_.n * sizeof(int), stream_));
+  {
+    // persistent RK5 grid: 4 waves per SIMD on every CU
+    int dev = 0, cus = 256;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    rk5_max_blocks_ = cus * 4;
+  }
+  if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
   d_stats_.alloc(5);
   HIPCHK(hipMemsetAsync(d_stats_.p, 0, 5 * sizeof(unsigned long long), stream_));
   // poisson generator tables and direct edges, src/nestgpu.cu:1190-1226
@@ -1479,7 +1490,8 @@ static ParamTable make_table(const Group& g) {
 
 void Engine::launch_step() {
   SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
-  for (auto* g : groups_) {
+  for (size_t gi = 0; gi < groups_.size(); ++gi) {
+    Group* g = groups_[gi];
     SpikeBook sb{d_last_idx_.p, d_last_h_.p, g->spike_count.p, g->rec_times.p, g->n_rec.p,
                  g->max_rec};
     switch (g->md->id) {
@@ -1487,22 +1499,24 @@ void Engine::launch_step() {
       case MODEL_AEIF_COND_ALPHA_MS:
       case MODEL_AEIF_PSC_EXP:
       case MODEL_IZH_COND_BETA: {
-        RK5Args a;
+        RK5Args a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
         a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
         a.stats = d_stats_.p;
+        a.work = rk5_persistent_ ? d_work_.p + gi : nullptr;
+        a.max_blocks = rk5_max_blocks_;
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
       }
       case MODEL_IAF_PSC_EXP: {
-        IafArgs a;
+        IafArgs a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.tab = make_table(*g);
         a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
         timed(KT_IAF, [&] { launch_iaf(a, stream_); });
         break;
       }
       case MODEL_SPIKE_GEN: {
-        SpikeGenArgs a;
+        SpikeGenArgs a{};
         a.n = g->n; a.i0 = g->i0; a.off = g->sg_off.p; a.t_idx = g->sg_tidx.p; a.height = g->sg_h.p;
         a.next = g->sg_next.p; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
         timed(KT_OTHER, [&] { launch_spike_gen(a, stream_); });
@@ -1522,13 +1536,13 @@ void Engine::launch_step() {
                   rec_row_base_, rec_buf_rows_};
     timed(KT_OTHER, [&] { launch_record(ra, stream_); });
   }
-  DeliverArgs da;
+  DeliverArgs da{};
   da.ctx = d_ctx_.p; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
   da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_; da.events = d_stats_.p + 3;
   timed(KT_DELIVER, [&] { launch_deliver(da, 1024, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
-    FoldArgs fa;
+    FoldArgs fa{};
     std::memset(&fa, 0, sizeof(fa));
     fa.n = g->n; fa.n_port = g->n_port; fa.ch_base = g->ch_base; fa.var = g->var.p;
     const ModelDesc& m = *g->md;
@@ -1551,7 +1565,7 @@ void Engine::launch_step() {
     }
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
-  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, stream_); });
+  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, d_work_.p, (int)groups_.size(), stream_); });
   if (timing_ && ev_pending_.size() >= 4096) flush_timing();
 }
 

@@ -274,6 +274,9 @@ class Engine {
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
   DBuf<StepCtx> d_ctx_;
+  DBuf<int> d_work_;                   // per population: persistent-kernel work counter
+  bool rk5_persistent_ = true;
+  int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_stats_;   // attempts, rejected, substeps, events, spikes
   // poisson generator tables
   int n_gen_ = 0;

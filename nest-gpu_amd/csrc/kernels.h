@@ -33,6 +33,8 @@ struct RK5Args {
   SpikeList sl;
   SpikeBook sb;
   unsigned long long* stats;  // [3] attempts, rejected, substeps (or null)
+  int* work;             // work counter of the persistent kernel (zero at step start), or null
+  int max_blocks;        // grid limit of the persistent kernel
 };
 
 struct IafArgs {
@@ -128,6 +130,6 @@ void launch_detector(const DetectorArgs& a, hipStream_t s);
 void launch_deliver(const DeliverArgs& a, int grid, hipStream_t s);
 void launch_fold(const FoldArgs& a, hipStream_t s);
 void launch_record(const RecordArgs& a, hipStream_t s);
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s);
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, int* work, int n_work, hipStream_t s);
 
 }  // namespace ngpu

@@ -38,7 +38,13 @@ constexpr float e1 = (float)(37.0 / 378.0 - 2825.0 / 27648.0),
                 e5 = (float)(-277.00 / 14336.0), e6 = (float)(512.0 / 1771.0 - 0.25);
 // step factors err^-0.25 (reject) and err^-0.2 (accept): stepfactor.h
 constexpr float eps = 1.0e-6f, safety = 0.9f, err_min = 1.889568e-4f, scal_min = 0.1f;
+// Bound on RK5 attempts per neuron-step.  Finite runs need at most
+// (dt/h_min + 1) substeps x ~67 rejections each (h shrinks by >= 1/0.9 per
+// rejection down to h_min), i.e. < 70,000 at h_min = dt/1000; the bound only
+// ends the loop of a neuron whose state became non-finite.
+constexpr int kMaxAttempts = 100000;
 }  // namespace ck
+using ck::kMaxAttempts;
 
 // ---------------------------------------------------------------------------
 // Neuron models (right-hand sides and post-step updates).
@@ -195,6 +201,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
     const float ref_in = p[M::i_ref];
 
     bool end_step = false;
+    int att_i = 0;
     while (!end_step) {
       const float h_max = (float)(x1 - x);
       // ---- RK5Step ----
@@ -239,6 +246,10 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
         }
         err /= ck::eps;
         ++n_att;
+        if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)
+          x = x1 - h;
+          break;
+        }
         if (err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5)) break;
         ++n_rej;
         const float h_new = h * ck::safety * pow_m025(err);
@@ -288,9 +299,181 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent RK5 update with lane refill (wavefront-ballot compaction).
+//
+// 99.7% of neuron-steps take a single RK5 attempt, a spiking neuron's step
+// ~40.  With one lane per neuron a wave holding one spiking neuron runs ~40
+// attempts with 63 idle lanes.  Here each wave pulls neurons from a work
+// counter; every iteration is one attempt for all busy lanes (preceded by the
+// k1 evaluation for lanes that start a new substep); lanes whose neuron
+// finished the step store it and take the next neuron, 64 lanes at a time
+// with one atomic per wave (ballot + lane rank).  Per neuron the arithmetic is
+// exactly that of rk5_kernel.  UNI: every parameter except refractory_step is
+// uniform over the population, so parameters are read from the kernel
+// argument table into scalar registers.
+template <class M, bool UNI>
+__global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* work) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int n = a.n;
+  const double x1 = a.ctx->t1;
+  const long long t_idx = a.ctx->t_idx;
+  const int parity = a.ctx->parity;
+  const float h_min = a.h_min;
+  float y[M::NV], p[M::NPAR], k1[M::NV], ys[M::NV];
+  double x = 0.0;
+  float h = 0.0f, ref_in = 0.0f;
+  int i = -1;
+  int att_i = 0;   // attempts of the current neuron in this step
+  bool need_k1 = false;
+  bool exhausted = false;
+  unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
+#pragma unroll
+  for (int j = 0; j < M::NPAR; ++j) p[j] = a.tab.val[j];
+  for (;;) {
+    // ---- refill idle lanes: one atomic per wave ----
+    const bool want = (i < 0) && !exhausted;
+    const unsigned long long m = __ballot(want);
+    if (m) {
+      const int cnt = __popcll(m);
+      const int first = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == first) base = atomicAdd(work, cnt);
+      base = __shfl(base, first, kWave);
+      if (base + cnt >= n) exhausted = true;
+      if (want) {
+        const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (idx < n) {
+          i = idx;
+#pragma unroll
+          for (int j = 0; j < M::NV; ++j) y[j] = a.var[(size_t)j * n + i];
+          if (UNI) {
+            p[M::i_ref] = a.param[(size_t)M::i_ref * n + i];
+          } else {
+#pragma unroll
+            for (int j = 0; j < M::NPAR; ++j)
+              if (a.tab.is_per_neuron(j)) p[j] = a.param[(size_t)j * n + i];
+          }
+          ref_in = p[M::i_ref];
+          x = a.x[i];
+          h = a.h[i];
+          need_k1 = true;
+          att_i = 0;
+        }
+      }
+    }
+    if (!__any(i >= 0)) break;
+    if (i >= 0) {
+      // ---- start of a substep: k1 and the error scale (src/rk5.h:83-86) ----
+      if (need_k1) {
+        M::deriv(y, k1, p);
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
+        need_k1 = false;
+      }
+      // ---- one attempt (src/rk5.h:88-131) ----
+      const float h_max = (float)(x1 - x);
+      if (h > h_max) h = h_max;
+      if (h < h_min) h = h_min;
+      float k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV], yn[M::NV];
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
+      M::deriv(yn, k2, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
+      M::deriv(yn, k3, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j)
+        yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
+      M::deriv(yn, k4, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j)
+        yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
+      M::deriv(yn, k5, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j)
+        yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
+                            ck::a64 * k4[j] + ck::a65 * k5[j]);
+      M::deriv(yn, k6, p);
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j)
+        yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
+      float err = 0.0f;
+#pragma unroll
+      for (int j = 0; j < M::NV; ++j) {
+        float v = h * (ck::e1 * k1[j] + ck::e3 * k3[j] + ck::e4 * k4[j] + ck::e5 * k5[j] +
+                       ck::e6 * k6[j]);
+        v /= ys[j];
+        err = fmaxf(err, fabsf(v));
+      }
+      err /= ck::eps;
+      ++n_att;
+      if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)
+        x = x1;
+        err = 0.0f;
+      }
+      if (!(err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5))) {
+        ++n_rej;   // rejected: shrink h, retry with the same k1 and scale
+        const float h_new = h * ck::safety * pow_m025(err);
+        h = (float)fmax((double)h_new, 0.1 * (double)h);
+      } else {
+        ++n_sub;   // accepted (src/rk5.h:135-159), then ExternalUpdate
+        if (att_i <= kMaxAttempts) x += h;
+        h = (err > ck::err_min) ? h * ck::safety * pow_m02(err) : 5.0f * h;
+#pragma unroll
+        for (int j = 0; j < M::NV; ++j) y[j] = yn[j];
+        need_k1 = true;
+        const bool end_step = (x >= x1 - (double)h_min);
+        if (y[M::iV] < -1.0e3 || y[M::iw] < -1.0e6 || y[M::iw] > 1.0e6) {
+          y[M::iV] = p[M::i_reset];
+          y[M::iw] = 0.0f;
+        } else if (p[M::i_ref] > 0.0f) {
+          y[M::iV] = p[M::i_reset];
+          if (end_step) p[M::i_ref] -= 1.0f;
+        } else if (y[M::iV] >= p[M::i_thr]) {
+          emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, t_idx, x1);
+          y[M::iV] = p[M::i_reset];
+          y[M::iw] += p[M::i_jump];
+          const float r = (float)(int)roundf(p[M::i_tref] / a.dt);
+          p[M::i_ref] = r < 0.0f ? 0.0f : r;
+        }
+        if (end_step) {
+#pragma unroll
+          for (int j = 0; j < M::NV; ++j) a.var[(size_t)j * n + i] = y[j];
+          a.x[i] = x;
+          a.h[i] = h;
+          if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
+          i = -1;
+        }
+      }
+    }
+  }
+  if (a.stats) {
+    unsigned long long s0 = wave_sum(n_att), s1 = wave_sum(n_rej), s2 = wave_sum(n_sub);
+    if (lane == 0 && s0) {
+      atomicAdd(&a.stats[0], s0);
+      atomicAdd(&a.stats[1], s1);
+      atomicAdd(&a.stats[2], s2);
+    }
+  }
+}
+
 template <class M>
 static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
   const int block = 256;
+  if (a.work) {
+    // persistent grid: enough waves to fill the GPU, at most one lane per neuron
+    const int waves_needed = (a.n + kWave - 1) / kWave;
+    int grid = (waves_needed + 3) / 4;
+    if (a.max_blocks > 0 && grid > a.max_blocks) grid = a.max_blocks;
+    uint64_t rest = a.tab.per_neuron[0] & ~(1ull << M::i_ref);
+    const bool uni = rest == 0 && a.tab.per_neuron[1] == 0;
+    if (uni)
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, true>), dim3(grid), dim3(block), 0, s, a, a.work);
+    else
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, false>), dim3(grid), dim3(block), 0, s, a, a.work);
+    return;
+  }
   const int grid = (a.n + block - 1) / block;
   hipLaunchKernelGGL(rk5_kernel<M>, dim3(grid), dim3(block), 0, s, a);
 }
@@ -535,8 +718,10 @@ void launch_record(const RecordArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(record_kernel, dim3((a.ncol + 1 + 255) / 256), dim3(256), 0, s, a);
 }
 
-// Advance the step context (src/nestgpu.cu:486-490 semantics).
-__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot) {
+// Advance the step context (src/nestgpu.cu:486-490 semantics) and clear the
+// work counters of the persistent update kernels.
+__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot, int* work, int n_work) {
+  for (int k = 0; k < n_work; ++k) work[k] = 0;
   StepCtx c = *ctx;
   c.it += 1;
   c.t_idx += 1;
@@ -546,8 +731,8 @@ __global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot) {
   *ctx = c;
 }
 
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s) {
-  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot);
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, int* work, int n_work, hipStream_t s) {
+  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot, work, n_work);
 }
 
 }  // namespace ngpu

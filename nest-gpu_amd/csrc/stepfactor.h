@@ -22,8 +22,10 @@
 
 namespace ngpu {
 
-// x^(-1/4) for x > 0
+// x^(-1/4) for x >= 0 (inf -> 0, NaN -> NaN, as powf)
 NGPU_HD inline float pow_m025(float x) {
+  if (!(x == x)) return x;
+  if (x == INFINITY) return 0.0f;
   const double d = (double)x;
   return (float)(1.0 / sqrt(sqrt(d)));
 }
@@ -32,6 +34,9 @@ NGPU_HD inline float pow_m025(float x) {
 // x^(1/5) = 2^q * t^(1/5) with t = m*2^r in [0.5, 16); Newton's iteration
 // y <- (4y + t/y^4)/5 from a linear start converges in 6 steps.
 NGPU_HD inline float pow_m02(float x) {
+  if (!(x == x)) return x;
+  if (x == INFINITY) return 0.0f;
+  if (x == 0.0f) return INFINITY;
   int e;
   const double m = frexp((double)x, &e);
   int q = e / 5, r = e % 5;

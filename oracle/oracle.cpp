@@ -284,8 +284,17 @@ inline float exp_model(float x) {
 }
 inline double exp_model(double x) { return std::exp(x); }
 
-inline float step_factor_dec(float err) { return (float)(1.0 / std::sqrt(std::sqrt((double)err))); }
-inline float step_factor_inc(float err) { return (float)(1.0 / fifth_root_ieee((double)err)); }
+inline float step_factor_dec(float err) {
+  if (!(err == err)) return err;
+  if (err == INFINITY) return 0.0f;
+  return (float)(1.0 / std::sqrt(std::sqrt((double)err)));
+}
+inline float step_factor_inc(float err) {
+  if (!(err == err)) return err;
+  if (err == INFINITY) return 0.0f;
+  if (err == 0.0f) return INFINITY;
+  return (float)(1.0 / fifth_root_ieee((double)err));
+}
 inline double step_factor_dec(double err) { return std::pow(err, -0.25); }
 inline double step_factor_inc(double err) { return std::pow(err, -0.2); }
 

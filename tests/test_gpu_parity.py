@@ -268,3 +268,27 @@ def test_two_simulate_calls_continue(ngpu):
     vo = o.record_data(r)[1]
     assert len(v) == len(vo) == 751
     assert np.max(np.abs(v - vo) / np.abs(vo)) < 1e-5
+
+
+def test_persistent_rk5_matches_lane_per_neuron_kernel(ngpu):
+    # the persistent (lane-refill) RK5 kernel and the one-lane-per-neuron
+    # kernel run the same per-neuron arithmetic: identical rasters and state
+    net = netbuild.c1_network(seed=7)
+    res = []
+    for mode in ("1", "0"):
+        os.environ["NGPU_RK5_PERSISTENT"] = mode
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            t, n = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", 60.0,
+                                poiss_w=netbuild.C1_POISSON_W, status=netbuild.C1_RECEPTORS)
+            v = np.array(ngpu.GetStatus(ngpu.NodeSeq(0, net["n"]), "V_m"))
+            res.append((t, n, v, ngpu.GetStepStats()))
+        finally:
+            os.environ.pop("NGPU_RK5_PERSISTENT", None)
+    (t1, n1, v1, s1), (t0, n0, v0, s0) = res
+    assert len(t1) > 50
+    np.testing.assert_array_equal(t1, t0)
+    np.testing.assert_array_equal(n1, n0)
+    np.testing.assert_array_equal(v1, v0)
+    assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
