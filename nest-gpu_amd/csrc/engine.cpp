@@ -1358,8 +1358,6 @@ int Engine::Calibrate() {
   d_last_h_.alloc(std::max(n_nodes_, 1));
   HIPCHK(hipMemsetAsync(d_last_h_.p, 0, d_last_h_.n * sizeof(float), stream_));
   d_ctx_.alloc(1);
-  d_work_.alloc(std::max<size_t>(groups_.size(), 1));
-  HIPCHK(hipMemsetAsync(d_work_.p, 0, d_work_.n * sizeof(int), stream_));
   {
     // persistent RK5 grid: 4 waves per SIMD on every CU
     int dev = 0, cus = 256;
@@ -1368,8 +1366,15 @@ int Engine::Calibrate() {
     rk5_max_blocks_ = cus * 4;
   }
   if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
-  d_stats_.alloc(5);
-  HIPCHK(hipMemsetAsync(d_stats_.p, 0, 5 * sizeof(unsigned long long), stream_));
+  // per-wave counter rows: as many as the widest launch that adds to them
+  long long rows = 4LL * kDeliverBlocks;
+  for (auto* g : groups_)
+    if (g->md->rk5)
+      rows = std::max(rows, rk5_waves(g->n, rk5_persistent_ ? 1 : 0, rk5_max_blocks_));
+  d_ctr_.alloc((size_t)rows * kCtrCols);
+  HIPCHK(hipMemsetAsync(d_ctr_.p, 0, d_ctr_.n * sizeof(unsigned long long), stream_));
+  d_stats_.alloc(1);
+  HIPCHK(hipMemsetAsync(d_stats_.p, 0, sizeof(unsigned long long), stream_));
   // poisson generator tables and direct edges, src/nestgpu.cu:1190-1226
   n_gen_ = 0;
   for (auto* g : groups_)
@@ -1502,8 +1507,8 @@ void Engine::launch_step() {
         RK5Args a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
         a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
-        a.stats = d_stats_.p;
-        a.work = rk5_persistent_ ? d_work_.p + gi : nullptr;
+        a.ctr = d_ctr_.p;
+        a.persistent = rk5_persistent_ ? 1 : 0;
         a.max_blocks = rk5_max_blocks_;
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
@@ -1538,8 +1543,9 @@ void Engine::launch_step() {
   }
   DeliverArgs da{};
   da.ctx = d_ctx_.p; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
-  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_; da.events = d_stats_.p + 3;
-  timed(KT_DELIVER, [&] { launch_deliver(da, 1024, stream_); });
+  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_;
+  da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
+  timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
     FoldArgs fa{};
@@ -1565,7 +1571,7 @@ void Engine::launch_step() {
     }
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
-  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, d_work_.p, (int)groups_.size(), stream_); });
+  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, stream_); });
   if (timing_ && ev_pending_.size() >= 4096) flush_timing();
 }
 
@@ -1749,7 +1755,12 @@ void Engine::Stats(unsigned long long* out) {
   for (int k = 0; k < 6; ++k) out[k] = 0;
   if (!calibrated_) return;
   HIPCHK(hipStreamSynchronize(stream_));
-  HIPCHK(hipMemcpy(out, d_stats_.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> rows(d_ctr_.n);
+  HIPCHK(hipMemcpy(rows.data(), d_ctr_.p, rows.size() * sizeof(unsigned long long),
+                   hipMemcpyDeviceToHost));
+  for (size_t r = 0; r < rows.size(); r += kCtrCols)
+    for (int k = 0; k < kCtrCols; ++k) out[k] += rows[r + k];
+  HIPCHK(hipMemcpy(&out[4], d_stats_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
   int ov = 0;
   HIPCHK(hipMemcpy(&ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
   out[5] = (unsigned long long)ov;

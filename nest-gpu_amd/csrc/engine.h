@@ -274,10 +274,10 @@ class Engine {
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
   DBuf<StepCtx> d_ctx_;
-  DBuf<int> d_work_;                   // per population: persistent-kernel work counter
-  bool rk5_persistent_ = true;
+  bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
   int rk5_max_blocks_ = 1024;
-  DBuf<unsigned long long> d_stats_;   // attempts, rejected, substeps, events, spikes
+  DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]
+  DBuf<unsigned long long> d_stats_;   // delivered spikes
   // poisson generator tables
   int n_gen_ = 0;
   DBuf<int> d_gen_chunks_;

@@ -7,6 +7,13 @@
 
 namespace ngpu {
 
+// Per-wave counter slots: each wave of a launch adds its counts into row
+// (global wave index) with plain loads and stores, and the host sums the
+// rows when asked.  Columns: RK5 attempts, rejected attempts, accepted
+// substeps, delivered (spike, synapse) pairs.
+constexpr int kCtrCols = 4;
+constexpr int kDeliverBlocks = 1024;
+
 enum ModelId {
   MODEL_AEIF_COND_BETA_MS = 0,
   MODEL_AEIF_COND_ALPHA_MS,
@@ -32,8 +39,8 @@ struct RK5Args {
   const StepCtx* ctx;
   SpikeList sl;
   SpikeBook sb;
-  unsigned long long* stats;  // [3] attempts, rejected, substeps (or null)
-  int* work;             // work counter of the persistent kernel (zero at step start), or null
+  unsigned long long* ctr;    // per-wave counters [slot][kCtrCols] or null
+  int persistent;        // 1: persistent lane-refill kernel, 0: one lane per neuron
   int max_blocks;        // grid limit of the persistent kernel
 };
 
@@ -75,7 +82,8 @@ struct DeliverArgs {
   double* ring;                // [n_slot][n_ch]
   long long n_ch;
   int n_slot;
-  unsigned long long* events;  // [2]: delivered (spike, synapse) pairs, spikes; or null
+  unsigned long long* ctr;     // per-wave counters [slot][kCtrCols] (column 3) or null
+  unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 
 struct FoldArgs {
@@ -124,12 +132,13 @@ struct RecordArgs {
 
 int rk5_supported(int model, int n_port);
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);
+long long rk5_waves(int n, int persistent, int max_blocks);   // waves of one launch
 void launch_iaf(const IafArgs& a, hipStream_t s);
 void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);
-void launch_deliver(const DeliverArgs& a, int grid, hipStream_t s);
+void launch_deliver(const DeliverArgs& a, hipStream_t s);   // kDeliverBlocks x 256
 void launch_fold(const FoldArgs& a, hipStream_t s);
 void launch_record(const RecordArgs& a, hipStream_t s);
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, int* work, int n_work, hipStream_t s);
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s);
 
 }  // namespace ngpu

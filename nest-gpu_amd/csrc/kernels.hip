@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -49,6 +50,23 @@ using ck::kMaxAttempts;
 // ---------------------------------------------------------------------------
 // Neuron models (right-hand sides and post-step updates).
 // Parameter and variable indices follow the reference's name tables.
+//
+// Every division of a right-hand side is by a parameter (M::divisor(j)).
+// D::div(x, p, r, j) returns x / p[j]: DivIEEE divides; DivRecip, for
+// populations whose divisor parameters are uniform with a usable reciprocal,
+// multiplies by r[j] = RN(1/p[j]) and applies Markstein's remainder
+// correction (div_rn), which gives the same float as the IEEE division with
+// 3 multiply/fma instead of the scaled reciprocal-iteration sequence.
+struct DivIEEE {
+  __device__ static inline float div(float x, const float* p, const float*, int j) {
+    return x / p[j];
+  }
+};
+struct DivRecip {
+  __device__ static inline float div(float x, const float* p, const float* r, int j) {
+    return div_rn(x, p[j], r[j]);
+  }
+};
 
 // aeif_cond_beta_multisynapse, src/aeif_cond_beta_multisynapse_kernel.h:165-226
 template <int NP>
@@ -58,19 +76,24 @@ struct AeifCondBeta {
          t_ref, ref, den };
   static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
                        i_jump = b, i_tref = t_ref, i_ref = ref;
-  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+  static constexpr bool divisor(int j) {
+    return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 4 >= 1 && (j - 14) % 4 <= 2);
+  }
+  template <class D>
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
-    const float V_spike = p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
-    dy[0] = (p[ref] > 0) ? 0.0f
-                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e]) / p[C_m];
-    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    const float V_spike = p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+    dy[0] = (p[ref] > 0)
+                ? 0.0f
+                : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e], p, r, C_m);
+    dy[1] = D::div(p[a] * (V - p[E_L]) - y[1], p, r, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 4 * i + 1];
-      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 4 * i + 2];
+      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 4 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 14 + 4 * i + 2);
     }
   }
 };
@@ -83,19 +106,24 @@ struct AeifCondAlpha {
          t_ref, ref, den };
   static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
                        i_jump = b, i_tref = t_ref, i_ref = ref;
-  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+  static constexpr bool divisor(int j) {
+    return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 3 == 1);
+  }
+  template <class D>
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
-    const float V_spike = p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
-    dy[0] = (p[ref] > 0) ? 0.0f
-                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e]) / p[C_m];
-    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    const float V_spike = p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+    dy[0] = (p[ref] > 0)
+                ? 0.0f
+                : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e], p, r, C_m);
+    dy[1] = D::div(p[a] * (V - p[E_L]) - y[1], p, r, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 3 * i + 1];
-      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 3 * i + 1];
+      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 3 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 14 + 3 * i + 1);
     }
   }
 };
@@ -108,17 +136,23 @@ struct AeifPscExp {
          V_reset, t_ref, ref, den };
   static constexpr int iV = V_m, iw = w, i_reset = V_reset, i_thr = V_peak,
                        i_jump = b, i_tref = t_ref, i_ref = ref;
-  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+  static constexpr bool divisor(int j) {
+    return j == tau_ex || j == tau_in || j == Delta_T || j == C_m || j == tau_w;
+  }
+  template <class D>
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn_tot = 0.0f;
     I_syn_tot += y[I_ex] - y[I_in];
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[V_m], p[V_peak]);
-    const float V_spike = p[Delta_T] == 0.0f ? 0.0f
-                                             : p[Delta_T] * exp_ieee((V - p[V_th]) / p[Delta_T]);
-    dy[V_m] = (p[ref] > 0) ? 0.0f
-                           : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot - y[w] + p[I_e]) / p[C_m];
-    dy[w] = (p[a] * (V - p[E_L]) - y[w]) / p[tau_w];
-    dy[I_ex] = -y[I_ex] / p[tau_ex];
-    dy[I_in] = -y[I_in] / p[tau_in];
+    const float V_spike = p[Delta_T] == 0.0f
+                              ? 0.0f
+                              : p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+    dy[V_m] = (p[ref] > 0)
+                  ? 0.0f
+                  : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot - y[w] + p[I_e], p, r, C_m);
+    dy[w] = D::div(p[a] * (V - p[E_L]) - y[w], p, r, tau_w);
+    dy[I_ex] = D::div(-y[I_ex], p, r, tau_ex);
+    dy[I_in] = D::div(-y[I_in], p, r, tau_in);
   }
 };
 
@@ -129,7 +163,9 @@ struct IzhCondBeta {
   enum { V_th, a, b, c, d, I_e, t_ref, ref, den };
   static constexpr int iV = 0, iw = 1, i_reset = c, i_thr = V_th, i_jump = d,
                        i_tref = t_ref, i_ref = ref;
-  __device__ static inline void deriv(const float* y, float* dy, const float* p) {
+  static constexpr bool divisor(int j) { return j >= 9 && (j - 9) % 4 >= 1 && (j - 9) % 4 <= 2; }
+  template <class D>
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[c] : y[0];
 #pragma unroll
@@ -140,8 +176,8 @@ struct IzhCondBeta {
     dy[1] = p[a] * (p[b] * V - y[1]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = -y[3 + 2 * i] / p[9 + 4 * i + 1];
-      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[9 + 4 * i + 2];
+      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 9 + 4 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 9 + 4 * i + 2);
     }
   }
 };
@@ -177,8 +213,28 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// Adds a wave's counters into its own slot of the per-wave counter array
+// (slot = global wave index of the launch; no two waves of a launch share a
+// slot and launches are stream-ordered, so plain read-add-write suffices and
+// no atomic contends on a single address).
+__device__ inline void add_wave_counters(unsigned long long* ctr, int slot, int lane,
+                                         unsigned long long c0, unsigned long long c1,
+                                         unsigned long long c2) {
+  if (!ctr) return;
+  c0 = wave_sum(c0);
+  c1 = wave_sum(c1);
+  c2 = wave_sum(c2);
+  if (lane == 0 && c0) {
+    unsigned long long* s = ctr + (size_t)slot * kCtrCols;
+    s[0] += c0;
+    s[1] += c1;
+    s[2] += c2;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Adaptive RK5 update of one population, src/rk5.h:69-207.
+// Adaptive RK5 update of one population, src/rk5.h:69-207: one lane per
+// neuron, IEEE divisions (the reference's arithmetic operation by operation).
 template <class M>
 __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -207,7 +263,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
       // ---- RK5Step ----
       float k1[M::NV], k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV];
       float yn[M::NV], ys[M::NV];
-      M::deriv(y, k1, p);
+      M::template deriv<DivIEEE>(y, k1, p, nullptr);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
       float err;
@@ -216,23 +272,23 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
         if (h < h_min) h = h_min;
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
-        M::deriv(yn, k2, p);
+        M::template deriv<DivIEEE>(yn, k2, p, nullptr);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
-        M::deriv(yn, k3, p);
+        M::template deriv<DivIEEE>(yn, k3, p, nullptr);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j)
           yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
-        M::deriv(yn, k4, p);
+        M::template deriv<DivIEEE>(yn, k4, p, nullptr);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j)
           yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
-        M::deriv(yn, k5, p);
+        M::template deriv<DivIEEE>(yn, k5, p, nullptr);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j)
           yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
                               ck::a64 * k4[j] + ck::a65 * k5[j]);
-        M::deriv(yn, k6, p);
+        M::template deriv<DivIEEE>(yn, k6, p, nullptr);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j)
           yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
@@ -289,14 +345,8 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
     a.h[i] = h;
     if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
   }
-  if (a.stats) {
-    unsigned long long s0 = wave_sum(n_att), s1 = wave_sum(n_rej), s2 = wave_sum(n_sub);
-    if ((threadIdx.x & (kWave - 1)) == 0 && s0) {
-      atomicAdd(&a.stats[0], s0);
-      atomicAdd(&a.stats[1], s1);
-      atomicAdd(&a.stats[2], s2);
-    }
-  }
+  add_wave_counters(a.ctr, (blockIdx.x * blockDim.x + threadIdx.x) / kWave,
+                    threadIdx.x & (kWave - 1), n_att, n_rej, n_sub);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,46 +354,51 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 //
 // 99.7% of neuron-steps take a single RK5 attempt, a spiking neuron's step
 // ~40.  With one lane per neuron a wave holding one spiking neuron runs ~40
-// attempts with 63 idle lanes.  Here each wave pulls neurons from a work
-// counter; every iteration is one attempt for all busy lanes (preceded by the
-// k1 evaluation for lanes that start a new substep); lanes whose neuron
-// finished the step store it and take the next neuron, 64 lanes at a time
-// with one atomic per wave (ballot + lane rank).  Per neuron the arithmetic is
-// exactly that of rk5_kernel.  UNI: every parameter except refractory_step is
-// uniform over the population, so parameters are read from the kernel
-// argument table into scalar registers.
+// attempts with 63 idle lanes.  Here each wave owns a contiguous chunk of
+// neurons; every iteration is one attempt for all busy lanes (preceded by
+// the k1 evaluation for lanes that start a new substep), and lanes whose
+// neuron finished the step store it and take the next neurons of the chunk
+// (ballot + lane rank; no atomics).  Per neuron the arithmetic is exactly
+// that of rk5_kernel.  UNI: every parameter except refractory_step is
+// uniform over the population, so parameters come from the kernel-argument
+// table and divisions by them use reciprocals (DivRecip, bit-identical).
 template <class M, bool UNI>
-__global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* work) {
+__global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
+  using D = typename std::conditional<UNI, DivRecip, DivIEEE>::type;
   const int lane = threadIdx.x & (kWave - 1);
+  const int wave = blockIdx.x * (blockDim.x / kWave) +
+                   __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * (blockDim.x / kWave);
   const int n = a.n;
+  const int chunk = ((n + n_waves - 1) / n_waves + kWave - 1) & ~(kWave - 1);
+  const long long c0 = (long long)wave * chunk;
+  int next = c0 < n ? (int)c0 : n;
+  const int end = c0 + chunk < n ? (int)(c0 + chunk) : n;
   const double x1 = a.ctx->t1;
   const long long t_idx = a.ctx->t_idx;
   const int parity = a.ctx->parity;
   const float h_min = a.h_min;
-  float y[M::NV], p[M::NPAR], k1[M::NV], ys[M::NV];
+  const float r_eps = recip_rn(ck::eps);
+  float y[M::NV], p[M::NPAR], r[M::NPAR], k1[M::NV], ys[M::NV];
   double x = 0.0;
   float h = 0.0f, ref_in = 0.0f;
   int i = -1;
   int att_i = 0;   // attempts of the current neuron in this step
   bool need_k1 = false;
-  bool exhausted = false;
   unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
 #pragma unroll
-  for (int j = 0; j < M::NPAR; ++j) p[j] = a.tab.val[j];
+  for (int j = 0; j < M::NPAR; ++j) {
+    p[j] = a.tab.val[j];
+    r[j] = UNI ? recip_rn(p[j]) : 0.0f;
+  }
   for (;;) {
-    // ---- refill idle lanes: one atomic per wave ----
-    const bool want = (i < 0) && !exhausted;
+    // ---- refill idle lanes from the wave's chunk ----
+    const bool want = (i < 0) && next < end;
     const unsigned long long m = __ballot(want);
     if (m) {
-      const int cnt = __popcll(m);
-      const int first = __ffsll((long long)m) - 1;
-      int base = 0;
-      if (lane == first) base = atomicAdd(work, cnt);
-      base = __shfl(base, first, kWave);
-      if (base + cnt >= n) exhausted = true;
       if (want) {
-        const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
-        if (idx < n) {
+        const int idx = next + __popcll(m & ((1ull << lane) - 1ull));
+        if (idx < end) {
           i = idx;
 #pragma unroll
           for (int j = 0; j < M::NV; ++j) y[j] = a.var[(size_t)j * n + i];
@@ -361,12 +416,13 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* wor
           att_i = 0;
         }
       }
+      next += __popcll(m);
     }
     if (!__any(i >= 0)) break;
     if (i >= 0) {
       // ---- start of a substep: k1 and the error scale (src/rk5.h:83-86) ----
       if (need_k1) {
-        M::deriv(y, k1, p);
+        M::template deriv<D>(y, k1, p, r);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
         need_k1 = false;
@@ -378,23 +434,23 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* wor
       float k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV], yn[M::NV];
 #pragma unroll
       for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
-      M::deriv(yn, k2, p);
+      M::template deriv<D>(yn, k2, p, r);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
-      M::deriv(yn, k3, p);
+      M::template deriv<D>(yn, k3, p, r);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j)
         yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
-      M::deriv(yn, k4, p);
+      M::template deriv<D>(yn, k4, p, r);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j)
         yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
-      M::deriv(yn, k5, p);
+      M::template deriv<D>(yn, k5, p, r);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j)
         yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
                             ck::a64 * k4[j] + ck::a65 * k5[j]);
-      M::deriv(yn, k6, p);
+      M::template deriv<D>(yn, k6, p, r);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j)
         yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
@@ -406,7 +462,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* wor
         v /= ys[j];
         err = fmaxf(err, fabsf(v));
       }
-      err /= ck::eps;
+      err = div_rn(err, ck::eps, r_eps);
       ++n_att;
       if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)
         x = x1;
@@ -434,8 +490,8 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* wor
           emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, t_idx, x1);
           y[M::iV] = p[M::i_reset];
           y[M::iw] += p[M::i_jump];
-          const float r = (float)(int)roundf(p[M::i_tref] / a.dt);
-          p[M::i_ref] = r < 0.0f ? 0.0f : r;
+          const float rr = (float)(int)roundf(p[M::i_tref] / a.dt);
+          p[M::i_ref] = rr < 0.0f ? 0.0f : rr;
         }
         if (end_step) {
 #pragma unroll
@@ -448,30 +504,35 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a, int* wor
       }
     }
   }
-  if (a.stats) {
-    unsigned long long s0 = wave_sum(n_att), s1 = wave_sum(n_rej), s2 = wave_sum(n_sub);
-    if (lane == 0 && s0) {
-      atomicAdd(&a.stats[0], s0);
-      atomicAdd(&a.stats[1], s1);
-      atomicAdd(&a.stats[2], s2);
-    }
-  }
+  add_wave_counters(a.ctr, wave, lane, n_att, n_rej, n_sub);
+}
+
+static int persistent_grid(int n, int max_blocks) {
+  const int waves_needed = (n + kWave - 1) / kWave;
+  int grid = (waves_needed + 3) / 4;
+  if (max_blocks > 0 && grid > max_blocks) grid = max_blocks;
+  return grid;
+}
+
+long long rk5_waves(int n, int persistent, int max_blocks) {
+  if (n <= 0) return 0;
+  return persistent ? 4LL * persistent_grid(n, max_blocks) : 4LL * ((n + 255) / 256);
 }
 
 template <class M>
 static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
   const int block = 256;
-  if (a.work) {
+  if (a.persistent) {
     // persistent grid: enough waves to fill the GPU, at most one lane per neuron
-    const int waves_needed = (a.n + kWave - 1) / kWave;
-    int grid = (waves_needed + 3) / 4;
-    if (a.max_blocks > 0 && grid > a.max_blocks) grid = a.max_blocks;
-    uint64_t rest = a.tab.per_neuron[0] & ~(1ull << M::i_ref);
-    const bool uni = rest == 0 && a.tab.per_neuron[1] == 0;
+    const int grid = persistent_grid(a.n, a.max_blocks);
+    const uint64_t rest = a.tab.per_neuron[0] & ~(1ull << M::i_ref);
+    bool uni = rest == 0 && a.tab.per_neuron[1] == 0;
+    for (int j = 0; j < M::NPAR && uni; ++j)
+      if (M::divisor(j) && recip_rn(a.tab.val[j]) == 0.0f) uni = false;
     if (uni)
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, true>), dim3(grid), dim3(block), 0, s, a, a.work);
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, true>), dim3(grid), dim3(block), 0, s, a);
     else
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, false>), dim3(grid), dim3(block), 0, s, a, a.work);
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, false>), dim3(grid), dim3(block), 0, s, a);
     return;
   }
   const int grid = (a.n + block - 1) / block;
@@ -599,48 +660,70 @@ void launch_detector(const DetectorArgs& a, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // Delivery: every spike of this step is expanded over its source's synapse
-// segment (source-major, delay groups ascending) by one wavefront; each
+// segment (source-major, delay groups ascending) by one 256-lane block; each
 // synapse adds height*weight into ring slot (t_idx+1+d_int) mod n_slot at its
 // channel.  Replaces the reference's per-node spike ring + send list +
 // per-(spike, group) blocks (src/spike_buffer.cu:247-317,
 // src/get_spike.cu:60-108): the arrival step t_idx+1+d_int is the step whose
 // SpikeBufferUpdate would send the spike in the reference.  FP64 hardware
 // atomics accumulate (double)(height*weight), the reference's accumulator
-// type, so sums of float products are exact and order-independent.
+// type, so sums of float products are exact and order-independent.  Each
+// lane loads the synapse records of up to 4 synapses before issuing their
+// atomics, so a 1000-synapse segment is one round of loads and adds.
+constexpr int kDeliverUnroll = 4;
+
 __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
   const int parity = a.ctx->parity;
   int n_spk = a.sl.count[parity];
   if (n_spk > a.sl.cap) n_spk = a.sl.cap;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int n_waves = gridDim.x * blockDim.x / kWave;
-  int base = (int)((a.ctx->t_idx + 1) % a.n_slot);
+  const int base = (int)((a.ctx->t_idx + 1) % a.n_slot);
   const int* srcs = a.sl.src + (size_t)parity * a.sl.cap;
   const float* hts = a.sl.height + (size_t)parity * a.sl.cap;
+  const int stride = blockDim.x * kDeliverUnroll;
   unsigned long long n_ev = 0;
-  for (int s = wave; s < n_spk; s += n_waves) {
+  for (int s = blockIdx.x; s < n_spk; s += gridDim.x) {
     const int src = srcs[s];
     const float hgt = hts[s];
     const long long b = a.row_ptr[src], e = a.row_ptr[src + 1];
     n_ev += (unsigned long long)(e - b);
-    for (long long k = b + lane; k < e; k += kWave) {
-      const unsigned int ch = a.syn_ch[k];
-      const float w = a.syn_w[k];
-      int slot = base + (int)a.syn_d[k];
-      if (slot >= a.n_slot) slot -= a.n_slot;
-      unsafeAtomicAdd(&a.ring[(size_t)slot * a.n_ch + ch], (double)(hgt * w));
+    for (long long k0 = b + threadIdx.x; k0 < e; k0 += stride) {
+      unsigned int ch[kDeliverUnroll];
+      float w[kDeliverUnroll];
+      int d[kDeliverUnroll];
+#pragma unroll
+      for (int u = 0; u < kDeliverUnroll; ++u) {
+        const long long k = k0 + (long long)u * blockDim.x;
+        if (k < e) {
+          ch[u] = a.syn_ch[k];
+          w[u] = a.syn_w[k];
+          d[u] = a.syn_d[k];
+        } else {
+          d[u] = -1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kDeliverUnroll; ++u) {
+        if (d[u] < 0) continue;
+        int slot = base + d[u];
+        if (slot >= a.n_slot) slot -= a.n_slot;
+        unsafeAtomicAdd(&a.ring[(size_t)slot * a.n_ch + ch[u]], (double)(hgt * w[u]));
+      }
     }
   }
-  if (a.events && lane == 0 && n_ev) atomicAdd(a.events, n_ev);
-  if (a.events && blockIdx.x == 0 && threadIdx.x == 0 && n_spk)
-    atomicAdd(a.events + 1, (unsigned long long)n_spk);
-  // the next step writes the other list: clear its counter now (it was
-  // consumed by the previous step's delivery)
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.sl.count[parity ^ 1] = 0;
+  if (threadIdx.x == 0) {
+    if (a.ctr && n_ev)
+      a.ctr[(size_t)blockIdx.x * (blockDim.x / kWave) * kCtrCols + 3] += n_ev;
+    if (blockIdx.x == 0) {
+      if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
+      // the next step writes the other list: clear its counter now (it was
+      // consumed by the previous step's delivery)
+      a.sl.count[parity ^ 1] = 0;
+    }
+  }
 }
 
-void launch_deliver(const DeliverArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(deliver_kernel, dim3(grid), dim3(256), 0, s, a);
+void launch_deliver(const DeliverArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(deliver_kernel, dim3(kDeliverBlocks), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,10 +801,8 @@ void launch_record(const RecordArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(record_kernel, dim3((a.ncol + 1 + 255) / 256), dim3(256), 0, s, a);
 }
 
-// Advance the step context (src/nestgpu.cu:486-490 semantics) and clear the
-// work counters of the persistent update kernels.
-__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot, int* work, int n_work) {
-  for (int k = 0; k < n_work; ++k) work[k] = 0;
+// Advance the step context (src/nestgpu.cu:486-490 semantics).
+__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot) {
   StepCtx c = *ctx;
   c.it += 1;
   c.t_idx += 1;
@@ -731,8 +812,8 @@ __global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot, int* work, i
   *ctx = c;
 }
 
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, int* work, int n_work, hipStream_t s) {
-  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot, work, n_work);
+void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s) {
+  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot);
 }
 
 }  // namespace ngpu

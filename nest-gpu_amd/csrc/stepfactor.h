@@ -1,14 +1,9 @@
 // Elementary functions of the RK5 path evaluated from IEEE basic operations
 // only, so every IEEE machine returns the same float (vendor libm expf/powf
-// differ by an ulp between host and device):
-//
-// Step-size factors err^(-1/4) and err^(-1/5) of the RK5 controller
-// (src/rk5.h:135-153, src/rk5.cu:68-70), evaluated in double with IEEE
-// basic operations only (sqrt, division, multiply, add, exact scaling), then
-// rounded to float.  The result is the correctly rounded float power except
-// in ties closer than double rounding, and it is the same on every IEEE
-// machine, so host and device integrate identical step sequences (a libm
-// powf differs between vendors by an ulp).
+// differ by an ulp between host and device): the step-size factors
+// err^(-1/4) and err^(-1/5) of the RK5 controller (src/rk5.h:135-153,
+// src/rk5.cu:68-70), the AdEx exponential, and division by a uniform
+// parameter through its reciprocal.
 #pragma once
 #include <cmath>
 
@@ -22,33 +17,86 @@
 
 namespace ngpu {
 
-// x^(-1/4) for x >= 0 (inf -> 0, NaN -> NaN, as powf)
+// x = m 2^e with m in [1, 2)
+NGPU_HD inline double split_binade(float x, int* e) {
+  int ex;
+  const double m = frexp((double)x, &ex);
+  *e = ex - 1;
+  return 2.0 * m;
+}
+
+// x^(-1/4) for x >= 0 (inf -> 0, 0 -> inf, NaN -> NaN, as powf).  With
+// x = m 2^(4q+r), x^(-1/4) = 2^(-q) 2^(-r/4) m^(-1/4); m^(-1/4) on [1, 2)
+// from a quadratic start (relative error 2.2e-3) and three Newton steps
+// z <- z (5 - m z^4) / 4 for the inverse root (error 2.5 e^2 per step),
+// all in double with +,-,* only.  Equal to the correctly rounded float power
+// for every positive float (exhaustive check against long-double powl).
 NGPU_HD inline float pow_m025(float x) {
   if (!(x == x)) return x;
   if (x == INFINITY) return 0.0f;
-  const double d = (double)x;
-  return (float)(1.0 / sqrt(sqrt(d)));
+  if (x == 0.0f) return INFINITY;
+  int e;
+  const double m = split_binade(x, &e);
+  int q = e / 4, r = e % 4;
+  if (r < 0) { r += 4; q -= 1; }
+  const double c4[4] = {1.0, 0.8408964152537145, 0.7071067811865476, 0.5946035575013605};
+  double z = 1.2852 + m * (-0.35326 + m * 0.06590);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double z2 = z * z;
+    const double z4 = z2 * z2;
+    z = z * ((5.0 - m * z4) * 0.25);
+  }
+  return (float)ldexp(z * c4[r], -q);
 }
 
-// x^(-1/5) for x > 0: x = m*2^e (m in [0.5,1)), e = 5q + r, so
-// x^(1/5) = 2^q * t^(1/5) with t = m*2^r in [0.5, 16); Newton's iteration
-// y <- (4y + t/y^4)/5 from a linear start converges in 6 steps.
+// x^(-1/5) for x >= 0, the same construction: x = m 2^(5q+r), a quadratic
+// start for m^(-1/5) (relative error 1.7e-3) and three Newton steps
+// z <- z (6 - m z^5) / 5.  Correctly rounded for every positive float.
 NGPU_HD inline float pow_m02(float x) {
   if (!(x == x)) return x;
   if (x == INFINITY) return 0.0f;
   if (x == 0.0f) return INFINITY;
   int e;
-  const double m = frexp((double)x, &e);
+  const double m = split_binade(x, &e);
   int q = e / 5, r = e % 5;
   if (r < 0) { r += 5; q -= 1; }
-  const double t = ldexp(m, r);
-  double y = 0.75 + 0.12 * t;
-  for (int k = 0; k < 6; ++k) {
-    const double y2 = y * y;
-    const double y4 = y2 * y2;
-    y = (4.0 * y + t / y4) / 5.0;
+  const double c5[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
+                        0.5743491774985174};
+  double z = 1.2284 + m * (-0.28173 + m * 0.05166);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double z2 = z * z;
+    const double z4 = z2 * z2;
+    const double z5 = z4 * z;
+    z = z * ((6.0 - m * z5) * 0.2);
   }
-  return (float)(1.0 / ldexp(y, q));
+  return (float)ldexp(z * c5[r], -q);
+}
+
+// RN(1/d) for a divisor usable by div_rn (normal, |d| in [2^-32, 2^32]),
+// else 0.  1/d rounded to double and then to float is RN(1/d): the exact
+// reciprocal of a float is never within 2^-49 (relative) of a float midpoint.
+NGPU_HD inline float recip_rn(float d) {
+  const float a = fabsf(d);
+  if (!(a >= 0x1p-32f && a <= 0x1p32f)) return 0.0f;
+  return (float)(1.0 / (double)d);
+}
+
+// a / d correctly rounded, given r = recip_rn(d) != 0: q = RN(a r) is within
+// one ulp of a/d, the remainder a - d q is exact with an fma, and
+// RN(q + rem r) is RN(a/d) (Markstein's theorem).  Exhaustive checks over all
+// 2^32 dividends show the identity for |a| in [2^-64, 2^64) and failures only
+// where the quotient or remainder reaches the subnormal range or a is
+// infinite, so dividends outside that range (zeros, subnormals, inf, NaN)
+// take the IEEE division: one integer range test on the magnitude bits.
+NGPU_HD inline float div_rn(float a, float d, float r) {
+  unsigned int u;
+  __builtin_memcpy(&u, &a, sizeof(u));
+  if ((u & 0x7fffffffu) - 0x1f800000u >= 0x40000000u) return a / d;
+  const float q = a * r;
+  const float rem = fmaf(-q, d, a);
+  return fmaf(rem, r, q);
 }
 
 // e^x for the exponential term of the AdEx models: Cody-Waite reduction

@@ -249,21 +249,37 @@ struct StepStats {
 };
 
 // Step factors err^(-1/4) and err^(-1/5) of src/rk5.h:135-153.  The float
-// path evaluates them in double from IEEE basic operations and rounds to
-// float (the correctly rounded power except for double-rounding ties); the
-// FP64 path uses the library pow.  Pinned against pow in tests/test_oracle.py.
-inline double fifth_root_ieee(double x) {
+// path evaluates them in double from +,-,* only and rounds to float: with
+// x = m 2^(nq+r), m in [1,2), x^(-1/n) = 2^(-q) 2^(-r/n) m^(-1/n), where
+// m^(-1/n) comes from a quadratic start and three Newton steps
+// z <- z (n+1 - m z^n) / n of the inverse root.  The result is the correctly
+// rounded float power (pinned against pow in tests/test_oracle.py); the
+// FP64 path uses the library pow.
+inline double inv_root_ieee(float x, int n) {
   int e;
-  double m = std::frexp(x, &e);
-  int q = e / 5, r = e % 5;
-  if (r < 0) { r += 5; q -= 1; }
-  double t = std::ldexp(m, r);
-  double y = 0.75 + 0.12 * t;
-  for (int k = 0; k < 6; ++k) {
-    double y2 = y * y, y4 = y2 * y2;
-    y = (4.0 * y + t / y4) / 5.0;
+  double m = std::frexp((double)x, &e);
+  m *= 2.0;
+  e -= 1;
+  int q = e / n, r = e % n;
+  if (r < 0) { r += n; q -= 1; }
+  static const double c4[4] = {1.0, 0.8408964152537145, 0.7071067811865476, 0.5946035575013605};
+  static const double c5[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
+                               0.5743491774985174};
+  double z;
+  if (n == 4) {
+    z = 1.2852 + m * (-0.35326 + m * 0.06590);
+    for (int k = 0; k < 3; ++k) {
+      double z2 = z * z, z4 = z2 * z2;
+      z = z * ((5.0 - m * z4) * 0.25);
+    }
+    return std::ldexp(z * c4[r], -q);
   }
-  return std::ldexp(y, q);
+  z = 1.2284 + m * (-0.28173 + m * 0.05166);
+  for (int k = 0; k < 3; ++k) {
+    double z2 = z * z, z4 = z2 * z2, z5 = z4 * z;
+    z = z * ((6.0 - m * z5) * 0.2);
+  }
+  return std::ldexp(z * c5[r], -q);
 }
 // e^x of the AdEx exponential term: the float path evaluates it in double
 // (Cody-Waite reduction + degree-11 Taylor polynomial) and rounds to float,
@@ -287,13 +303,14 @@ inline double exp_model(double x) { return std::exp(x); }
 inline float step_factor_dec(float err) {
   if (!(err == err)) return err;
   if (err == INFINITY) return 0.0f;
-  return (float)(1.0 / std::sqrt(std::sqrt((double)err)));
+  if (err == 0.0f) return INFINITY;
+  return (float)inv_root_ieee(err, 4);
 }
 inline float step_factor_inc(float err) {
   if (!(err == err)) return err;
   if (err == INFINITY) return 0.0f;
   if (err == 0.0f) return INFINITY;
-  return (float)(1.0 / fifth_root_ieee((double)err));
+  return (float)inv_root_ieee(err, 5);
 }
 inline double step_factor_dec(double err) { return std::pow(err, -0.25); }
 inline double step_factor_inc(double err) { return std::pow(err, -0.2); }
@@ -1237,6 +1254,13 @@ int orc_poisson(unsigned long long seed, unsigned int stream, long long t_idx, d
 float orc_step_factor_dec(float e) { return step_factor_dec(e); }
 float orc_exp_model(float x) { return exp_model(x); }
 float orc_step_factor_inc(float e) { return step_factor_inc(e); }
+void orc_step_factors(const float* x, long long n, float* dec, float* inc) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < n; ++i) {
+    dec[i] = step_factor_dec(x[i]);
+    inc[i] = step_factor_inc(x[i]);
+  }
+}
 
 double orc_propagator_32(double tau_syn, double tau, double C, double h) {
   return propagator_32(tau_syn, tau, C, h);

@@ -195,5 +195,17 @@ def step_factor_inc(e):
     return lib().orc_step_factor_inc(e)
 
 
+def step_factors(x):
+    """Vectorised step_factor_dec / step_factor_inc over a float32 array."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    dec = np.empty_like(x)
+    inc = np.empty_like(x)
+    f = lib().orc_step_factors
+    f.restype = None
+    f(x.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(x.size),
+      dec.ctypes.data_as(ctypes.c_void_p), inc.ctypes.data_as(ctypes.c_void_p))
+    return dec, inc
+
+
 def exp_model(x):
     return lib().orc_exp_model(x)

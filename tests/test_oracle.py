@@ -218,6 +218,21 @@ def test_step_factors_are_correctly_rounded_powers():
         assert O.step_factor_inc(float(x)) == i
 
 
+def test_step_factors_strided_sweep_of_all_positive_floats():
+    # every 127th positive float bit pattern (denormals to FLT_MAX, 17M values)
+    # against the double power rounded to float; a tie closer than double
+    # rounding is the only way the two may differ, and none is expected
+    u = np.arange(1, 0x7F800000, 127, dtype=np.uint32)
+    x = u.view(np.float32)
+    dec, inc = O.step_factors(x)
+    xd = x.astype(np.float64)
+    np.testing.assert_array_equal(dec, (xd ** -0.25).astype(np.float32))
+    np.testing.assert_array_equal(inc, (xd ** -0.2).astype(np.float32))
+    d0, i0 = O.step_factors(np.array([0.0, np.inf, np.nan], np.float32))
+    assert d0[0] == np.inf and i0[0] == np.inf and d0[1] == 0 and i0[1] == 0
+    assert np.isnan(d0[2]) and np.isnan(i0[2])
+
+
 def test_exp_model_is_correctly_rounded_exp():
     # the AdEx exponential term (src/aeif_cond_beta_multisynapse_kernel.h:177)
     # evaluated in double and rounded: equals float32(exp(float64 x)) except
