@@ -223,6 +223,40 @@ int NESTGPU_AMD_ResetKernel();
 /* Blocks until all queued device work of the simulator is done. */
 int NESTGPU_AMD_Synchronize();
 
+/* ---- learned step-size predictor (no counterpart in the reference) ----
+ * An MLP 8 -> 32 -> 32 -> 1 proposes each RK5 substep's step size for
+ * aeif_cond_beta_multisynapse populations with uniform parameters (persistent
+ * RK5 kernel); the RK5 error estimate still accepts or rejects every attempt.
+ * Parameter vector (1393 floats): mu[8], inv_sigma[8], W1[32][8], b1[32],
+ * W2[32][32], b2[32], W3[32], b3; output log2(h/dt).  Raw inputs: V, w, four
+ * conductance variables, log2(h_ctrl/dt), log2(last accepted error ratio).
+ * The predictor outlives NESTGPU_AMD_ResetKernel.
+ * mode: 0 controller only, 1 log (state, accepted h) records, 2 learned. */
+int NESTGPU_AMD_SetStepPredictorMode(int mode);
+/* Allocates (and clears) the log: capacity records of 10 floats (8 raw
+ * inputs, target log2(h/dt), censored flag), every substep of the neurons
+ * whose index within their population is a multiple of stride. */
+int NESTGPU_AMD_StartPredictorLog(long long capacity, int stride);
+long long NESTGPU_AMD_GetPredictorLogCount();
+int NESTGPU_AMD_GetPredictorLog(float *out, long long first, long long n);
+int NESTGPU_AMD_AppendPredictorLog(const float *records, long long n);
+/* Input normalisation from the log and He-normal weights (Philox seed). */
+int NESTGPU_AMD_InitPredictor(unsigned long long seed);
+/* steps Adam updates on minibatches of batch records (multiple of 64) drawn
+ * from the log; loss[steps] (or NULL) receives each batch's mean loss. */
+int NESTGPU_AMD_TrainPredictor(int steps, int batch, float lr, unsigned long long seed,
+                               float *loss);
+int NESTGPU_AMD_SetPredictorParams(const float *params, int n);
+int NESTGPU_AMD_GetPredictorParams(float *params, int n);
+/* Inference through the matrix-core path of the RK5 kernel: raw[n][8] ->
+ * out[n] = log2(h/dt). */
+int NESTGPU_AMD_PredictorInfer(const float *raw, long long n, float *out);
+/* Mean censored loss of a batch and its gradient (1377 floats, the
+ * parameters from W1 on) from the training kernels. */
+int NESTGPU_AMD_PredictorLossGrad(const float *raw, const float *target,
+                                  const unsigned char *censored, int n, float *loss,
+                                  float *grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@
 #endif
 
 #include "philox.h"
+#include "predictor_host.h"
 
 namespace ngpu {
 
@@ -1436,6 +1437,10 @@ int Engine::Calibrate() {
       std::vector<float> h(g->n, g->h0);
       upload(g->x, x, stream_);
       upload(g->h, h, stream_);
+      if (rk5_predictor_capable(g->md->id, g->n_port)) {
+        std::vector<float> e(g->n, 1.0f);   // last accepted error ratio, predictor input
+        upload(g->err_last, e, stream_);
+      }
     }
     for (auto& iv : g->int_var) {
       std::vector<int> z(g->n, 0);
@@ -1510,6 +1515,16 @@ void Engine::launch_step() {
         a.ctr = d_ctr_.p;
         a.persistent = rk5_persistent_ ? 1 : 0;
         a.max_blocks = rk5_max_blocks_;
+        if (g->err_last.p && rk5_persistent_) {
+          Predictor& pr = Predictor::get();
+          a.pred_mode = pr.mode();
+          a.err_last = g->err_last.p;
+          a.pred_params = pr.d_params();
+          a.log_buf = pr.d_log();
+          a.log_count = pr.d_log_count();
+          a.log_cap = pr.log_cap();
+          a.log_stride = pr.log_stride();
+        }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
       }

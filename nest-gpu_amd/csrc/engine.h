@@ -56,6 +56,7 @@ struct Group {
   DBuf<float> var, param;
   DBuf<double> x;
   DBuf<float> h;
+  DBuf<float> err_last;         // last accepted RK5 error ratio (step-size predictor input)
   std::vector<float> uni_val;   // value of each parameter if uniform
   std::vector<char> uni;        // parameter is equal over the population
   float h_min_rel = 1.0e-3f, h0_rel = 1.0e-2f, h_min = 0.f, h0 = 0.f;

@@ -42,6 +42,14 @@ struct RK5Args {
   unsigned long long* ctr;    // per-wave counters [slot][kCtrCols] or null
   int persistent;        // 1: persistent lane-refill kernel, 0: one lane per neuron
   int max_blocks;        // grid limit of the persistent kernel
+  // step-size predictor (aeif_cond_beta_multisynapse with uniform parameters)
+  int pred_mode;                   // 0 controller only, 1 log (state, accepted h), 2 learned proposal
+  float* err_last;                 // [n] last accepted error ratio (modes 1, 2)
+  const float* pred_params;        // predictor parameter vector (mode 2)
+  float* log_buf;                  // [log_cap][kLogCols] (mode 1)
+  unsigned long long* log_count;   // records written (may exceed log_cap)
+  long long log_cap;
+  int log_stride;                  // log neurons with index % log_stride == 0
 };
 
 struct IafArgs {
@@ -133,6 +141,7 @@ struct RecordArgs {
 int rk5_supported(int model, int n_port);
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);
 long long rk5_waves(int n, int persistent, int max_blocks);   // waves of one launch
+int rk5_predictor_capable(int model, int n_port);                // model supports pred_mode 1, 2
 void launch_iaf(const IafArgs& a, hipStream_t s);
 void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);

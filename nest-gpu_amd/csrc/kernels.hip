@@ -15,6 +15,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "philox.h"
+#include "predictor_device.h"
 #include "stepfactor.h"
 
 namespace ngpu {
@@ -71,7 +72,7 @@ struct DivRecip {
 // aeif_cond_beta_multisynapse, src/aeif_cond_beta_multisynapse_kernel.h:165-226
 template <int NP>
 struct AeifCondBeta {
-  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 4 * NP;
+  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 4 * NP, NPORT = NP;
   enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
          t_ref, ref, den };
   static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
@@ -181,6 +182,11 @@ struct IzhCondBeta {
     }
   }
 };
+
+// Models the step-size predictor reads (its raw inputs are V, w and the
+// conductance variables of aeif_cond_beta_multisynapse).
+template <class M> struct PredCapable : std::false_type {};
+template <int NP> struct PredCapable<AeifCondBeta<NP>> : std::true_type {};
 
 // ---------------------------------------------------------------------------
 // Spike emission: append to this step's spike list and do the per-node
@@ -362,9 +368,10 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // that of rk5_kernel.  UNI: every parameter except refractory_step is
 // uniform over the population, so parameters come from the kernel-argument
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
-template <class M, bool UNI>
+template <class M, bool UNI, int MODE>
 __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
   using D = typename std::conditional<UNI, DivRecip, DivIEEE>::type;
+  static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * (blockDim.x / kWave) +
                    __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -385,6 +392,14 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
   int i = -1;
   int att_i = 0;   // attempts of the current neuron in this step
   bool need_k1 = false;
+  float el = 1.0f;      // last accepted error ratio (predictor input)
+  float h_sub = 0.0f;   // controller proposal at the start of the substep
+  bool clip = false;    // accepted attempt was cut by the end of the step
+  __shared__ PredLDS s_pred;
+  if constexpr (MODE == 2) {
+    pred_stage(s_pred, a.pred_params);
+    __syncthreads();
+  }
   unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
 #pragma unroll
   for (int j = 0; j < M::NPAR; ++j) {
@@ -412,6 +427,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           ref_in = p[M::i_ref];
           x = a.x[i];
           h = a.h[i];
+          if constexpr (MODE != 0) el = a.err_last[i];
           need_k1 = true;
           att_i = 0;
         }
@@ -419,9 +435,27 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
       next += __popcll(m);
     }
     if (!__any(i >= 0)) break;
+    if constexpr (MODE == 2) {
+      // learned proposal for every lane starting a substep (whole wave on
+      // the matrix cores; lanes without a neuron feed zeros)
+      const bool start = (i >= 0) && need_k1;
+      if (__any(start)) {
+        float raw[kPI];
+        if (i >= 0) {
+          pred_raw_inputs<M::NPORT>(y, h, el, a.dt, raw);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kPI; ++k) raw[k] = 0.0f;
+        }
+        const float o = pred_infer_wave(s_pred, raw);
+        // floor at 2 h_min: a proposal at h_min would be accepted whatever its error
+        if (start && o > -60.0f && o < 60.0f) h = fmaxf(a.dt * exp2f(o), 2.0f * h_min);
+      }
+    }
     if (i >= 0) {
       // ---- start of a substep: k1 and the error scale (src/rk5.h:83-86) ----
       if (need_k1) {
+        if constexpr (MODE == 1) h_sub = h;
         M::template deriv<D>(y, k1, p, r);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
@@ -429,7 +463,11 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
       }
       // ---- one attempt (src/rk5.h:88-131) ----
       const float h_max = (float)(x1 - x);
-      if (h > h_max) h = h_max;
+      clip = false;
+      if (h > h_max) {
+        h = h_max;
+        clip = true;
+      }
       if (h < h_min) h = h_min;
       float k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV], yn[M::NV];
 #pragma unroll
@@ -474,6 +512,21 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
         h = (float)fmax((double)h_new, 0.1 * (double)h);
       } else {
         ++n_sub;   // accepted (src/rk5.h:135-159), then ExternalUpdate
+        if constexpr (MODE == 1) {
+          if (i % a.log_stride == 0) {
+            float raw[kPI];
+            pred_raw_inputs<M::NPORT>(y, h_sub, el, a.dt, raw);
+            const unsigned long long pos = atomicAdd(a.log_count, 1ull);
+            if (pos < (unsigned long long)a.log_cap) {
+              float* rec = a.log_buf + pos * kLogCols;
+#pragma unroll
+              for (int k = 0; k < kPI; ++k) rec[k] = raw[k];
+              rec[kPI] = __log2f(h / a.dt);
+              rec[kPI + 1] = clip ? 1.0f : 0.0f;   // h cut by the end of the step: lower bound
+            }
+          }
+        }
+        if constexpr (MODE != 0) el = err;
         if (att_i <= kMaxAttempts) x += h;
         h = (err > ck::err_min) ? h * ck::safety * pow_m02(err) : 5.0f * h;
 #pragma unroll
@@ -498,6 +551,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           for (int j = 0; j < M::NV; ++j) a.var[(size_t)j * n + i] = y[j];
           a.x[i] = x;
           a.h[i] = h;
+          if constexpr (MODE != 0) a.err_last[i] = el;
           if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
           i = -1;
         }
@@ -529,10 +583,20 @@ static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
     bool uni = rest == 0 && a.tab.per_neuron[1] == 0;
     for (int j = 0; j < M::NPAR && uni; ++j)
       if (M::divisor(j) && recip_rn(a.tab.val[j]) == 0.0f) uni = false;
+    if constexpr (PredCapable<M>::value) {
+      if (uni && a.pred_mode == 1) {
+        hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 1>), dim3(grid), dim3(block), 0, s, a);
+        return;
+      }
+      if (uni && a.pred_mode == 2) {
+        hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 2>), dim3(grid), dim3(block), 0, s, a);
+        return;
+      }
+    }
     if (uni)
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, true>), dim3(grid), dim3(block), 0, s, a);
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 0>), dim3(grid), dim3(block), 0, s, a);
     else
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, false>), dim3(grid), dim3(block), 0, s, a);
+      hipLaunchKernelGGL((rk5_persistent_kernel<M, false, 0>), dim3(grid), dim3(block), 0, s, a);
     return;
   }
   const int grid = (a.n + block - 1) / block;
@@ -552,6 +616,10 @@ static void dispatch_ports(int n_port, const RK5Args& a, hipStream_t s) {
     case 8: launch_rk5_t<M<8>>(a, s); break;
     default: break;
   }
+}
+
+int rk5_predictor_capable(int model, int n_port) {
+  return model == MODEL_AEIF_COND_BETA_MS && n_port >= 1 && n_port <= kMaxPort;
 }
 
 int rk5_supported(int model, int n_port) {

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "predictor_host.h"
 
 using ngpu::Engine;
 using ngpu::Error;
@@ -627,6 +628,71 @@ int NESTGPU_AMD_ResetKernel() {
 int NESTGPU_AMD_Synchronize() {
   return guarded(0, [&] {
     E().Synchronize();
+    return 0;
+  });
+}
+
+int NESTGPU_AMD_SetStepPredictorMode(int mode) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().SetMode(mode);
+    return 0;
+  });
+}
+int NESTGPU_AMD_StartPredictorLog(long long capacity, int stride) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().StartLog(capacity, stride);
+    return 0;
+  });
+}
+long long NESTGPU_AMD_GetPredictorLogCount() {
+  return guarded(0LL, [&] { return ngpu::Predictor::get().LogCount(); });
+}
+int NESTGPU_AMD_GetPredictorLog(float* out, long long first, long long n) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().GetLog(out, first, n);
+    return 0;
+  });
+}
+int NESTGPU_AMD_AppendPredictorLog(const float* records, long long n) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().AppendLog(records, n);
+    return 0;
+  });
+}
+int NESTGPU_AMD_InitPredictor(unsigned long long seed) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().Init(seed);
+    return 0;
+  });
+}
+int NESTGPU_AMD_TrainPredictor(int steps, int batch, float lr, unsigned long long seed, float* loss) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().Train(steps, batch, lr, seed, loss);
+    return 0;
+  });
+}
+int NESTGPU_AMD_SetPredictorParams(const float* params, int n) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().SetParams(params, n);
+    return 0;
+  });
+}
+int NESTGPU_AMD_GetPredictorParams(float* params, int n) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().GetParams(params, n);
+    return 0;
+  });
+}
+int NESTGPU_AMD_PredictorInfer(const float* raw, long long n, float* out) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().Infer(raw, n, out);
+    return 0;
+  });
+}
+int NESTGPU_AMD_PredictorLossGrad(const float* raw, const float* target, const unsigned char* censored,
+                                  int n, float* loss, float* grad) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().LossGrad(raw, target, censored, n, loss, grad);
     return 0;
   });
 }

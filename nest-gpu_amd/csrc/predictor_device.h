@@ -1,0 +1,124 @@
+// Device side of the learned step-size predictor (layout and conventions in
+// predictor.h): LDS staging of the parameters, the raw inputs of a neuron,
+// and the wavefront inference on the matrix cores.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "predictor.h"
+
+namespace ngpu {
+
+// Predictor parameters staged in LDS by every block of a kernel that infers.
+struct PredLDS {
+  float mu[kPI], is[kPI];
+  unsigned short w1[kPH * kPI];   // bfloat16, [hidden][input]
+  unsigned short w2[kPH * kPH];   // bfloat16, [out][in]
+  float b1[kPH], b2[kPH], w3[kPH];
+  float b3;
+};
+
+// All threads of the block call this, then __syncthreads().
+__device__ inline void pred_stage(PredLDS& s, const float* P) {
+  for (int k = threadIdx.x; k < kPH * kPH; k += blockDim.x) {
+    s.w2[k] = (unsigned short)bf16_bits(P[kPOffW2 + k]);
+    if (k < kPH * kPI) s.w1[k] = (unsigned short)bf16_bits(P[kPOffW1 + k]);
+    if (k < kPH) {
+      s.b1[k] = P[kPOffB1 + k];
+      s.b2[k] = P[kPOffB2 + k];
+      s.w3[k] = P[kPOffW3 + k];
+    }
+    if (k < kPI) {
+      s.mu[k] = P[kPOffMu + k];
+      s.is[k] = P[kPOffIs + k];
+    }
+    if (k == 0) s.b3 = P[kPOffB3];
+  }
+}
+
+// Raw inputs of a neuron of aeif_cond_beta_multisynapse with NP ports
+// (state [V, w, g_0, g1_0, g_1, g1_1, ...]).
+template <int NP>
+__device__ inline void pred_raw_inputs(const float* y, float h_ctrl, float err_last, float dt,
+                                       float* r) {
+  r[0] = y[0];
+  r[1] = y[1];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float v = 0.0f;
+    if (k < NP) v = y[2 + 2 * k];
+    else if (k - NP < NP) v = y[3 + 2 * (k - NP)];
+    r[2 + k] = v;
+  }
+  r[6] = __log2f(fmaxf(h_ctrl, 1.0e-30f) / dt);
+  r[7] = __log2f(fmaxf(err_last, 1.0e-12f));
+}
+
+typedef short pred_s4 __attribute__((ext_vector_type(4)));
+typedef float pred_f16 __attribute__((ext_vector_type(16)));
+
+__device__ inline pred_s4 pred_pack4(unsigned int lo, unsigned int hi) {
+  pred_s4 v;
+  v[0] = (short)(lo & 0xffffu);
+  v[1] = (short)(lo >> 16);
+  v[2] = (short)(hi & 0xffffu);
+  v[3] = (short)(hi >> 16);
+  return v;
+}
+__device__ inline unsigned int pred_pair(float a, float b) {
+  return bf16_bits(a) | (bf16_bits(b) << 16);
+}
+
+// Output log2(h/dt) for the neuron held by this lane.  Every lane of the
+// wave must call it together (the matrix instructions read all 64 lanes);
+// lanes without a neuron pass any finite inputs and ignore the result.
+__device__ inline float pred_infer_wave(const PredLDS& s, const float* raw) {
+  const int lane = threadIdx.x & 63;
+  const int half = lane >> 5, col = lane & 31;
+  float x[kPI];
+#pragma unroll
+  for (int i = 0; i < kPI; ++i) x[i] = (raw[i] - s.mu[i]) * s.is[i];
+  const unsigned int f01 = pred_pair(x[0], x[1]), f23 = pred_pair(x[2], x[3]),
+                     f45 = pred_pair(x[4], x[5]), f67 = pred_pair(x[6], x[7]);
+  const unsigned int g01 = __shfl_xor(f01, 32), g23 = __shfl_xor(f23, 32),
+                     g45 = __shfl_xor(f45, 32), g67 = __shfl_xor(f67, 32);
+  // B operand, k = 4*half + j, column = neuron: block 0 holds neurons 0..31,
+  // block 1 neurons 32..63 (inputs 0..3 come from lanes < 32, 4..7 from >= 32)
+  const pred_s4 b0 = half ? pred_pack4(g45, g67) : pred_pack4(f01, f23);
+  const pred_s4 b1 = half ? pred_pack4(f45, f67) : pred_pack4(g01, g23);
+  // A operand W1[row = col][k = 4*half + j]
+  const unsigned short* w1 = s.w1 + col * kPI + 4 * half;
+  const pred_s4 a1 = pred_pack4(w1[0] | ((unsigned int)w1[1] << 16),
+                                w1[2] | ((unsigned int)w1[3] << 16));
+  pred_f16 c1, c2;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int row = 8 * (i >> 2) + 4 * half + (i & 3);
+    c1[i] = s.b1[row];
+    c2[i] = s.b2[row];
+  }
+  float out[2];
+#pragma unroll
+  for (int blk = 0; blk < 2; ++blk) {
+    pred_f16 h1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, blk ? b1 : b0, c1, 0, 0, 0);
+    pred_f16 h2 = c2;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const unsigned short* w2 = s.w2 + col * kPH + 8 * kb + 4 * half;
+      const pred_s4 a2 = pred_pack4(w2[0] | ((unsigned int)w2[1] << 16),
+                                    w2[2] | ((unsigned int)w2[3] << 16));
+      const pred_s4 bb = pred_pack4(pred_pair(fmaxf(h1[4 * kb], 0.0f), fmaxf(h1[4 * kb + 1], 0.0f)),
+                                    pred_pair(fmaxf(h1[4 * kb + 2], 0.0f), fmaxf(h1[4 * kb + 3], 0.0f)));
+      h2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, bb, h2, 0, 0, 0);
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int row = 8 * (i >> 2) + 4 * half + (i & 3);
+      part += s.w3[row] * fmaxf(h2[i], 0.0f);
+    }
+    out[blk] = part + __shfl_xor(part, 32);
+  }
+  return (half ? out[1] : out[0]) + s.b3;
+}
+
+}  // namespace ngpu

@@ -66,6 +66,11 @@ def lib():
         L.orc_step_factor_inc.argtypes = [ctypes.c_float]
         L.orc_exp_model.restype = ctypes.c_float
         L.orc_exp_model.argtypes = [ctypes.c_float]
+        L.orc_mlp_n_params.restype = ctypes.c_int
+        L.orc_mlp_forward.restype = None
+        L.orc_mlp_forward.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int, vp]
+        L.orc_mlp_loss_grad.restype = None
+        L.orc_mlp_loss_grad.argtypes = [vp, vp, vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -205,6 +210,33 @@ def step_factors(x):
     f(x.ctypes.data_as(ctypes.c_void_p), ctypes.c_longlong(x.size),
       dec.ctypes.data_as(ctypes.c_void_p), inc.ctypes.data_as(ctypes.c_void_p))
     return dec, inc
+
+
+def mlp_n_params():
+    """Length of the step-size predictor's parameter vector."""
+    return lib().orc_mlp_n_params()
+
+
+def mlp_forward(params, raw, quant=1):
+    """Predictor output log2(h/dt) for raw feature rows [n, 8] (FP64 host)."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    X = np.ascontiguousarray(raw, dtype=np.float32).reshape(-1, 8)
+    out = np.empty(X.shape[0], np.float64)
+    lib().orc_mlp_forward(P.ctypes.data, X.ctypes.data, X.shape[0], quant, out.ctypes.data)
+    return out
+
+
+def mlp_loss_grad(params, raw, target, cens, quant=1):
+    """Mean censored squared loss and its gradient (FP64 host)."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    X = np.ascontiguousarray(raw, dtype=np.float32).reshape(-1, 8)
+    T = np.ascontiguousarray(target, dtype=np.float32)
+    C = np.ascontiguousarray(cens, dtype=np.uint8)
+    g = np.empty(P.size, np.float64)
+    loss = ctypes.c_double()
+    lib().orc_mlp_loss_grad(P.ctypes.data, X.ctypes.data, T.ctypes.data, C.ctypes.data,
+                            X.shape[0], quant, ctypes.addressof(loss), g.ctypes.data)
+    return loss.value, g
 
 
 def exp_model(x):

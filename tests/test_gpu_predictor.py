@@ -1,0 +1,159 @@
+"""Learned step-size predictor on the GPU (MI355X only): matrix-core
+inference and training gradients against the oracle's FP64 reference, and
+what a network trained on plain-RK5 logs gains over the controller alone.
+
+The predictor persists across ResetKernel(), so every test leaves it in
+mode 0 (controller only)."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+import netbuild
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NI, NH = 8, 32
+O_W1 = 16
+O_B1 = O_W1 + NH * NI
+O_W2 = O_B1 + NH
+O_B2 = O_W2 + NH * NH
+O_W3 = O_B2 + NH
+O_B3 = O_W3 + NH
+
+
+@pytest.fixture(scope="module")
+def ngpu():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    import nestgpu
+    nestgpu.SetVerbosityLevel(0)
+    return nestgpu
+
+
+@pytest.fixture(autouse=True)
+def fresh(ngpu):
+    print("case start", flush=True)
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    yield
+    ngpu.SetStepPredictorMode(0)
+
+
+def _random_params(rng, grid=False):
+    P = np.zeros(ngpu_n_params(), np.float64)
+    if grid:
+        # bfloat16-exact weights and inputs on a coarse grid: the layer-1
+        # sums are exact in FP32 and FP64, so both sides round identically
+        P[8:16] = 1.0
+        P[O_W1:O_B1] = rng.integers(-32, 33, O_B1 - O_W1) / 64.0
+        P[O_B1:O_W2] = rng.integers(-16, 17, NH) / 64.0
+        P[O_W2:O_B2] = rng.integers(-32, 33, NH * NH) / 256.0
+        P[O_B2:O_W3] = rng.integers(-16, 17, NH) / 64.0
+        P[O_W3:O_B3] = rng.integers(-32, 33, NH) / 64.0
+        P[O_B3] = 0.5
+    else:
+        P[0:8] = rng.normal(0, 1, 8)
+        P[8:16] = rng.uniform(0.5, 2.0, 8)
+        P[O_W1:O_B1] = rng.normal(0, 0.5, O_B1 - O_W1)
+        P[O_B1:O_W2] = rng.normal(0, 0.1, NH)
+        P[O_W2:O_B2] = rng.normal(0, 0.25, NH * NH)
+        P[O_B2:O_W3] = rng.normal(0, 0.1, NH)
+        P[O_W3:O_B3] = rng.normal(0, 0.3, NH)
+        P[O_B3] = -1.0
+    return P
+
+
+def ngpu_n_params():
+    return O.mlp_n_params()
+
+
+def test_matrix_core_inference_matches_oracle(ngpu):
+    rng = np.random.default_rng(21)
+    P = _random_params(rng)
+    ngpu.SetPredictorParams(P.astype(np.float32))
+    X = rng.normal(0, 2.0, (1000, NI)).astype(np.float32)   # not a multiple of 64
+    out = ngpu.PredictorInfer(X)
+    ref = O.mlp_forward(P.astype(np.float32).astype(np.float64), X, quant=1)
+    err = np.abs(out - ref)
+    scale = np.maximum(1.0, np.abs(ref))
+    print("inference |gpu - oracle|: median %.3g max %.3g (output scale %.3g)"
+          % (np.median(err), err.max(), np.abs(ref).mean()))
+    # FP32 accumulation on the matrix cores vs FP64; an input or activation
+    # that rounds to the other bfloat16 neighbour moves the output by ~2^-8
+    assert np.median(err / scale) < 1e-4
+    assert np.max(err / scale) < 3e-2
+
+
+def test_training_gradient_matches_oracle(ngpu):
+    rng = np.random.default_rng(5)
+    P = _random_params(rng, grid=True)
+    ngpu.SetPredictorParams(P.astype(np.float32))
+    n = 300
+    X = (rng.integers(-24, 25, (n, NI)) / 16.0).astype(np.float32)
+    out = O.mlp_forward(P, X, quant=1)
+    T = (out + rng.normal(0, 1.0, n)).astype(np.float32)
+    C = (rng.random(n) < 0.3).astype(np.uint8)
+    loss, g = ngpu.PredictorLossGrad(X, T, C)
+    lref, gref = O.mlp_loss_grad(P, X, T, C, quant=1)
+    print("loss gpu %.8g oracle %.8g; max |dg| %.3g of max |g| %.3g"
+          % (loss, lref, np.abs(g - gref[16:]).max(), np.abs(gref).max()))
+    assert abs(loss - lref) <= 1e-5 * lref
+    np.testing.assert_allclose(g, gref[16:], rtol=1e-4, atol=1e-5 * np.abs(gref).max())
+
+
+def _stats(t_idx, node, n, t0_idx, t1_idx):
+    m = (t_idx >= t0_idx) & (t_idx < t1_idx)
+    rate = m.sum() / n / ((t1_idx - t0_idx) * 1e-4)
+    cvs = []
+    for i in np.unique(node[m]):
+        ts = np.sort(t_idx[m][node[m] == i])
+        if len(ts) > 3:
+            isi = np.diff(ts).astype(np.float64)
+            cvs.append(isi.std() / isi.mean())
+    return rate, float(np.mean(cvs))
+
+
+def _run(ngpu, net, sim_ms, mode):
+    from test_gpu_parity import _gpu_network
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    ngpu.SetStepPredictorMode(mode)
+    t, nd = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", sim_ms, poiss_w=0.27,
+                         status=netbuild.C1_RECEPTORS)
+    st = ngpu.GetStepStats()
+    ngpu.SetStepPredictorMode(0)
+    return t, nd, st
+
+
+def test_learned_step_size_gain(ngpu):
+    # 1) plain RK5 with logging on one network, 2) training on the GPU,
+    # 3) plain vs learned on another network: fewer RK5 attempts, far fewer
+    # rejections, same spike statistics (rate, CV) after the first 100 ms
+    n = 2000
+    train_net = netbuild.c1_network(seed=11, n=n, k=100)
+    ngpu.StartPredictorLog(4_000_000, 3)
+    _run(ngpu, train_net, 600.0, "log")
+    n_log = ngpu.GetPredictorLogCount()
+    recs = ngpu.GetPredictorLog(0, min(n_log, 200000))
+    print("logged records", n_log, "censored fraction %.3f" % recs[:, 9].mean())
+    assert n_log > 1_000_000
+    assert np.all(np.isfinite(recs))
+    ngpu.InitPredictor(7)
+    loss = ngpu.TrainPredictor(3000, 8192, 2e-3, 3)
+    print("training loss: first %.4f, last %.4f" % (loss[:20].mean(), loss[-50:].mean()))
+    assert loss[-50:].mean() < 0.5 * loss[:20].mean()
+
+    eval_net = netbuild.c1_network(seed=12, n=n, k=100)
+    tp, np_, sp = _run(ngpu, eval_net, 500.0, "plain")
+    tl, nl, sl = _run(ngpu, eval_net, 500.0, "learned")
+    rp, cp = _stats(tp, np_, n, 1000, 5000)
+    rl, cl = _stats(tl, nl, n, 1000, 5000)
+    fp = sp["rejected"] / sp["attempts"]
+    fl = sl["rejected"] / sl["attempts"]
+    print("plain:   attempts %d rejected %.4f rate %.3f Hz CV %.4f" % (sp["attempts"], fp, rp, cp))
+    print("learned: attempts %d rejected %.4f rate %.3f Hz CV %.4f" % (sl["attempts"], fl, rl, cl))
+    assert fl < 0.5 * fp
+    assert sl["attempts"] < sp["attempts"]
+    assert abs(rl - rp) < 0.03 * rp
+    assert abs(cl - cp) < 0.03 * cp
