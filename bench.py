@@ -1,22 +1,24 @@
-"""Benchmark of the MI355X simulation step on the 1M-neuron AdEx network.
+"""Benchmark of the MI355X simulation step on the 1M-neuron AdEx networks.
 
-Workload (config 3): 1,000,000 aeif_cond_beta_multisynapse neurons, 2
-receptor ports (E_rev 0/-85 mV, 1 ms beta time constants), 80/20 exc/inh,
-fixed indegree 1000 (800 exc -> port 0, 200 inh -> port 1; 1e9 synapses),
-weights U(0.025,0.075) nS exc / U(0.175,0.525) nS inh, delays U{1..50} x 0.1
-ms, one Poisson generator (20 kHz, 0.237 nS) per neuron tuned to ~5 Hz,
-FP32 state, FP64 time, dt = 0.1 ms.  A "step" is one 0.1 ms simulation step
-of the whole network.  The network is built through the nestgpu API, as a
+Default workload (scale_adex, nest-gpu_amd/workloads.py): 1,000,000
+aeif_cond_beta_multisynapse neurons, 2 receptor ports, 80/20 exc/inh, fixed
+indegree 1000 (1e9 synapses), delays U{1..50} x 0.1 ms, Poisson drive tuned to
+~5 Hz, FP32 state, FP64 time, dt = 0.1 ms.  A "step" is one 0.1 ms simulation
+step of the whole network.  The network is built through the nestgpu API, as a
 user script would; build and calibration are not timed (as the reference's
 "Time to simulate").
+
+Timed windows: `--steps` steps with the plain RK5 controller (the reported
+`value`), then, when a trained step-size predictor exists for the workload
+(nest-gpu_amd/predictors/<workload>.json or --predictor), the same number of
+steps with the learned proposal on the same network, with its RK5 statistics,
+spike rate and ISI CV beside the plain window's.
 
 Prints ONE JSON line on stdout.  Progress goes to stderr.
 """
 import argparse
-import ctypes
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -26,11 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "nest-gpu_amd"))
 sys.path.insert(0, ROOT)
 
-DT = 0.1
+import workloads  # noqa: E402
+
+DT = workloads.DT
 N_FULL = 1_000_000
-K_EXC, K_INH = 800, 200
-POISSON_RATE, POISSON_W = 20000.0, 0.237
-RECEPTORS = {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [1.0, 1.0]}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -47,42 +48,20 @@ def rk5_flops_per_attempt(n_port):
     return 56 * nv + 6 * (15 + 6 * n_port)
 
 
-def build_network(ngpu, n, seed):
-    rng = np.random.default_rng(seed)
-    ne = int(0.8 * n)
-    neurons = ngpu.Create("aeif_cond_beta_multisynapse", n, 2)
-    ngpu.SetStatus(neurons, RECEPTORS)
-    pg = ngpu.Create("poisson_generator")
-    ngpu.SetStatus(pg, "rate", POISSON_RATE)
-    ngpu.Connect(pg, neurons, {"rule": "all_to_all"},
-                 {"weight": POISSON_W, "delay": DT, "receptor": 0})
-    lib = ngpu._nestgpu
-    for (src, k, port, lo, hi) in ((neurons[0:ne], K_EXC, 0, 0.025, 0.075),
-                                   (neurons[ne:n], K_INH, 1, 0.175, 0.525)):
-        m = n * k
-        w = rng.random(m, dtype=np.float32)
-        w *= np.float32(hi - lo)
-        w += np.float32(lo)
-        d = rng.integers(1, 51, size=m, dtype=np.int16).astype(np.float32)
-        d *= np.float32(DT)
-        ngpu.ConnSpecInit()
-        ngpu.SynSpecInit()
-        ngpu.SetConnSpecParam("rule", 3)
-        ngpu.SetConnSpecParam("indegree", k)
-        ngpu.SetSynSpecIntParam("receptor", port)
-        lib.NESTGPU_SetSynSpecFloatPtParam(b"weight_array", w.ctypes.data_as(ctypes.c_void_p))
-        lib.NESTGPU_SetSynSpecFloatPtParam(b"delay_array", d.ctypes.data_as(ctypes.c_void_p))
-        lib.NESTGPU_ConnectSeqSeq(src.i0, src.n, neurons.i0, neurons.n)
-        ngpu._check()
-        del w, d
-    return neurons
+# algorithmic bytes: delivery per (spike, synapse) event = synapse record
+# (u32 channel + f32 weight + u16 delay) + FP64 accumulator read and write
+DELIVER_BYTES_PER_EVENT = 4 + 4 + 2 + 16
+# fold per (neuron, port) channel = ring slot read + clear (16 B) + input
+# variable read and write (8 B); Poisson edges (20 B each) counted separately
+FOLD_BYTES_PER_CHANNEL = 24
+POISSON_BYTES_PER_EDGE = 20
 
 
 def cpu_baseline(n_sample, sim_ms, seed):
     """The oracle (host C++/OpenMP restatement of the reference step) on a
-    slice of the same network: n_sample neurons with the full in-degree 1000
-    and the same per-neuron input; the per-neuron-step cost is scaled to the
-    1M-neuron workload."""
+    slice of the scale_adex network: n_sample neurons with the full in-degree
+    1000 and the same per-neuron input; the per-neuron-step cost is scaled to
+    the 1M-neuron workload."""
     from oracle.oracle import Oracle, threads
     rng = np.random.default_rng(seed)
     n = n_sample
@@ -90,14 +69,14 @@ def cpu_baseline(n_sample, sim_ms, seed):
     o = Oracle(prec=0)
     n0 = o.create("aeif_cond_beta_multisynapse", n, 2)
     for p in range(2):
-        o.set(n0, n, "E_rev", [RECEPTORS["E_rev"][p]] * n, port=p)
-        o.set(n0, n, "tau_rise", [1.0] * n, port=p)
-        o.set(n0, n, "tau_decay", [1.0] * n, port=p)
+        for k, v in workloads.SCALE_RECEPTORS.items():
+            o.set(n0, n, k, [v[p]] * n, port=p)
     pg = o.create("poisson_generator", 1, 0)
-    o.set(pg, 1, "rate", [POISSON_RATE])
-    o.connect(np.full(n, pg, np.int32), np.arange(n, dtype=np.int32) + n0, 0, POISSON_W, DT)
-    for (lo_src, hi_src, k, port, lo, hi) in ((0, ne, K_EXC, 0, 0.025, 0.075),
-                                               (ne, n, K_INH, 1, 0.175, 0.525)):
+    o.set(pg, 1, "rate", [workloads.SCALE_POISSON[0]])
+    o.connect(np.full(n, pg, np.int32), np.arange(n, dtype=np.int32) + n0, 0,
+              workloads.SCALE_POISSON[1], DT)
+    for (lo_src, hi_src, k, port, lo, hi) in ((0, ne, 800, 0, 0.025, 0.075),
+                                               (ne, n, 200, 1, 0.175, 0.525)):
         m = n * k
         src = rng.integers(lo_src, hi_src, size=m).astype(np.int32) + n0
         tgt = np.repeat(np.arange(n, dtype=np.int32), k) + n0
@@ -115,18 +94,95 @@ def cpu_baseline(n_sample, sim_ms, seed):
     rtf_full = per_neuron_step * N_FULL * (1000.0 / DT)
     return {"value": rtf_full, "unit": "s_wall/s_bio", "cores": threads(), "kind": "port",
             "sample": "oracle on %d neurons x %d synapses/neuron for %.0f ms (%.1f s wall); "
-                      "per-neuron-step cost scaled to 1e6 neurons x 1 s" % (n, K_EXC + K_INH, sim_ms, wall),
+                      "per-neuron-step cost scaled to 1e6 neurons x 1 s" % (n, 1000, sim_ms, wall),
             "neuron_updates_per_s": n * steps / wall,
             "syn_events_per_s": (st1["syn_events"] - st0["syn_events"]) / wall}
+
+
+def timed_window(ngpu, steps):
+    """Simulate `steps` steps between two synchronisations; returns wall time,
+    kernel times (HIP events on the simulator stream) and statistics deltas."""
+    st0 = ngpu.GetStepStats()
+    ngpu.SetKernelTiming(True)
+    ngpu.Synchronize()
+    t0 = time.perf_counter()
+    ngpu.Simulate(steps * DT)
+    ngpu.Synchronize()
+    wall = time.perf_counter() - t0
+    kt = ngpu.GetKernelTimes()
+    st1 = ngpu.GetStepStats()
+    ngpu.SetKernelTiming(False)
+    d = {k: st1[k] - st0[k] for k in ("attempts", "rejected", "substeps", "syn_events", "spikes",
+                                      "hot_listed")}
+    return {"wall": wall, "kt": kt, "stats": d}
+
+
+def window_spike_stats(times, t0_ms, t1_ms, n):
+    """Mean rate over all neurons and mean ISI CV over neurons with >= 4
+    spikes inside [t0, t1)."""
+    cnt = 0
+    cvs = []
+    for ts in times:
+        a = np.asarray(ts, np.float64)
+        a = a[(a >= t0_ms) & (a < t1_ms)]
+        cnt += a.size
+        if a.size >= 4:
+            isi = np.diff(np.sort(a))
+            cvs.append(isi.std() / isi.mean())
+    return cnt / n / ((t1_ms - t0_ms) * 1e-3), (float(np.mean(cvs)) if cvs else None), len(cvs)
+
+
+def rooflines(w, n, n_port, steps, n_poisson_edges):
+    kt, st = w["kt"], w["stats"]
+    out = {}
+    ms, cnt = kt["rk5_update"]
+    if cnt:
+        flops = st["attempts"] * rk5_flops_per_attempt(n_port)
+        a = flops / (ms * 1e-3) / 1e12
+        out["rk5_update"] = {"bound": "mfma", "unit": "TFLOP/s", "achieved": a, "peak": PEAK_FP32_TFLOPS,
+                             "frac": a / PEAK_FP32_TFLOPS, "avg_launch_us": ms * 1e3 / cnt,
+                             "flops_per_launch": flops / cnt,
+                             "note": "FP32 vector peak (the RK5 arithmetic is FP32 VALU work)"}
+    ms, cnt = kt["deliver"]
+    if cnt:
+        b = st["syn_events"] * DELIVER_BYTES_PER_EVENT
+        a = b / (ms * 1e-3) / 1e9
+        out["deliver"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
+                          "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
+                          "bytes_per_launch": b / cnt}
+    ms, cnt = kt["fold_poisson"]
+    if cnt:
+        b = (n * n_port * FOLD_BYTES_PER_CHANNEL + n_poisson_edges * POISSON_BYTES_PER_EDGE) * steps
+        a = b / (ms * 1e-3) / 1e9
+        out["fold_poisson"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
+                               "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
+                               "bytes_per_launch": b / cnt}
+    return out
+
+
+def window_summary(w, steps, n):
+    bio = steps * DT / 1000.0
+    st = w["stats"]
+    return {"rtf": w["wall"] / bio, "ms_per_step": w["wall"] * 1e3 / steps,
+            "neuron_updates_per_s": n * steps / w["wall"],
+            "synaptic_events_per_s": st["syn_events"] / w["wall"],
+            "rk5_attempts_per_neuron_step": st["attempts"] / (n * steps),
+            "rk5_rejected_fraction": st["rejected"] / max(st["attempts"], 1),
+            "mean_rate_hz": st["spikes"] / n / bio,
+            "hot_listed_per_step": st["hot_listed"] / steps,
+            "kernel_ms": {k: v[0] for k, v in w["kt"].items()}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000, help="timed 0.1 ms steps (10000 = 1 s)")
+    ap.add_argument("--steps", type=int, default=10000, help="timed 0.1 ms steps per window (10000 = 1 s)")
     ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", default="scale_adex", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--neurons", type=int, default=N_FULL)
-    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--predictor", default=None,
+                    help="trained predictor JSON ('none' to skip the learned window)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--cpu-ms", type=float, default=100.0)
@@ -138,6 +194,15 @@ def main():
         print("bench.py: this build runs on one GPU only (multi-GPU spike exchange is out of "
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(0)
+    seed = args.seed if args.seed is not None else (12345 if args.workload == "scale_adex" else 200)
+    pred_path = args.predictor
+    if pred_path is None:
+        p = os.path.join(ROOT, "nest-gpu_amd", "predictors", args.workload + ".json")
+        pred_path = p if os.path.exists(p) else "none"
+    predictor = None
+    if pred_path != "none":
+        with open(pred_path) as f:
+            predictor = json.load(f)
 
     if args.copy_calibration:
         import torch
@@ -150,83 +215,92 @@ def main():
 
     import nestgpu as ngpu
     ngpu.SetVerbosityLevel(0)
-    ngpu.SetRandomSeed(args.seed)
+    ngpu.SetStepPredictorMode("plain")
     n = args.neurons
+    n_port = 2 if args.workload == "scale_adex" else 4
+    n_poisson_edges = n
     t0 = time.perf_counter()
-    build_network(ngpu, n, args.seed)
-    log("built %d neurons, %d connections in %.1f s" % (n, n * (K_EXC + K_INH) + n, time.perf_counter() - t0))
+    neurons = workloads.WORKLOADS[args.workload](ngpu, n, seed)
+    log("built %s: %d neurons, %d synapses + %d Poisson edges in %.1f s"
+        % (args.workload, n, n * 1000, n, time.perf_counter() - t0))
+    if predictor is not None:
+        ngpu.ActivateRecSpikeTimes(neurons, 150)
     t0 = time.perf_counter()
     ngpu.Calibrate()
     log("calibrated in %.1f s" % (time.perf_counter() - t0))
     ngpu.Simulate(args.warmup * DT)
     ngpu.Synchronize()
     log("warmup %d steps done" % args.warmup)
-    st0 = ngpu.GetStepStats()
-    ngpu.SetKernelTiming(True)
-    ngpu.Synchronize()
-    t0 = time.perf_counter()
-    ngpu.Simulate(args.steps * DT)
-    ngpu.Synchronize()
-    wall = time.perf_counter() - t0
-    kt = ngpu.GetKernelTimes()
-    st1 = ngpu.GetStepStats()
-    ngpu.SetKernelTiming(False)
-    bio_s = args.steps * DT / 1000.0
-    rtf = wall / bio_s
-    att = st1["attempts"] - st0["attempts"]
-    rej = st1["rejected"] - st0["rejected"]
-    ev = st1["syn_events"] - st0["syn_events"]
-    spk = st1["spikes"] - st0["spikes"]
-    log("timed %d steps: wall %.3f s, RTF %.4f, kernel ms %s" % (args.steps, wall, rtf, kt))
+    t_plain0 = (args.warmup) * DT
+    wp = timed_window(ngpu, args.steps)
+    sp = window_summary(wp, args.steps, n)
+    log("plain: %d steps, wall %.3f s, RTF %.4f, kernel ms %s" % (args.steps, wp["wall"], sp["rtf"], wp["kt"]))
+    roof_all = rooflines(wp, n, n_port, args.steps, n_poisson_edges)
 
-    # dominant kernel and its roofline
-    dom = max(kt, key=lambda k: kt[k][0])
-    rk5_ms, rk5_n = kt["rk5_update"]
-    if dom == "rk5_update":
-        flops_total = att * rk5_flops_per_attempt(2)
-        achieved = flops_total / (rk5_ms * 1e-3) / 1e12
-        roof = {"kernel": "rk5_kernel<AeifCondBeta<2>>", "bound": "mfma", "unit": "TFLOP/s",
-                "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "frac": achieved / PEAK_FP32_TFLOPS,
-                "traffic": None, "avg_launch_us": rk5_ms * 1e3 / max(rk5_n, 1),
-                "flops_per_launch": flops_total / max(rk5_n, 1)}
-    else:
-        ms, cnt = kt[dom]
-        bytes_total = ev * 10.0 if dom == "deliver" else n * 2 * 16.0 * cnt
-        achieved = bytes_total / (ms * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "unit": "GB/s", "achieved": achieved,
-                "peak": PEAK_HBM_GBS, "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                "avg_launch_us": ms * 1e3 / max(cnt, 1)}
+    learned = None
+    if predictor is not None:
+        ngpu.SetPredictorParams(np.asarray(predictor["params"], np.float32))
+        ngpu.SetStepPredictorMode("learned")
+        ngpu.Simulate(200 * DT)     # switch-over, untimed
+        t_learn0 = t_plain0 + (args.steps + 200) * DT
+        wl = timed_window(ngpu, args.steps)
+        ngpu.SetStepPredictorMode("plain")
+        sl = window_summary(wl, args.steps, n)
+        log("learned: wall %.3f s, RTF %.4f, kernel ms %s" % (wl["wall"], sl["rtf"], wl["kt"]))
+        t0 = time.perf_counter()
+        times = ngpu.GetRecSpikeTimes(neurons)
+        rp, cvp, ncp = window_spike_stats(times, t_plain0, t_plain0 + args.steps * DT, n)
+        rl, cvl, ncl = window_spike_stats(times, t_learn0, t_learn0 + args.steps * DT, n)
+        log("spike statistics in %.1f s: plain rate %.3f CV %s (%d), learned rate %.3f CV %s (%d)"
+            % (time.perf_counter() - t0, rp, cvp, ncp, rl, cvl, ncl))
+        sp["isi_cv"] = cvp
+        sl["isi_cv"] = cvl
+        learned = dict(sl)
+        learned["speedup_vs_plain"] = sp["rtf"] / sl["rtf"]
+        learned["rk5_kernel_speedup_vs_plain"] = (wp["kt"]["rk5_update"][0] / max(wl["kt"]["rk5_update"][0], 1e-9))
+        learned["rate_rel_diff"] = (rl - rp) / rp if rp else None
+        learned["cv_rel_diff"] = (cvl - cvp) / cvp if (cvp and cvl) else None
+        learned["predictor"] = {k: predictor[k] for k in predictor if k != "params"}
+
+    # dominant kernel of the plain window
+    dom = max(wp["kt"], key=lambda k: wp["kt"][k][0])
+    roof = dict(roof_all.get(dom, {}))
+    roof["kernel"] = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>>" % n_port}.get(dom, dom)
+    roof["traffic"] = None
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload == "scale_adex":
         log("cpu baseline: oracle on %d neurons for %.0f ms" % (args.cpu_sample, args.cpu_ms))
-        cpu = cpu_baseline(args.cpu_sample, args.cpu_ms, args.seed)
+        cpu = cpu_baseline(args.cpu_sample, args.cpu_ms, seed)
 
     out = {
         "metric": "real-time factor (1M AdEx, 1e9 syn) + synaptic events/s; learned-h vs plain RK5",
-        "value": rtf,
+        "value": sp["rtf"],
         "unit": "s_wall/s_bio",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": sp["ms_per_step"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "aeif_cond_beta_multisynapse %d neurons, 2 ports, indegree 1000 "
-                               "(%.2g synapses), delays U{1..50}x0.1 ms, Poisson 20 kHz, plain RK5, "
-                               "1 s bio" % (n, n * 1000.0),
-                   "neurons": n, "synapses": n * 1000, "dt_ms": DT, "method": "plain RK5 (eps 1e-6)"},
-        "synaptic_events_per_s": ev / wall,
-        "neuron_updates_per_s": n * args.steps / wall,
-        "mean_rate_hz": spk / n / bio_s,
-        "dropped_spikes": st1["dropped_spikes"],
-        "rk5_attempts_per_neuron_step": att / (n * args.steps),
-        "rk5_rejected_fraction": rej / max(att, 1),
-        "kernel_ms": {k: v[0] for k, v in kt.items()},
+        "config": {"workload": "%s: aeif_cond_beta_multisynapse %d neurons, %d ports, indegree 1000 "
+                               "(%.2g synapses), delays U{1..50}x0.1 ms, Poisson input, plain RK5 "
+                               "(eps 1e-6), 1 s bio per window" % (args.workload, n, n_port, n * 1000.0),
+                   "neurons": n, "synapses": n * 1000, "dt_ms": DT, "seed": seed,
+                   "method": "plain RK5 (value); learned step-size proposal in 'learned'"},
+        "synaptic_events_per_s": sp["synaptic_events_per_s"],
+        "neuron_updates_per_s": sp["neuron_updates_per_s"],
+        "mean_rate_hz": sp["mean_rate_hz"],
+        "rk5_attempts_per_neuron_step": sp["rk5_attempts_per_neuron_step"],
+        "rk5_rejected_fraction": sp["rk5_rejected_fraction"],
+        "kernel_ms": sp["kernel_ms"],
+        "plain": sp,
+        "learned": learned,
         "roofline": roof,
+        "rooflines": roof_all,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

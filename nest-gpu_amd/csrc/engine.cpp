@@ -405,6 +405,30 @@ static void col_write(bool cal, Group& g, std::vector<float>& stage, DBuf<float>
   }
 }
 
+// Writes vals[k] at rows rel[k] of column j: into the host staging before
+// calibration (O(entries)), else to the device (one copy for a contiguous
+// ascending range, one read-modify-write of the column otherwise).
+static void col_write_entries(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev,
+                              int j, const std::vector<int>& rel, const float* vals) {
+  if (rel.empty()) return;
+  if (!cal) {
+    float* c = stage.data() + (size_t)j * g.n;
+    for (size_t k = 0; k < rel.size(); ++k) c[rel[k]] = vals[k];
+    return;
+  }
+  bool contiguous = true;
+  for (size_t k = 1; k < rel.size() && contiguous; ++k) contiguous = rel[k] == rel[0] + (int)k;
+  if (contiguous) {
+    HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n + rel[0], vals, rel.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+    return;
+  }
+  std::vector<float> col(g.n);
+  HIPCHK(hipMemcpy(col.data(), dev.p + (size_t)j * g.n, g.n * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
+  HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n, col.data(), g.n * sizeof(float), hipMemcpyHostToDevice));
+}
+
 void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const float* vals,
                              bool whole) {
   int gi = group_index(groups_, &g);
@@ -418,26 +442,22 @@ void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const
     }
     return;
   }
-  std::vector<float> col(g.n);
-  col_read(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
-  for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
-  col_write(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
-  bool same = std::all_of(col.begin(), col.end(), [&](float v) { return v == col[0]; });
-  if (same && !(g.md->rk5 && j == param_idx(g, "refractory_step", 0))) {
-    g.uni[j] = 1;
-    g.uni_val[j] = col[0];
-  } else {
-    g.uni[j] = 0;
+  col_write_entries(calibrated_, g, stage, g.param, j, rel, vals);
+  // stays uniform only if every written value equals the uniform value (a
+  // column made equal again by partial writes is just read per neuron)
+  if (g.uni[j]) {
+    for (size_t k = 0; k < rel.size(); ++k)
+      if (vals[k] != g.uni_val[j]) {
+        g.uni[j] = 0;
+        break;
+      }
   }
 }
 
 void Engine::set_var_index(Group& g, int j, const std::vector<int>& rel, const float* vals) {
   int gi = group_index(groups_, &g);
   auto& stage = stage_var_[gi];
-  std::vector<float> col(g.n);
-  col_read(calibrated_, g, stage, g.var, j, 0, g.n, col.data());
-  for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
-  col_write(calibrated_, g, stage, g.var, j, 0, g.n, col.data());
+  col_write_entries(calibrated_, g, stage, g.var, j, rel, vals);
 }
 
 static std::vector<int> iota_rel(int rel0, int n) {
@@ -1441,6 +1461,22 @@ int Engine::Calibrate() {
         std::vector<float> e(g->n, 1.0f);   // last accepted error ratio, predictor input
         upload(g->err_last, e, stream_);
       }
+      const char* hot_env = getenv("NGPU_RK5_HOT");
+      if (rk5_persistent_ && !(hot_env && atoi(hot_env) == 0)) {
+        // hot-first lists: one region per wave, as long as a wave's chunk + 64
+        g->hot_waves = (int)rk5_waves(g->n, 1, rk5_max_blocks_);
+        const int chunk = ((g->n + g->hot_waves - 1) / g->hot_waves + 63) & ~63;
+        g->hot_k = chunk + 64;
+        g->hot_flag.alloc((size_t)2 * g->n);
+        g->hot_reg.alloc((size_t)2 * g->hot_waves * g->hot_k);
+        g->hot_dense.alloc((size_t)2 * g->hot_waves * g->hot_k);
+        g->hot_cnt.alloc((size_t)2 * g->hot_waves);
+        g->hot_off.alloc(g->hot_waves);
+        g->hot_n.alloc(2);
+        HIPCHK(hipMemsetAsync(g->hot_flag.p, 0, g->hot_flag.n, stream_));
+        HIPCHK(hipMemsetAsync(g->hot_cnt.p, 0, g->hot_cnt.n * sizeof(int), stream_));
+        HIPCHK(hipMemsetAsync(g->hot_n.p, 0, 2 * sizeof(int), stream_));
+      }
     }
     for (auto& iv : g->int_var) {
       std::vector<int> z(g->n, 0);
@@ -1525,7 +1561,27 @@ void Engine::launch_step() {
           a.log_cap = pr.log_cap();
           a.log_stride = pr.log_stride();
         }
+        if (g->hot_flag.p) {
+          a.hot_flag = g->hot_flag.p;
+          a.hot_reg = g->hot_reg.p;
+          a.hot_cnt = g->hot_cnt.p;
+          a.hot_dense = g->hot_dense.p;
+          a.hot_n = g->hot_n.p;
+          a.hot_k = g->hot_k;
+        }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
+        if (g->hot_flag.p) {
+          HotGatherArgs ha{};
+          ha.ctx = d_ctx_.p;
+          ha.hot_reg = g->hot_reg.p;
+          ha.hot_cnt = g->hot_cnt.p;
+          ha.hot_off = g->hot_off.p;
+          ha.hot_dense = g->hot_dense.p;
+          ha.hot_n = g->hot_n.p;
+          ha.waves = g->hot_waves;
+          ha.hot_k = g->hot_k;
+          timed(KT_OTHER, [&] { launch_hot_gather(ha, stream_); });
+        }
         break;
       }
       case MODEL_IAF_PSC_EXP: {
@@ -1593,6 +1649,7 @@ void Engine::launch_step() {
 void Engine::timed(int cat, const std::function<void()>& launch) {
   if (!timing_) {
     launch();
+    HIPCHK(hipGetLastError());   // a launch that cannot run fails the step loudly
     return;
   }
   auto get = [&]() {
@@ -1608,6 +1665,7 @@ void Engine::timed(int cat, const std::function<void()>& launch) {
   EvPair p{get(), get(), cat};
   HIPCHK(hipEventRecord(p.a, stream_));
   launch();
+  HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(p.b, stream_));
   ev_pending_.push_back(p);
 }
@@ -1767,14 +1825,16 @@ int Engine::Simulate() {
 }
 
 void Engine::Stats(unsigned long long* out) {
-  for (int k = 0; k < 6; ++k) out[k] = 0;
+  for (int k = 0; k < 7; ++k) out[k] = 0;
   if (!calibrated_) return;
   HIPCHK(hipStreamSynchronize(stream_));
   std::vector<unsigned long long> rows(d_ctr_.n);
   HIPCHK(hipMemcpy(rows.data(), d_ctr_.p, rows.size() * sizeof(unsigned long long),
                    hipMemcpyDeviceToHost));
-  for (size_t r = 0; r < rows.size(); r += kCtrCols)
-    for (int k = 0; k < kCtrCols; ++k) out[k] += rows[r + k];
+  for (size_t r = 0; r < rows.size(); r += kCtrCols) {
+    for (int k = 0; k < 4; ++k) out[k] += rows[r + k];
+    out[6] += rows[r + 4];
+  }
   HIPCHK(hipMemcpy(&out[4], d_stats_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
   int ov = 0;
   HIPCHK(hipMemcpy(&ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -1791,6 +1851,7 @@ template struct DBuf<int>;
 template struct DBuf<long long>;
 template struct DBuf<unsigned int>;
 template struct DBuf<unsigned short>;
+template struct DBuf<unsigned char>;
 template struct DBuf<unsigned long long>;
 template struct DBuf<StepCtx>;
 template struct DBuf<RecCol>;

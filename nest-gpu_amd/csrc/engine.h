@@ -57,6 +57,10 @@ struct Group {
   DBuf<double> x;
   DBuf<float> h;
   DBuf<float> err_last;         // last accepted RK5 error ratio (step-size predictor input)
+  // hot-first scheduling of the persistent RK5 kernel
+  DBuf<unsigned char> hot_flag;
+  DBuf<int> hot_reg, hot_cnt, hot_off, hot_dense, hot_n;
+  int hot_waves = 0, hot_k = 0;
   std::vector<float> uni_val;   // value of each parameter if uniform
   std::vector<char> uni;        // parameter is equal over the population
   float h_min_rel = 1.0e-3f, h0_rel = 1.0e-2f, h_min = 0.f, h0 = 0.f;
@@ -208,7 +212,7 @@ class Engine {
   int EndSimulation();
 
   // ---- instrumentation (not part of the reference API)
-  void Stats(unsigned long long* out);   // attempts, rejected, substeps, events, spikes, dropped
+  void Stats(unsigned long long* out);   // attempts, rejected, substeps, events, spikes, dropped, hot listed
   // Kernel timing with HIP events on the simulator stream, per category:
   // 0 RK5 update, 1 iaf_psc_exp update, 2 delivery, 3 fold+Poisson, 4 other.
   enum { KT_RK5 = 0, KT_IAF, KT_DELIVER, KT_FOLD, KT_OTHER, KT_N };

@@ -10,8 +10,8 @@ namespace ngpu {
 // Per-wave counter slots: each wave of a launch adds its counts into row
 // (global wave index) with plain loads and stores, and the host sums the
 // rows when asked.  Columns: RK5 attempts, rejected attempts, accepted
-// substeps, delivered (spike, synapse) pairs.
-constexpr int kCtrCols = 4;
+// substeps, delivered (spike, synapse) pairs, neurons put on a hot list.
+constexpr int kCtrCols = 5;
 constexpr int kDeliverBlocks = 1024;
 
 enum ModelId {
@@ -50,6 +50,27 @@ struct RK5Args {
   unsigned long long* log_count;   // records written (may exceed log_cap)
   long long log_cap;
   int log_stride;                  // log neurons with index % log_stride == 0
+  // hot-first scheduling of the persistent kernel (buffers by step parity)
+  unsigned char* hot_flag;         // [2][n] neuron listed hot for the step, or null
+  int* hot_reg;                    // [2][waves][hot_k] per-wave lists written at step end
+  int* hot_cnt;                    // [2][waves]
+  const int* hot_dense;            // [2][waves * hot_k] packed list of the step
+  const int* hot_n;                // [2] entries of the packed list
+  int hot_k;
+};
+
+// Packs the per-wave hot lists written by an RK5 launch into the list the
+// next step's launch processes first (a one-block scan of the counts, then
+// one wave per list copying it).
+struct HotGatherArgs {
+  const StepCtx* ctx;
+  const int* hot_reg;
+  const int* hot_cnt;
+  int* hot_off;          // [waves] exclusive prefix of the counts (scratch)
+  int* hot_dense;
+  int* hot_n;
+  int waves;
+  int hot_k;
 };
 
 struct IafArgs {
@@ -142,6 +163,7 @@ int rk5_supported(int model, int n_port);
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);
 long long rk5_waves(int n, int persistent, int max_blocks);   // waves of one launch
 int rk5_predictor_capable(int model, int n_port);                // model supports pred_mode 1, 2
+void launch_hot_gather(const HotGatherArgs& a, hipStream_t s);
 void launch_iaf(const IafArgs& a, hipStream_t s);
 void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);

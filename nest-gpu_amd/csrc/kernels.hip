@@ -80,6 +80,8 @@ struct AeifCondBeta {
   static constexpr bool divisor(int j) {
     return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 4 >= 1 && (j - 14) % 4 <= 2);
   }
+  // membrane potential from which a neuron is scheduled first (spike upswing)
+  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
@@ -110,6 +112,7 @@ struct AeifCondAlpha {
   static constexpr bool divisor(int j) {
     return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 3 == 1);
   }
+  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
@@ -140,6 +143,7 @@ struct AeifPscExp {
   static constexpr bool divisor(int j) {
     return j == tau_ex || j == tau_in || j == Delta_T || j == C_m || j == tau_w;
   }
+  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn_tot = 0.0f;
@@ -165,6 +169,7 @@ struct IzhCondBeta {
   static constexpr int iV = 0, iw = 1, i_reset = c, i_thr = V_th, i_jump = d,
                        i_tref = t_ref, i_ref = ref;
   static constexpr bool divisor(int j) { return j >= 9 && (j - 9) % 4 >= 1 && (j - 9) % 4 <= 2; }
+  __device__ static inline float hot_v(const float*) { return -45.0f; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
@@ -377,13 +382,44 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
                    __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_waves = gridDim.x * (blockDim.x / kWave);
   const int n = a.n;
-  const int chunk = ((n + n_waves - 1) / n_waves + kWave - 1) & ~(kWave - 1);
-  const long long c0 = (long long)wave * chunk;
-  int next = c0 < n ? (int)c0 : n;
-  const int end = c0 + chunk < n ? (int)(c0 + chunk) : n;
   const double x1 = a.ctx->t1;
   const long long t_idx = a.ctx->t_idx;
   const int parity = a.ctx->parity;
+  // this step's hot list first, then every neuron (those on the hot list
+  // are skipped there)
+  const int hot_cap = n_waves * a.hot_k;
+  int n_hot = 0;
+  const unsigned char* flag_in = nullptr;
+  unsigned char* flag_out = nullptr;
+  const int* dense = nullptr;
+  int* reg_out = nullptr;
+  if (a.hot_flag) {
+    n_hot = a.hot_n[parity];
+    if (n_hot > hot_cap) n_hot = hot_cap;
+    flag_in = a.hot_flag + (size_t)parity * n;
+    flag_out = a.hot_flag + (size_t)(parity ^ 1) * n;
+    dense = a.hot_dense + (size_t)parity * hot_cap;
+    reg_out = a.hot_reg + ((size_t)(parity ^ 1) * n_waves + wave) * a.hot_k;
+  }
+  // wave w takes hot entries [w*hp, w*hp + hp) (one per lane, so the long
+  // attempt chains of neurons about to spike start together in the first
+  // iteration and never queue behind each other in a lane), then its equal
+  // share of all neurons; indices below are relative to the wave's range
+  const int hp = n_hot <= n_waves * kWave
+                     ? kWave
+                     : (int)((((long long)n_hot + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1));
+  const long long hs = (long long)wave * hp;
+  const int n_hw = hs < n_hot ? (int)((hs + hp < n_hot ? hs + hp : n_hot) - hs) : 0;
+  const long long bchunk = (((long long)n + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1);
+  const long long b0 = (long long)wave * bchunk;
+  const int n_bw = b0 < n ? (int)((b0 + bchunk < n ? b0 + bchunk : n) - b0) : 0;
+  long long next = 0;
+  const long long end = (long long)n_hw + n_bw;
+  // the waves holding the chains issue first; the others fill the free slots
+  if (n_hw > 0) __builtin_amdgcn_s_setprio(3);
+  int hcnt = 0;         // entries of this wave's hot list for the next step
+  int pend = 0;         // bit 0: neuron finished this iteration, bit 1: it is hot
+  int pend_i = 0;
   const float h_min = a.h_min;
   const float r_eps = recip_rn(ck::eps);
   float y[M::NV], p[M::NPAR], r[M::NPAR], k1[M::NV], ys[M::NV];
@@ -412,9 +448,21 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
     const unsigned long long m = __ballot(want);
     if (m) {
       if (want) {
-        const int idx = next + __popcll(m & ((1ull << lane) - 1ull));
-        if (idx < end) {
-          i = idx;
+        const long long v = next + __popcll(m & ((1ull << lane) - 1ull));
+        // (written without a short-circuit on the flag: that form was
+        // compiled with the candidate register overwritten by the flag address)
+        int cand = -1;
+        if (v < end) {
+          if (v < n_hw) {
+            cand = dense[hs + v];
+          } else {
+            const int b = (int)(b0 + (v - n_hw));
+            const int f = flag_in ? (int)flag_in[b] : 0;
+            cand = f ? -1 : b;
+          }
+        }
+        if (cand >= 0) {
+          i = cand;
 #pragma unroll
           for (int j = 0; j < M::NV; ++j) y[j] = a.var[(size_t)j * n + i];
           if (UNI) {
@@ -434,7 +482,9 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
       }
       next += __popcll(m);
     }
-    if (!__any(i >= 0)) break;
+    // all lanes idle with the chunk not exhausted (every candidate was on
+    // the hot list): the masked iteration below does nothing, refill again
+    if (!__any(i >= 0) && next >= end) break;
     if constexpr (MODE == 2) {
       // learned proposal for every lane starting a substep (whole wave on
       // the matrix cores; lanes without a neuron feed zeros)
@@ -448,8 +498,12 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           for (int k = 0; k < kPI; ++k) raw[k] = 0.0f;
         }
         const float o = pred_infer_wave(s_pred, raw);
-        // floor at 2 h_min: a proposal at h_min would be accepted whatever its error
-        if (start && o > -60.0f && o < 60.0f) h = fmaxf(a.dt * exp2f(o), 2.0f * h_min);
+        // the learned proposal only shortens the controller's (the error
+        // scale |y| + |k1 h| + 0.1 grows with h, so a longer step than the
+        // controller's could pass the test less accurately); floored at
+        // 2 h_min, since a step at h_min is accepted whatever its error
+        if (start && o > -60.0f && o < 60.0f)
+          h = fminf(h, fmaxf(a.dt * exp2f(o), 2.0f * h_min));
       }
     }
     if (i >= 0) {
@@ -553,12 +607,84 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           a.h[i] = h;
           if constexpr (MODE != 0) a.err_last[i] = el;
           if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
+          pend = 1 | ((p[M::i_ref] <= 0.0f && y[M::iV] >= M::hot_v(p)) ? 2 : 0);
+          pend_i = i;
           i = -1;
         }
       }
     }
+    // neurons that finished the step: next step's hot list and flags
+    if (a.hot_flag) {
+      const unsigned long long mh = __ballot(pend == 3);
+      if (pend) {
+        bool listed = false;
+        if (pend == 3) {
+          const int pos = hcnt + __popcll(mh & ((1ull << lane) - 1ull));
+          if (pos < a.hot_k) {
+            reg_out[pos] = pend_i;
+            listed = true;
+          }
+        }
+        flag_out[pend_i] = listed ? 1 : 0;
+        pend = 0;
+      }
+      hcnt += __popcll(mh);
+    }
+  }
+  if (a.hot_flag && lane == 0) {
+    const int listed = hcnt < a.hot_k ? hcnt : a.hot_k;
+    a.hot_cnt[(size_t)(parity ^ 1) * n_waves + wave] = listed;
+    if (a.ctr) a.ctr[(size_t)wave * kCtrCols + 4] += (unsigned long long)listed;
   }
   add_wave_counters(a.ctr, wave, lane, n_att, n_rej, n_sub);
+}
+
+// Scan of the per-wave counts (one block): offsets of each wave's list in
+// the packed list of the next step (buffers of parity ^ 1) and its length.
+__global__ void __launch_bounds__(1024) hot_scan_kernel(HotGatherArgs a) {
+  __shared__ int part[1024];
+  const int q = a.ctx->parity ^ 1;
+  const int* cnt = a.hot_cnt + (size_t)q * a.waves;
+  const int per = (a.waves + blockDim.x - 1) / blockDim.x;
+  const int w0 = threadIdx.x * per;
+  int sum = 0;
+  for (int k = 0; k < per; ++k)
+    if (w0 + k < a.waves) sum += cnt[w0 + k];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  // inclusive scan (Hillis-Steele) of the per-thread sums
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int o = part[threadIdx.x] - sum;
+  for (int k = 0; k < per; ++k) {
+    const int w = w0 + k;
+    if (w >= a.waves) break;
+    a.hot_off[w] = o;
+    o += cnt[w];
+  }
+  if (threadIdx.x == blockDim.x - 1) a.hot_n[q] = part[threadIdx.x];
+}
+
+// Copy: one wave per per-wave list, 64 entries per iteration.
+__global__ void __launch_bounds__(256) hot_copy_kernel(HotGatherArgs a) {
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (w >= a.waves) return;
+  const int q = a.ctx->parity ^ 1;
+  const int c = a.hot_cnt[(size_t)q * a.waves + w];
+  const int o = a.hot_off[w];
+  const int* reg = a.hot_reg + ((size_t)q * a.waves + w) * a.hot_k;
+  int* dense = a.hot_dense + (size_t)q * a.waves * a.hot_k;
+  for (int e = lane; e < c; e += kWave) dense[o + e] = reg[e];
+}
+
+void launch_hot_gather(const HotGatherArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(hot_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(hot_copy_kernel, dim3((a.waves + 3) / 4), dim3(256), 0, s, a);
 }
 
 static int persistent_grid(int n, int max_blocks) {

@@ -1,9 +1,9 @@
 // Learned step-size predictor: an MLP 8 -> 32 -> 32 -> 1 (ReLU) that proposes
-// each RK5 substep's step size from the neuron's state, in place of the
-// proposal of the RK5 controller (src/rk5.h:135-153).  The embedded error
-// estimate still accepts or rejects every attempt, and a rejected attempt
-// falls back to the controller's shrink rule, so the plain integrator stays
-// the judge.  No counterpart exists in the reference.
+// each RK5 substep's step size from the neuron's state; the step tried is the
+// smaller of its proposal and the RK5 controller's (src/rk5.h:135-153).  The
+// embedded error estimate still accepts or rejects every attempt, and a
+// rejected attempt falls back to the controller's shrink rule, so the plain
+// integrator stays the judge.  No counterpart exists in the reference.
 //
 // Parameter vector (floats; the C ABI uses the same layout):
 //   mu[8], inv_sigma[8]          input normalisation (from the training log)

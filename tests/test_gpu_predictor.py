@@ -102,6 +102,28 @@ def test_training_gradient_matches_oracle(ngpu):
     np.testing.assert_allclose(g, gref[16:], rtol=1e-4, atol=1e-5 * np.abs(gref).max())
 
 
+def test_learned_mode_with_non_binding_predictor_is_plain(ngpu):
+    # a predictor whose proposal always exceeds the controller's leaves the
+    # step sequence unchanged: the learned-mode kernel must then reproduce
+    # the plain run bit for bit (rasters, state, attempt counts)
+    P = np.zeros(O.mlp_n_params(), np.float32)
+    P[8:16] = 1.0
+    P[O_B3] = 10.0          # log2(h/dt) = 10: h = 1024 dt
+    ngpu.SetPredictorParams(P)
+    net = netbuild.c1_network(seed=7)
+    res = []
+    for mode in ("plain", "learned"):
+        t, nd, st = _run(ngpu, net, 60.0, mode)
+        v = np.array(ngpu.GetStatus(ngpu.NodeSeq(0, net["n"]), "V_m"))
+        res.append((t, nd, v, st))
+    (t0, n0, v0, s0), (t1, n1, v1, s1) = res
+    assert len(t0) > 50
+    np.testing.assert_array_equal(t0, t1)
+    np.testing.assert_array_equal(n0, n1)
+    np.testing.assert_array_equal(v0, v1)
+    assert s0["attempts"] == s1["attempts"] and s0["rejected"] == s1["rejected"]
+
+
 def _stats(t_idx, node, n, t0_idx, t1_idx):
     m = (t_idx >= t0_idx) & (t_idx < t1_idx)
     rate = m.sum() / n / ((t1_idx - t0_idx) * 1e-4)
@@ -114,12 +136,14 @@ def _stats(t_idx, node, n, t0_idx, t1_idx):
     return rate, float(np.mean(cvs))
 
 
-def _run(ngpu, net, sim_ms, mode):
+def _run(ngpu, net, sim_ms, mode, poiss_w=0.27, seed=None):
     from test_gpu_parity import _gpu_network
     ngpu.ResetKernel()
     ngpu.SetVerbosityLevel(0)
+    if seed is not None:
+        ngpu.SetRandomSeed(seed)     # Poisson input realisation
     ngpu.SetStepPredictorMode(mode)
-    t, nd = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", sim_ms, poiss_w=0.27,
+    t, nd = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", sim_ms, poiss_w=poiss_w,
                          status=netbuild.C1_RECEPTORS)
     st = ngpu.GetStepStats()
     ngpu.SetStepPredictorMode(0)
@@ -128,16 +152,19 @@ def _run(ngpu, net, sim_ms, mode):
 
 def test_learned_step_size_gain(ngpu):
     # 1) plain RK5 with logging on one network, 2) training on the GPU,
-    # 3) plain vs learned on another network: fewer RK5 attempts, far fewer
-    # rejections, same spike statistics (rate, CV) after the first 100 ms
+    # 3) plain vs learned on another network, four Poisson realisations
+    # each: far fewer rejected attempts, fewer attempts, and mean rate and
+    # ISI CV (after the first 100 ms) inside the range of the plain
+    # realisations (rate varies by ~3% and CV by ~10% between realisations
+    # of this network, so single runs cannot be compared)
     n = 2000
     train_net = netbuild.c1_network(seed=11, n=n, k=100)
-    ngpu.StartPredictorLog(4_000_000, 3)
+    ngpu.StartPredictorLog(4_000_000, 5)   # ~2.5M records: the log never truncates
     _run(ngpu, train_net, 600.0, "log")
     n_log = ngpu.GetPredictorLogCount()
     recs = ngpu.GetPredictorLog(0, min(n_log, 200000))
     print("logged records", n_log, "censored fraction %.3f" % recs[:, 9].mean())
-    assert n_log > 1_000_000
+    assert 1_000_000 < n_log < 4_000_000
     assert np.all(np.isfinite(recs))
     ngpu.InitPredictor(7)
     loss = ngpu.TrainPredictor(3000, 8192, 2e-3, 3)
@@ -145,15 +172,23 @@ def test_learned_step_size_gain(ngpu):
     assert loss[-50:].mean() < 0.5 * loss[:20].mean()
 
     eval_net = netbuild.c1_network(seed=12, n=n, k=100)
-    tp, np_, sp = _run(ngpu, eval_net, 500.0, "plain")
-    tl, nl, sl = _run(ngpu, eval_net, 500.0, "learned")
-    rp, cp = _stats(tp, np_, n, 1000, 5000)
-    rl, cl = _stats(tl, nl, n, 1000, 5000)
-    fp = sp["rejected"] / sp["attempts"]
-    fl = sl["rejected"] / sl["attempts"]
-    print("plain:   attempts %d rejected %.4f rate %.3f Hz CV %.4f" % (sp["attempts"], fp, rp, cp))
-    print("learned: attempts %d rejected %.4f rate %.3f Hz CV %.4f" % (sl["attempts"], fl, rl, cl))
+    res = {}
+    for mode in ("plain", "learned"):
+        rates, cvs, att, rej = [], [], 0, 0
+        for seed in (1234, 999, 555, 77):
+            t, nd, st = _run(ngpu, eval_net, 500.0, mode, seed=seed)
+            r, cv = _stats(t, nd, n, 1000, 5000)
+            rates.append(r)
+            cvs.append(cv)
+            att += st["attempts"]
+            rej += st["rejected"]
+        res[mode] = (np.array(rates), np.array(cvs), att, rej / att)
+        print("%-7s rates %s CV %s attempts %d rejected %.4f"
+              % (mode, np.round(rates, 3), np.round(cvs, 4), att, rej / att))
+    (rp, cp, ap, fp), (rl, cl, al, fl) = res["plain"], res["learned"]
     assert fl < 0.5 * fp
-    assert sl["attempts"] < sp["attempts"]
-    assert abs(rl - rp) < 0.03 * rp
-    assert abs(cl - cp) < 0.03 * cp
+    assert al < ap
+    # mean over the learned realisations inside the range of the plain
+    # realisations, widened by 3%
+    assert 0.97 * rp.min() <= rl.mean() <= 1.03 * rp.max()
+    assert 0.97 * cp.min() <= cl.mean() <= 1.03 * cp.max()
