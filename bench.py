@@ -184,8 +184,8 @@ def main():
     ap.add_argument("--predictor", default=None,
                     help="trained predictor JSON ('none' to skip the learned window)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=10000)
-    ap.add_argument("--cpu-ms", type=float, default=100.0)
+    ap.add_argument("--cpu-sample", type=int, default=20000)
+    ap.add_argument("--cpu-ms", type=float, default=1000.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB fill and a 1 GiB elementwise multiply (torch) whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -239,6 +239,9 @@ def main():
 
     learned = None
     if predictor is not None:
+        # plain (above), learned, plain again: the learned window's spike
+        # statistics are compared with the mean of the two plain windows
+        # around it, so a slow drift of the network is not read as an effect
         ngpu.SetPredictorParams(np.asarray(predictor["params"], np.float32))
         ngpu.SetStepPredictorMode("learned")
         ngpu.Simulate(200 * DT)     # switch-over, untimed
@@ -247,26 +250,48 @@ def main():
         ngpu.SetStepPredictorMode("plain")
         sl = window_summary(wl, args.steps, n)
         log("learned: wall %.3f s, RTF %.4f, kernel ms %s" % (wl["wall"], sl["rtf"], wl["kt"]))
+        ngpu.Simulate(200 * DT)
+        t_plain2 = t_learn0 + (args.steps + 200) * DT
+        wp2 = timed_window(ngpu, args.steps)
+        sp2 = window_summary(wp2, args.steps, n)
+        log("plain again: wall %.3f s, RTF %.4f" % (wp2["wall"], sp2["rtf"]))
         t0 = time.perf_counter()
         times = ngpu.GetRecSpikeTimes(neurons)
         rp, cvp, ncp = window_spike_stats(times, t_plain0, t_plain0 + args.steps * DT, n)
         rl, cvl, ncl = window_spike_stats(times, t_learn0, t_learn0 + args.steps * DT, n)
-        log("spike statistics in %.1f s: plain rate %.3f CV %s (%d), learned rate %.3f CV %s (%d)"
-            % (time.perf_counter() - t0, rp, cvp, ncp, rl, cvl, ncl))
+        rp2, cvp2, _ = window_spike_stats(times, t_plain2, t_plain2 + args.steps * DT, n)
+        log("spike statistics in %.1f s: plain rate %.3f CV %s, learned rate %.3f CV %s, plain again "
+            "rate %.3f CV %s" % (time.perf_counter() - t0, rp, cvp, rl, cvl, rp2, cvp2))
         sp["isi_cv"] = cvp
         sl["isi_cv"] = cvl
+        sp2["isi_cv"] = cvp2
+        r_ref = 0.5 * (rp + rp2)
+        cv_ref = 0.5 * (cvp + cvp2) if (cvp and cvp2) else None
         learned = dict(sl)
-        learned["speedup_vs_plain"] = sp["rtf"] / sl["rtf"]
-        learned["rk5_kernel_speedup_vs_plain"] = (wp["kt"]["rk5_update"][0] / max(wl["kt"]["rk5_update"][0], 1e-9))
-        learned["rate_rel_diff"] = (rl - rp) / rp if rp else None
-        learned["cv_rel_diff"] = (cvl - cvp) / cvp if (cvp and cvl) else None
+        learned["plain_after"] = sp2
+        learned["speedup_vs_plain"] = 0.5 * (sp["rtf"] + sp2["rtf"]) / sl["rtf"]
+        learned["rk5_kernel_speedup_vs_plain"] = (0.5 * (wp["kt"]["rk5_update"][0] + wp2["kt"]["rk5_update"][0])
+                                                  / max(wl["kt"]["rk5_update"][0], 1e-9))
+        learned["rate_rel_diff"] = (rl - r_ref) / r_ref if r_ref else None
+        learned["cv_rel_diff"] = (cvl - cv_ref) / cv_ref if (cv_ref and cvl) else None
         learned["predictor"] = {k: predictor[k] for k in predictor if k != "params"}
 
-    # dominant kernel of the plain window
+    # dominant kernel of the plain window; HBM traffic per launch from the
+    # committed FETCH_SIZE/WRITE_SIZE summary of this workload, if there is one
     dom = max(wp["kt"], key=lambda k: wp["kt"][k][0])
     roof = dict(roof_all.get(dom, {}))
-    roof["kernel"] = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>>" % n_port}.get(dom, dom)
+    kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
+             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel"}.get(dom, dom)
+    roof["kernel"] = kname
     roof["traffic"] = None
+    tpath = os.path.join(ROOT, "profiles", "%s_traffic.json" % args.workload)
+    if os.path.exists(tpath) and n == N_FULL:
+        with open(tpath) as f:
+            tr = json.load(f)
+        if kname in tr["kernels"]:
+            roof["traffic"] = tr["kernels"][kname]["bytes"]
+            roof["traffic_source"] = "profiles/%s_traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, " \
+                                     "scaled by fill/copy calibration)" % args.workload
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "scale_adex":
