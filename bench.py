@@ -99,11 +99,12 @@ def cpu_baseline(n_sample, sim_ms, seed):
             "syn_events_per_s": (st1["syn_events"] - st0["syn_events"]) / wall}
 
 
-def timed_window(ngpu, steps):
+def timed_window(ngpu, steps, timing=2):
     """Simulate `steps` steps between two synchronisations; returns wall time,
-    kernel times (HIP events on the simulator stream) and statistics deltas."""
+    kernel times (HIP events on the simulator stream: timing 2 brackets only
+    the RK5 launches, 1 every launch) and statistics deltas."""
     st0 = ngpu.GetStepStats()
-    ngpu.SetKernelTiming(True)
+    ngpu.SetKernelTiming(timing)
     ngpu.Synchronize()
     t0 = time.perf_counter()
     ngpu.Simulate(steps * DT)
@@ -113,7 +114,7 @@ def timed_window(ngpu, steps):
     st1 = ngpu.GetStepStats()
     ngpu.SetKernelTiming(False)
     d = {k: st1[k] - st0[k] for k in ("attempts", "rejected", "substeps", "syn_events", "spikes",
-                                      "hot_listed")}
+                                      "hot_listed", "split_steps")}
     return {"wall": wall, "kt": kt, "stats": d}
 
 
@@ -170,6 +171,7 @@ def window_summary(w, steps, n):
             "rk5_rejected_fraction": st["rejected"] / max(st["attempts"], 1),
             "mean_rate_hz": st["spikes"] / n / bio,
             "hot_listed_per_step": st["hot_listed"] / steps,
+            "split_steps_per_step": st["split_steps"] / steps,
             "kernel_ms": {k: v[0] for k, v in w["kt"].items()}}
 
 
@@ -177,7 +179,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000, help="timed 0.1 ms steps per window (10000 = 1 s)")
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=3000,
+                    help="untimed steps first (the network's rate settles within ~300 ms)")
     ap.add_argument("--workload", default="scale_adex", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--neurons", type=int, default=N_FULL)
     ap.add_argument("--seed", type=int, default=None)
@@ -235,7 +238,6 @@ def main():
     wp = timed_window(ngpu, args.steps)
     sp = window_summary(wp, args.steps, n)
     log("plain: %d steps, wall %.3f s, RTF %.4f, kernel ms %s" % (args.steps, wp["wall"], sp["rtf"], wp["kt"]))
-    roof_all = rooflines(wp, n, n_port, args.steps, n_poisson_edges)
 
     learned = None
     if predictor is not None:
@@ -276,10 +278,19 @@ def main():
         learned["cv_rel_diff"] = (cvl - cv_ref) / cv_ref if (cv_ref and cvl) else None
         learned["predictor"] = {k: predictor[k] for k in predictor if k != "params"}
 
-    # dominant kernel of the plain window; HBM traffic per launch from the
-    # committed FETCH_SIZE/WRITE_SIZE summary of this workload, if there is one
-    dom = max(wp["kt"], key=lambda k: wp["kt"][k][0])
-    roof = dict(roof_all.get(dom, {}))
+    # every launch timed (events around each kernel slow the step a little,
+    # so this window is reported apart from the timed windows above): the
+    # per-kernel split of a step and the rooflines of all kernels
+    nb = min(args.steps, 2000)
+    wb = timed_window(ngpu, nb, timing=1)
+    log("breakdown: %d steps, wall %.3f s, kernel ms %s" % (nb, wb["wall"], wb["kt"]))
+    roof_all = rooflines(wb, n, n_port, nb, n_poisson_edges)
+    breakdown = {k: v[0] * 1e3 / nb for k, v in wb["kt"].items()}
+    # the dominant kernel's roofline from the plain timed window (its RK5
+    # launches were bracketed by events there); HBM traffic per launch from
+    # the committed FETCH_SIZE/WRITE_SIZE summary of this workload, if any
+    dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
+    roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_edges).get(dom, roof_all.get(dom, {})))
     kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
              "deliver": "deliver_kernel", "fold_poisson": "fold_kernel"}.get(dom, dom)
     roof["kernel"] = kname
@@ -322,6 +333,7 @@ def main():
         "rk5_attempts_per_neuron_step": sp["rk5_attempts_per_neuron_step"],
         "rk5_rejected_fraction": sp["rk5_rejected_fraction"],
         "kernel_ms": sp["kernel_ms"],
+        "kernel_us_per_step": breakdown,
         "plain": sp,
         "learned": learned,
         "roofline": roof,

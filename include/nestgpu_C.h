@@ -207,13 +207,15 @@ int NESTGPU_SetIntParam(char *param_name, int val);
 int NESTGPU_RemoteCreate(int i_host, char *model_name, int n_neuron, int n_port);
 
 /* ---- instrumentation of this build (no counterpart in the reference) ----
- * out[0..6] = RK5 attempts, rejected attempts, accepted substeps, delivered
+ * out[0..7] = RK5 attempts, rejected attempts, accepted substeps, delivered
  * (spike, synapse) events, emitted spikes, spikes dropped by a full spike
- * list, neurons put on the RK5 kernel's hot-first lists; all since
- * calibration (zeros before). */
+ * list, neurons put on the RK5 kernel's hot-first lists, neuron-steps split
+ * over two RK5 launches; all since calibration (zeros before). */
 int NESTGPU_AMD_GetStepStats(unsigned long long *out);
-/* Kernel timing with HIP events on the simulator's stream: on != 0 starts
- * (and zeroes) accumulation.  ms[5]/launches[5] receive the summed device
+/* Kernel timing with HIP events on the simulator's stream: on = 1 times
+ * every launch, on = 2 only the RK5 launches (two events per step instead
+ * of two per launch), 0 stops; a call zeroes the accumulation.
+ * ms[5]/launches[5] receive the summed device
  * time and launch count of: RK5 neuron update, iaf_psc_exp update, spike
  * delivery, fold + Poisson input, other (generators, records, step end). */
 int NESTGPU_AMD_SetKernelTiming(int on);

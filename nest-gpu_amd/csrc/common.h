@@ -22,6 +22,11 @@ namespace ngpu {
 constexpr int kMaxPort = 8;             // ports per RK5 population in this build
 constexpr int kMaxParam = 14 + 4 * kMaxPort;
 constexpr int kWave = 64;
+// persistent RK5 kernel: threads per block, and LDS entries per wave for the
+// neurons it lists hot in one launch (~1 per wave at 5 Hz; a wave with more
+// leaves the rest unlisted, which only delays them)
+constexpr int kRK5Block = 256;
+constexpr int kHotLds = 128;
 
 // Time of the step being computed, kept in device memory so the step's
 // kernels can be replayed (HIP graphs) without host arguments.
@@ -44,10 +49,14 @@ struct ParamTable {
   }
 };
 
-// Spike list of one step (double buffered by t_idx parity).
+// Spike list of one step (double buffered by t_idx parity).  A source with
+// bit 31 set (kSpikeLagBit) was emitted in the previous step by a neuron
+// whose step was split over two RK5 launches; it is delivered one slot
+// earlier, i.e. with its own emission time.
+constexpr unsigned int kSpikeLagBit = 0x80000000u;
 struct SpikeList {
   int* count;          // [2]
-  int* src;            // [2][cap]
+  int* src;            // [2][cap] global node id (| kSpikeLagBit)
   float* height;       // [2][cap]
   int cap;
   int* overflow;       // number of spikes dropped because the list was full

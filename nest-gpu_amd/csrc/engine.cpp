@@ -98,6 +98,25 @@ static const char* kOtherModels[] = {
     "aeif_psc_alpha_multisynapse", "parrot_neuron", "izhikevich", "izhikevich_psc_exp_5s",
     "izhikevich_psc_exp_2s", "izhikevich_psc_exp", "user_m1", "user_m2"};
 
+// Where a model's port input lands: variable index of port 0's input and the
+// stride between ports, parameter index of port 0's weight (-1: weight 1).
+struct FoldLayout {
+  int in_var, in_port_stride, w_param, w_port_stride;
+};
+static FoldLayout fold_layout(const ModelDesc& m) {
+  FoldLayout f;
+  if (m.in_port_block) {
+    f.in_var = (int)m.scal_var.size() + m.in_var;
+    f.in_port_stride = (int)m.port_var.size();
+  } else {
+    f.in_var = m.in_var;
+    f.in_port_stride = 1;
+  }
+  f.w_param = m.w_param >= 0 ? (int)m.scal_param.size() + m.w_param : -1;
+  f.w_port_stride = (int)m.port_param.size();
+  return f;
+}
+
 const ModelDesc* find_model(const std::string& name) {
   for (auto& m : model_table())
     if (name == m.name) return &m;
@@ -1387,6 +1406,10 @@ int Engine::Calibrate() {
     rk5_max_blocks_ = cus * 4;
   }
   if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_RK5_DEFER")) {
+    rk5_defer_cap_ = std::max(0, atoi(env));
+    rk5_defer_env_ = true;
+  }
   // per-wave counter rows: as many as the widest launch that adds to them
   long long rows = 4LL * kDeliverBlocks;
   for (auto* g : groups_)
@@ -1463,19 +1486,22 @@ int Engine::Calibrate() {
       }
       const char* hot_env = getenv("NGPU_RK5_HOT");
       if (rk5_persistent_ && !(hot_env && atoi(hot_env) == 0)) {
-        // hot-first lists: one region per wave, as long as a wave's chunk + 64
-        g->hot_waves = (int)rk5_waves(g->n, 1, rk5_max_blocks_);
-        const int chunk = ((g->n + g->hot_waves - 1) / g->hot_waves + 63) & ~63;
-        g->hot_k = chunk + 64;
+        // hot-first lists (flags by parity, lists by t_idx mod 3)
         g->hot_flag.alloc((size_t)2 * g->n);
-        g->hot_reg.alloc((size_t)2 * g->hot_waves * g->hot_k);
-        g->hot_dense.alloc((size_t)2 * g->hot_waves * g->hot_k);
-        g->hot_cnt.alloc((size_t)2 * g->hot_waves);
-        g->hot_off.alloc(g->hot_waves);
-        g->hot_n.alloc(2);
+        g->hot_list.alloc((size_t)3 * g->n);
+        g->hot_count.alloc(3);
         HIPCHK(hipMemsetAsync(g->hot_flag.p, 0, g->hot_flag.n, stream_));
-        HIPCHK(hipMemsetAsync(g->hot_cnt.p, 0, g->hot_cnt.n * sizeof(int), stream_));
-        HIPCHK(hipMemsetAsync(g->hot_n.p, 0, 2 * sizeof(int), stream_));
+        HIPCHK(hipMemsetAsync(g->hot_count.p, 0, 3 * sizeof(int), stream_));
+      }
+      if (rk5_persistent_ && rk5_defer_cap_ > 0) {
+        // split steps: the kernel adds a lagged neuron's stashed input itself
+        const FoldLayout fl = fold_layout(*g->md);
+        if (!rk5_fold_layout_matches(g->md->id, g->n_port, fl.in_var, fl.in_port_stride,
+                                     fl.w_param, fl.w_port_stride))
+          throw Error(std::string("RK5 input layout of ") + g->md->name + " differs from the fold's");
+        g->lag.alloc(g->n);
+        g->pend.alloc((size_t)std::max(g->n_port, 1) * g->n);
+        HIPCHK(hipMemsetAsync(g->lag.p, 0, g->lag.n, stream_));
       }
     }
     for (auto& iv : g->int_var) {
@@ -1534,6 +1560,26 @@ static ParamTable make_table(const Group& g) {
   return t;
 }
 
+FoldArgs Engine::make_fold_args(const Group& g) {
+  FoldArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.n = g.n; fa.n_port = g.n_port; fa.ch_base = g.ch_base; fa.var = g.var.p;
+  const FoldLayout fl = fold_layout(*g.md);
+  fa.in_var = fl.in_var;
+  fa.in_port_stride = fl.in_port_stride;
+  fa.param = g.param.p; fa.tab = make_table(g);
+  fa.w_param = fl.w_param;
+  fa.w_port_stride = fl.w_port_stride;
+  fa.ring = d_ring_.p; fa.n_ch = n_ch_; fa.ctx = d_ctx_.p;
+  if (g.has_pe) {
+    fa.pe_off = g.pe_off.p; fa.pe_gen = g.pe_gen.p; fa.pe_w = g.pe_w.p; fa.pe_delay = g.pe_delay.p;
+    fa.pe_stream = g.pe_stream.p; fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
+    fa.gen_origin = d_gen_origin_.p; fa.gen_start = d_gen_start_.p; fa.gen_stop = d_gen_stop_.p;
+    fa.seed = kernel_seed_;
+  }
+  return fa;
+}
+
 void Engine::launch_step() {
   SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
   for (size_t gi = 0; gi < groups_.size(); ++gi) {
@@ -1563,25 +1609,21 @@ void Engine::launch_step() {
         }
         if (g->hot_flag.p) {
           a.hot_flag = g->hot_flag.p;
-          a.hot_reg = g->hot_reg.p;
-          a.hot_cnt = g->hot_cnt.p;
-          a.hot_dense = g->hot_dense.p;
-          a.hot_n = g->hot_n.p;
-          a.hot_k = g->hot_k;
+          a.hot_list = g->hot_list.p;
+          a.hot_count = g->hot_count.p;
+        }
+        if (g->lag.p) {
+          a.lag = g->lag.p;
+          a.pend = g->pend.p;
+          // no split in a population a multimeter reads (it would see
+          // mid-step states), nor in the last step of a Simulate() call
+          // (state is complete between calls)
+          // learned proposals shorten the spike upswing's attempt chain, so
+          // the split point that balances two launches comes earlier
+          const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 16 : rk5_defer_cap_);
+          a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
         }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
-        if (g->hot_flag.p) {
-          HotGatherArgs ha{};
-          ha.ctx = d_ctx_.p;
-          ha.hot_reg = g->hot_reg.p;
-          ha.hot_cnt = g->hot_cnt.p;
-          ha.hot_off = g->hot_off.p;
-          ha.hot_dense = g->hot_dense.p;
-          ha.hot_n = g->hot_n.p;
-          ha.waves = g->hot_waves;
-          ha.hot_k = g->hot_k;
-          timed(KT_OTHER, [&] { launch_hot_gather(ha, stream_); });
-        }
         break;
       }
       case MODEL_IAF_PSC_EXP: {
@@ -1619,27 +1661,9 @@ void Engine::launch_step() {
   timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
-    FoldArgs fa{};
-    std::memset(&fa, 0, sizeof(fa));
-    fa.n = g->n; fa.n_port = g->n_port; fa.ch_base = g->ch_base; fa.var = g->var.p;
-    const ModelDesc& m = *g->md;
-    if (m.in_port_block) {
-      fa.in_var = (int)m.scal_var.size() + m.in_var;
-      fa.in_port_stride = (int)m.port_var.size();
-    } else {
-      fa.in_var = m.in_var;
-      fa.in_port_stride = 1;
-    }
-    fa.param = g->param.p; fa.tab = make_table(*g);
-    fa.w_param = m.w_param >= 0 ? (int)m.scal_param.size() + m.w_param : -1;
-    fa.w_port_stride = (int)m.port_param.size();
-    fa.ring = d_ring_.p; fa.n_ch = n_ch_; fa.ctx = d_ctx_.p;
-    if (g->has_pe) {
-      fa.pe_off = g->pe_off.p; fa.pe_gen = g->pe_gen.p; fa.pe_w = g->pe_w.p; fa.pe_delay = g->pe_delay.p;
-      fa.pe_stream = g->pe_stream.p; fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
-      fa.gen_origin = d_gen_origin_.p; fa.gen_start = d_gen_start_.p; fa.gen_stop = d_gen_stop_.p;
-      fa.seed = kernel_seed_;
-    }
+    FoldArgs fa = make_fold_args(*g);
+    fa.lag = g->lag.p;
+    fa.pend = g->pend.p;
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
   timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, stream_); });
@@ -1647,7 +1671,7 @@ void Engine::launch_step() {
 }
 
 void Engine::timed(int cat, const std::function<void()>& launch) {
-  if (!timing_) {
+  if (!timing_ || (timing_rk5_only_ && cat != KT_RK5)) {
     launch();
     HIPCHK(hipGetLastError());   // a launch that cannot run fails the step loudly
     return;
@@ -1684,9 +1708,10 @@ void Engine::flush_timing() {
   ev_pending_.clear();
 }
 
-void Engine::SetKernelTiming(bool on) {
-  if (timing_ && !on) flush_timing();
-  timing_ = on;
+void Engine::SetKernelTiming(int mode) {
+  if (timing_) flush_timing();
+  timing_ = mode != 0;
+  timing_rk5_only_ = mode == 2;
   for (int k = 0; k < KT_N; ++k) { kt_ms_[k] = 0.0; kt_n_[k] = 0; }
 }
 
@@ -1735,9 +1760,11 @@ int Engine::StartSimulation() {
   HIPCHK(hipMemcpyAsync(d_ctx_.p, &c, sizeof(c), hipMemcpyHostToDevice, stream_));
   // record columns and buffer for this call
   std::vector<RecCol> cols;
+  for (auto* g : groups_) g->recorded = false;
   for (auto& r : records_)
     for (size_t k = 0; k < r.var.size(); ++k) {
       Group* g = groups_[r.group[k]];
+      g->recorded = true;
       RecCol rc;
       rc.node = r.node[k];
       rc.has_out = has_out_[r.node[k]];
@@ -1819,13 +1846,17 @@ int Engine::EndSimulation() {
 int Engine::Simulate() {
   StartSimulation();
   if (verbosity_ >= 1) printf("Neural activity simulation time: %.3lf ms\n", (double)sim_time_);
-  for (long long it = 0; it < nt_; ++it) SimulationStep();
+  for (long long it = 0; it < nt_; ++it) {
+    defer_ok_ = it + 1 < nt_;
+    SimulationStep();
+  }
+  defer_ok_ = false;
   EndSimulation();
   return 0;
 }
 
 void Engine::Stats(unsigned long long* out) {
-  for (int k = 0; k < 7; ++k) out[k] = 0;
+  for (int k = 0; k < 8; ++k) out[k] = 0;
   if (!calibrated_) return;
   HIPCHK(hipStreamSynchronize(stream_));
   std::vector<unsigned long long> rows(d_ctr_.n);
@@ -1834,6 +1865,7 @@ void Engine::Stats(unsigned long long* out) {
   for (size_t r = 0; r < rows.size(); r += kCtrCols) {
     for (int k = 0; k < 4; ++k) out[k] += rows[r + k];
     out[6] += rows[r + 4];
+    out[7] += rows[r + 5];
   }
   HIPCHK(hipMemcpy(&out[4], d_stats_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
   int ov = 0;

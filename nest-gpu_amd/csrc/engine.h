@@ -59,8 +59,12 @@ struct Group {
   DBuf<float> err_last;         // last accepted RK5 error ratio (step-size predictor input)
   // hot-first scheduling of the persistent RK5 kernel
   DBuf<unsigned char> hot_flag;
-  DBuf<int> hot_reg, hot_cnt, hot_off, hot_dense, hot_n;
-  int hot_waves = 0, hot_k = 0;
+  DBuf<int> hot_list, hot_count;
+  // split RK5 steps (persistent kernel): lagged flag, and the fold input
+  // stashed for lagged neurons when the fold is a separate launch
+  DBuf<unsigned char> lag;
+  DBuf<double> pend;
+  bool recorded = false;        // a multimeter column reads this population
   std::vector<float> uni_val;   // value of each parameter if uniform
   std::vector<char> uni;        // parameter is equal over the population
   float h_min_rel = 1.0e-3f, h0_rel = 1.0e-2f, h_min = 0.f, h0 = 0.f;
@@ -216,7 +220,7 @@ class Engine {
   // Kernel timing with HIP events on the simulator stream, per category:
   // 0 RK5 update, 1 iaf_psc_exp update, 2 delivery, 3 fold+Poisson, 4 other.
   enum { KT_RK5 = 0, KT_IAF, KT_DELIVER, KT_FOLD, KT_OTHER, KT_N };
-  void SetKernelTiming(bool on);
+  void SetKernelTiming(int mode);   // 0 off, 1 every launch, 2 RK5 launches only
   void KernelTimes(double* ms, long long* launches);
   void Synchronize();
   hipStream_t stream() const { return stream_; }
@@ -231,6 +235,7 @@ class Engine {
   void calibrate_group(Group& g);
   void upload_poisson_tables();
   void launch_step();
+  FoldArgs make_fold_args(const Group& g);
   void fetch_records(long long n_rows);
   void buffer_rec_spike_times(Group& g);
   int param_idx(const Group& g, const std::string& name, int port) const;
@@ -280,6 +285,9 @@ class Engine {
   DBuf<float> d_last_h_;
   DBuf<StepCtx> d_ctx_;
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
+  int rk5_defer_cap_ = 24;             // attempts after which a step is split over two launches (0: never)
+  bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
+  bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]
   DBuf<unsigned long long> d_stats_;   // delivered spikes
@@ -296,6 +304,7 @@ class Engine {
 
   hipStream_t stream_ = nullptr;
   bool timing_ = false;
+  bool timing_rk5_only_ = false;
   std::vector<hipEvent_t> ev_free_;
   struct EvPair { hipEvent_t a, b; int cat; };
   std::vector<EvPair> ev_pending_;

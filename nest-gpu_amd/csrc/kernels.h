@@ -10,8 +10,9 @@ namespace ngpu {
 // Per-wave counter slots: each wave of a launch adds its counts into row
 // (global wave index) with plain loads and stores, and the host sums the
 // rows when asked.  Columns: RK5 attempts, rejected attempts, accepted
-// substeps, delivered (spike, synapse) pairs, neurons put on a hot list.
-constexpr int kCtrCols = 5;
+// substeps, delivered (spike, synapse) pairs, neurons put on a hot list,
+// neuron-steps split over two RK5 launches.
+constexpr int kCtrCols = 6;
 constexpr int kDeliverBlocks = 1024;
 
 enum ModelId {
@@ -24,6 +25,41 @@ enum ModelId {
   MODEL_SPIKE_GEN,
   MODEL_SPIKE_DETECTOR,
   N_MODEL_ID
+};
+
+// Fold, src/get_spike.cu:142-166 plus poisson_generator input: the ring
+// slot of a step and the Poisson draws of that step are added to each
+// channel's input variable.
+struct FoldArgs {
+  int n, n_port;
+  long long ch_base;
+  float* var;
+  int in_var;            // var index of port 0's input
+  int in_port_stride;    // var index stride between ports
+  const float* param;
+  ParamTable tab;
+  int w_param;           // param index of port 0's weight, -1 => 1.0
+  int w_port_stride;
+  double* ring;
+  long long n_ch;
+  const StepCtx* ctx;
+  // poisson direct connections into this population, CSR by local channel
+  const int* pe_off;     // [n*n_port+1] or null
+  const int* pe_gen;     // generator index
+  const float* pe_w;
+  const float* pe_delay;
+  const unsigned int* pe_stream;
+  const int* gen_chunks;
+  const double* gen_exp;
+  const float* gen_origin;
+  const float* gen_start;
+  const float* gen_stop;
+  unsigned long long seed;
+  // split RK5 steps with this fold launch: for a neuron still inside its
+  // step (lag[i] = 1) the channel's input is stored in pend[idx] instead of
+  // added; the neuron's next RK5 launch adds it when the step ends
+  const unsigned char* lag;
+  double* pend;
 };
 
 struct RK5Args {
@@ -50,28 +86,29 @@ struct RK5Args {
   unsigned long long* log_count;   // records written (may exceed log_cap)
   long long log_cap;
   int log_stride;                  // log neurons with index % log_stride == 0
-  // hot-first scheduling of the persistent kernel (buffers by step parity)
-  unsigned char* hot_flag;         // [2][n] neuron listed hot for the step, or null
-  int* hot_reg;                    // [2][waves][hot_k] per-wave lists written at step end
-  int* hot_cnt;                    // [2][waves]
-  const int* hot_dense;            // [2][waves * hot_k] packed list of the step
-  const int* hot_n;                // [2] entries of the packed list
-  int hot_k;
+  // hot-first scheduling of the persistent kernel: neurons that ended a
+  // step about to spike (or split) are listed, and the next launch gives
+  // them one lane each in the first waves.  Each wave collects its entries
+  // in LDS and appends them with one atomic when it exits; lists by
+  // t_idx mod 3 (read, written, cleared for the launch after next).
+  unsigned char* hot_flag;         // [2][n] neuron listed hot for the step (by t_idx parity), or null
+  int* hot_list;                   // [3][n]
+  int* hot_count;                  // [3]
+  // split steps: a neuron still integrating after defer_cap attempts of a
+  // step (at a substep boundary) is stored with lag = 1, finishes that step
+  // first in the next launch, then adds that step's input (stashed in pend
+  // by the fold launch) and goes on; its spike is delivered with its own
+  // emission time
+  unsigned char* lag;              // [n] or null (no split steps)
+  double* pend;                    // [n_port][n]
+  int defer_cap;                   // 0: no new split in this launch
 };
 
-// Packs the per-wave hot lists written by an RK5 launch into the list the
-// next step's launch processes first (a one-block scan of the counts, then
-// one wave per list copying it).
-struct HotGatherArgs {
-  const StepCtx* ctx;
-  const int* hot_reg;
-  const int* hot_cnt;
-  int* hot_off;          // [waves] exclusive prefix of the counts (scratch)
-  int* hot_dense;
-  int* hot_n;
-  int waves;
-  int hot_k;
-};
+// Whether the persistent RK5 kernel's compile-time input layout of a model
+// (input variable and weight parameter per port) is the fold's layout, so a
+// lagged neuron can apply its stashed input itself.
+bool rk5_fold_layout_matches(int model_id, int n_port, int in_var, int in_port_stride,
+                             int w_param, int w_port_stride);
 
 struct IafArgs {
   int n, i0;
@@ -115,32 +152,6 @@ struct DeliverArgs {
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 
-struct FoldArgs {
-  int n, n_port;
-  long long ch_base;
-  float* var;
-  int in_var;            // var index of port 0's input
-  int in_port_stride;    // var index stride between ports
-  const float* param;
-  ParamTable tab;
-  int w_param;           // param index of port 0's weight, -1 => 1.0
-  int w_port_stride;
-  double* ring;
-  long long n_ch;
-  const StepCtx* ctx;
-  // poisson direct connections into this population, CSR by local channel
-  const int* pe_off;     // [n*n_port+1] or null
-  const int* pe_gen;     // generator index
-  const float* pe_w;
-  const float* pe_delay;
-  const unsigned int* pe_stream;
-  const int* gen_chunks;
-  const double* gen_exp;
-  const float* gen_origin;
-  const float* gen_start;
-  const float* gen_stop;
-  unsigned long long seed;
-};
 
 struct RecCol {
   const float* ptr;      // value to copy, or null for the "spike" pseudo-variable
@@ -163,7 +174,6 @@ int rk5_supported(int model, int n_port);
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);
 long long rk5_waves(int n, int persistent, int max_blocks);   // waves of one launch
 int rk5_predictor_capable(int model, int n_port);                // model supports pred_mode 1, 2
-void launch_hot_gather(const HotGatherArgs& a, hipStream_t s);
 void launch_iaf(const IafArgs& a, hipStream_t s);
 void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <stdexcept>
 #include <type_traits>
 
 #include "common.h"
@@ -82,6 +83,9 @@ struct AeifCondBeta {
   }
   // membrane potential from which a neuron is scheduled first (spike upswing)
   __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
+  // input of port k: variable g1_k, weight g0_k (src/aeif_cond_beta_multisynapse.cu:151-159)
+  static constexpr int in_y(int k) { return 3 + 2 * k; }
+  static constexpr int in_w(int k) { return 14 + 4 * k + 3; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
@@ -104,7 +108,7 @@ struct AeifCondBeta {
 // aeif_cond_alpha_multisynapse, src/aeif_cond_alpha_multisynapse_kernel.h:163-226
 template <int NP>
 struct AeifCondAlpha {
-  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 3 * NP;
+  static constexpr int NV = 2 + 2 * NP, NPAR = 14 + 3 * NP, NPORT = NP;
   enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
          t_ref, ref, den };
   static constexpr int iV = 0, iw = 1, i_reset = V_reset, i_thr = V_peak,
@@ -113,6 +117,9 @@ struct AeifCondAlpha {
     return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 3 == 1);
   }
   __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
+  // input of port k: variable g1_k, weight g0_k
+  static constexpr int in_y(int k) { return 3 + 2 * k; }
+  static constexpr int in_w(int k) { return 14 + 3 * k + 2; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
     float I_syn = 0.0f;
@@ -134,8 +141,11 @@ struct AeifCondAlpha {
 
 // aeif_psc_exp, src/aeif_psc_exp_kernel.h:145-201 (2 fixed ports, I_ex - I_in)
 struct AeifPscExp {
-  static constexpr int NV = 4, NPAR = 16;
+  static constexpr int NV = 4, NPAR = 16, NPORT = 2;
   enum { I_ex, I_in, V_m, w };
+  // input of port k: I_syn_ex / I_syn_in, weight 1
+  static constexpr int in_y(int k) { return k; }
+  static constexpr int in_w(int) { return -1; }
   enum { tau_ex, tau_in, V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak,
          V_reset, t_ref, ref, den };
   static constexpr int iV = V_m, iw = w, i_reset = V_reset, i_thr = V_peak,
@@ -164,8 +174,11 @@ struct AeifPscExp {
 // izhikevich_cond_beta, src/izhikevich_cond_beta_kernel.h:152-213
 template <int NP>
 struct IzhCondBeta {
-  static constexpr int NV = 2 + 2 * NP, NPAR = 9 + 4 * NP;
+  static constexpr int NV = 2 + 2 * NP, NPAR = 9 + 4 * NP, NPORT = NP;
   enum { V_th, a, b, c, d, I_e, t_ref, ref, den };
+  // input of port k: variable g1_k, weight g0_k
+  static constexpr int in_y(int k) { return 3 + 2 * k; }
+  static constexpr int in_w(int k) { return 9 + 4 * k + 3; }
   static constexpr int iV = 0, iw = 1, i_reset = c, i_thr = V_th, i_jump = d,
                        i_tref = t_ref, i_ref = ref;
   static constexpr bool divisor(int j) { return j >= 9 && (j - 9) % 4 >= 1 && (j - 9) % 4 <= 2; }
@@ -198,10 +211,10 @@ template <int NP> struct PredCapable<AeifCondBeta<NP>> : std::true_type {};
 // bookkeeping of src/spike_buffer.cu:162-217.
 __device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
                                   int parity, int node, int rel, float height,
-                                  long long t_idx, double t1) {
+                                  long long t_idx, double t1, bool lag = false) {
   int pos = atomicAdd(&sl.count[parity], 1);
   if (pos < sl.cap) {
-    sl.src[(size_t)parity * sl.cap + pos] = node;
+    sl.src[(size_t)parity * sl.cap + pos] = lag ? (int)((unsigned int)node | kSpikeLagBit) : node;
     sl.height[(size_t)parity * sl.cap + pos] = height;
   } else {
     atomicAdd(sl.overflow, 1);
@@ -216,6 +229,39 @@ __device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
       sb.n_rec[rel] = k + 1;
     }
   }
+}
+
+// Input of channel idx (= port * n + local neuron) for the step with time
+// index t_idx ending at t1: the ring slot (cleared for reuse) plus the
+// Poisson draws of the generators connected to it, src/get_spike.cu:142-166
+// and src/poiss_gen.cu:55-86.  Poisson edges are sorted by channel, so each
+// lane sums its own edges (no atomics).  Contraction is off: the double
+// expressions round exactly as the reference's.
+__device__ inline double channel_input(const FoldArgs& f, long long idx, int slot, double t1,
+                                       long long t_idx) {
+#pragma clang fp contract(off)
+  double* rp = f.ring + (size_t)slot * f.n_ch + (size_t)(f.ch_base + idx);
+  double acc = *rp;
+  *rp = 0.0;
+  if (f.pe_off) {
+    const int b = f.pe_off[idx], e = f.pe_off[idx + 1];
+    for (int k = b; k < e; ++k) {
+      const int g = f.pe_gen[k];
+      const double t_rel = t1 - f.gen_origin[g] - f.pe_delay[k];
+      if (t_rel >= f.gen_start[g] && t_rel <= f.gen_stop[g]) {
+        const int cnt = poisson_draw(f.seed, f.pe_stream[k], t_idx, f.gen_chunks[g], f.gen_exp[g]);
+        if (cnt > 0) acc += (double)(f.pe_w[k] * (float)cnt);
+      }
+    }
+  }
+  return acc;
+}
+
+// input = (float)((double)input + acc * port_weight), skipped for acc == 0
+// (input + 0*w == input exactly)
+__device__ inline float fold_add(float input, double acc, float w) {
+#pragma clang fp contract(off)
+  return (float)((double)input + acc * (double)w);
 }
 
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
@@ -374,7 +420,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // uniform over the population, so parameters come from the kernel-argument
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
 template <class M, bool UNI, int MODE>
-__global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
+__global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
   using D = typename std::conditional<UNI, DivRecip, DivIEEE>::type;
   static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
@@ -385,26 +431,34 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
   const double x1 = a.ctx->t1;
   const long long t_idx = a.ctx->t_idx;
   const int parity = a.ctx->parity;
-  // this step's hot list first, then every neuron (those on the hot list
-  // are skipped there)
-  const int hot_cap = n_waves * a.hot_k;
-  int n_hot = 0;
+  // end of the previous step (the end_step_kernel formula one step back): a
+  // lagged neuron integrates up to it before starting this step
+  const double x1_prev = a.ctx->t0 + (double)a.dt * (double)a.ctx->it;
+  // the wave's own hot list first (neurons of its chunk that ended the last
+  // step about to spike, or split; ~1 per wave, so each takes a lane of its
+  // own and the long attempt chains start together in the first iteration),
+  // then its chunk, skipping the listed neurons; indices below are relative
+  // to the wave's range
   const unsigned char* flag_in = nullptr;
   unsigned char* flag_out = nullptr;
   const int* dense = nullptr;
-  int* reg_out = nullptr;
+  int n_hot = 0, q_out = 0;
+  __shared__ int s_hot[kRK5Block / kWave][kHotLds];
+  int* my_hot = s_hot[threadIdx.x / kWave];
   if (a.hot_flag) {
-    n_hot = a.hot_n[parity];
-    if (n_hot > hot_cap) n_hot = hot_cap;
+    const int q_in = (int)(t_idx % 3);
+    q_out = q_in == 2 ? 0 : q_in + 1;
+    const int q_clr = q_out == 2 ? 0 : q_out + 1;   // read by the previous launch
+    n_hot = a.hot_count[q_in];
+    if (n_hot > n) n_hot = n;
+    dense = a.hot_list + (size_t)q_in * n;
     flag_in = a.hot_flag + (size_t)parity * n;
     flag_out = a.hot_flag + (size_t)(parity ^ 1) * n;
-    dense = a.hot_dense + (size_t)parity * hot_cap;
-    reg_out = a.hot_reg + ((size_t)(parity ^ 1) * n_waves + wave) * a.hot_k;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.hot_count[q_clr] = 0;
   }
   // wave w takes hot entries [w*hp, w*hp + hp) (one per lane, so the long
-  // attempt chains of neurons about to spike start together in the first
-  // iteration and never queue behind each other in a lane), then its equal
-  // share of all neurons; indices below are relative to the wave's range
+  // attempt chains start together in the first iteration, packed into few
+  // waves), then its equal share of all neurons
   const int hp = n_hot <= n_waves * kWave
                      ? kWave
                      : (int)((((long long)n_hot + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1));
@@ -424,6 +478,8 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
   const float r_eps = recip_rn(ck::eps);
   float y[M::NV], p[M::NPAR], r[M::NPAR], k1[M::NV], ys[M::NV];
   double x = 0.0;
+  double cx1 = 0.0;     // end of the step the lane's neuron is in (x1, or x1_prev if lagged)
+  bool lagged = false;
   float h = 0.0f, ref_in = 0.0f;
   int i = -1;
   int att_i = 0;   // attempts of the current neuron in this step
@@ -437,6 +493,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
     __syncthreads();
   }
   unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
+  unsigned int n_split = 0;
 #pragma unroll
   for (int j = 0; j < M::NPAR; ++j) {
     p[j] = a.tab.val[j];
@@ -476,6 +533,8 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           x = a.x[i];
           h = a.h[i];
           if constexpr (MODE != 0) el = a.err_last[i];
+          lagged = a.lag ? a.lag[i] != 0 : false;
+          cx1 = lagged ? x1_prev : x1;
           need_k1 = true;
           att_i = 0;
         }
@@ -486,9 +545,11 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
     // the hot list): the masked iteration below does nothing, refill again
     if (!__any(i >= 0) && next >= end) break;
     if constexpr (MODE == 2) {
-      // learned proposal for every lane starting a substep (whole wave on
-      // the matrix cores; lanes without a neuron feed zeros)
-      const bool start = (i >= 0) && need_k1;
+      // learned proposal for every lane starting a substep whose controller
+      // proposal ends inside the step (one cut by the step end, the quiet
+      // neuron's usual single substep, is left alone); whole wave on the
+      // matrix cores, lanes without a neuron feed zeros
+      const bool start = (i >= 0) && need_k1 && h < (float)(cx1 - x);
       if (__any(start)) {
         float raw[kPI];
         if (i >= 0) {
@@ -516,7 +577,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
         need_k1 = false;
       }
       // ---- one attempt (src/rk5.h:88-131) ----
-      const float h_max = (float)(x1 - x);
+      const float h_max = (float)(cx1 - x);
       clip = false;
       if (h > h_max) {
         h = h_max;
@@ -557,7 +618,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
       err = div_rn(err, ck::eps, r_eps);
       ++n_att;
       if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)
-        x = x1;
+        x = cx1;
         err = 0.0f;
       }
       if (!(err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5))) {
@@ -586,7 +647,7 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) y[j] = yn[j];
         need_k1 = true;
-        const bool end_step = (x >= x1 - (double)h_min);
+        const bool end_step = (x >= cx1 - (double)h_min);
         if (y[M::iV] < -1.0e3 || y[M::iw] < -1.0e6 || y[M::iw] > 1.0e6) {
           y[M::iV] = p[M::i_reset];
           y[M::iw] = 0.0f;
@@ -594,20 +655,43 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
           y[M::iV] = p[M::i_reset];
           if (end_step) p[M::i_ref] -= 1.0f;
         } else if (y[M::iV] >= p[M::i_thr]) {
-          emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, t_idx, x1);
+          emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, lagged ? t_idx - 1 : t_idx, cx1,
+                     lagged);
           y[M::iV] = p[M::i_reset];
           y[M::iw] += p[M::i_jump];
           const float rr = (float)(int)roundf(p[M::i_tref] / a.dt);
           p[M::i_ref] = rr < 0.0f ? 0.0f : rr;
         }
-        if (end_step) {
+        if (end_step && lagged) {
+          // the previous step is complete: add its input (stashed by that
+          // step's fold launch, same arithmetic), then go on with this step
+#pragma unroll
+          for (int k = 0; k < M::NPORT; ++k) {
+            const double acc = a.pend[(size_t)k * n + i];
+            if (acc != 0.0) {
+              const int wj = M::in_w(k);
+              const float w = wj >= 0 ? p[wj < 0 ? 0 : wj] : 1.0f;
+              y[M::in_y(k)] = fold_add(y[M::in_y(k)], acc, w);
+            }
+          }
+          a.lag[i] = 0;
+          lagged = false;
+          cx1 = x1;
+          att_i = 0;
+        } else if (end_step || (a.defer_cap > 0 && !lagged && att_i >= a.defer_cap)) {
+          // end of the step, or split: the neuron is stored mid-step with
+          // lag = 1 (and listed hot, so the next launch resumes it first)
+          if (!end_step) {
+            a.lag[i] = 1;
+            ++n_split;
+          }
 #pragma unroll
           for (int j = 0; j < M::NV; ++j) a.var[(size_t)j * n + i] = y[j];
           a.x[i] = x;
           a.h[i] = h;
           if constexpr (MODE != 0) a.err_last[i] = el;
           if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
-          pend = 1 | ((p[M::i_ref] <= 0.0f && y[M::iV] >= M::hot_v(p)) ? 2 : 0);
+          pend = 1 | ((!end_step || (p[M::i_ref] <= 0.0f && y[M::iV] >= M::hot_v(p))) ? 2 : 0);
           pend_i = i;
           i = -1;
         }
@@ -620,8 +704,8 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
         bool listed = false;
         if (pend == 3) {
           const int pos = hcnt + __popcll(mh & ((1ull << lane) - 1ull));
-          if (pos < a.hot_k) {
-            reg_out[pos] = pend_i;
+          if (pos < kHotLds) {
+            my_hot[pos] = pend_i;
             listed = true;
           }
         }
@@ -631,77 +715,42 @@ __global__ void __launch_bounds__(256) rk5_persistent_kernel(RK5Args a) {
       hcnt += __popcll(mh);
     }
   }
-  if (a.hot_flag && lane == 0) {
-    const int listed = hcnt < a.hot_k ? hcnt : a.hot_k;
-    a.hot_cnt[(size_t)(parity ^ 1) * n_waves + wave] = listed;
-    if (a.ctr) a.ctr[(size_t)wave * kCtrCols + 4] += (unsigned long long)listed;
+  if (a.hot_flag) {
+    // append the wave's list to the next step's (one atomic per wave; the
+    // order of the entries only affects scheduling)
+    const int listed = hcnt < kHotLds ? hcnt : kHotLds;
+    if (listed > 0) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(a.hot_count + q_out, listed);
+      base = __shfl(base, 0, kWave);
+      int* out = a.hot_list + (size_t)q_out * n + base;
+      for (int e = lane; e < listed; e += kWave) out[e] = my_hot[e];
+      if (lane == 0 && a.ctr) a.ctr[(size_t)wave * kCtrCols + 4] += (unsigned long long)listed;
+    }
+  }
+  if (a.lag && a.ctr) {
+    const unsigned long long ns = wave_sum(n_split);
+    if (lane == 0 && ns) a.ctr[(size_t)wave * kCtrCols + 5] += ns;
   }
   add_wave_counters(a.ctr, wave, lane, n_att, n_rej, n_sub);
 }
 
-// Scan of the per-wave counts (one block): offsets of each wave's list in
-// the packed list of the next step (buffers of parity ^ 1) and its length.
-__global__ void __launch_bounds__(1024) hot_scan_kernel(HotGatherArgs a) {
-  __shared__ int part[1024];
-  const int q = a.ctx->parity ^ 1;
-  const int* cnt = a.hot_cnt + (size_t)q * a.waves;
-  const int per = (a.waves + blockDim.x - 1) / blockDim.x;
-  const int w0 = threadIdx.x * per;
-  int sum = 0;
-  for (int k = 0; k < per; ++k)
-    if (w0 + k < a.waves) sum += cnt[w0 + k];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  // inclusive scan (Hillis-Steele) of the per-thread sums
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int o = part[threadIdx.x] - sum;
-  for (int k = 0; k < per; ++k) {
-    const int w = w0 + k;
-    if (w >= a.waves) break;
-    a.hot_off[w] = o;
-    o += cnt[w];
-  }
-  if (threadIdx.x == blockDim.x - 1) a.hot_n[q] = part[threadIdx.x];
-}
-
-// Copy: one wave per per-wave list, 64 entries per iteration.
-__global__ void __launch_bounds__(256) hot_copy_kernel(HotGatherArgs a) {
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (w >= a.waves) return;
-  const int q = a.ctx->parity ^ 1;
-  const int c = a.hot_cnt[(size_t)q * a.waves + w];
-  const int o = a.hot_off[w];
-  const int* reg = a.hot_reg + ((size_t)q * a.waves + w) * a.hot_k;
-  int* dense = a.hot_dense + (size_t)q * a.waves * a.hot_k;
-  for (int e = lane; e < c; e += kWave) dense[o + e] = reg[e];
-}
-
-void launch_hot_gather(const HotGatherArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(hot_scan_kernel, dim3(1), dim3(1024), 0, s, a);
-  hipLaunchKernelGGL(hot_copy_kernel, dim3((a.waves + 3) / 4), dim3(256), 0, s, a);
-}
-
 static int persistent_grid(int n, int max_blocks) {
   const int waves_needed = (n + kWave - 1) / kWave;
-  int grid = (waves_needed + 3) / 4;
+  int grid = (waves_needed + kRK5Block / kWave - 1) / (kRK5Block / kWave);
   if (max_blocks > 0 && grid > max_blocks) grid = max_blocks;
   return grid;
 }
 
 long long rk5_waves(int n, int persistent, int max_blocks) {
   if (n <= 0) return 0;
-  return persistent ? 4LL * persistent_grid(n, max_blocks) : 4LL * ((n + 255) / 256);
+  return persistent ? (long long)(kRK5Block / kWave) * persistent_grid(n, max_blocks)
+                    : 4LL * ((n + 255) / 256);
 }
 
 template <class M>
 static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
-  const int block = 256;
+  const int block = kRK5Block;
   if (a.persistent) {
     // persistent grid: enough waves to fill the GPU, at most one lane per neuron
     const int grid = persistent_grid(a.n, a.max_blocks);
@@ -761,8 +810,50 @@ int rk5_supported(int model, int n_port) {
   }
 }
 
+template <class M>
+static bool fold_layout_is(int in_var, int in_stride, int w_param, int w_stride) {
+  for (int k = 0; k < M::NPORT; ++k) {
+    if (M::in_y(k) != in_var + k * in_stride) return false;
+    if (M::in_w(k) != (w_param >= 0 ? w_param + k * w_stride : -1)) return false;
+  }
+  return true;
+}
+
+template <template <int> class M>
+static bool fold_layout_ports(int n_port, int iv, int is, int wp, int ws) {
+  switch (n_port) {
+    case 1: return fold_layout_is<M<1>>(iv, is, wp, ws);
+    case 2: return fold_layout_is<M<2>>(iv, is, wp, ws);
+    case 3: return fold_layout_is<M<3>>(iv, is, wp, ws);
+    case 4: return fold_layout_is<M<4>>(iv, is, wp, ws);
+    case 5: return fold_layout_is<M<5>>(iv, is, wp, ws);
+    case 6: return fold_layout_is<M<6>>(iv, is, wp, ws);
+    case 7: return fold_layout_is<M<7>>(iv, is, wp, ws);
+    case 8: return fold_layout_is<M<8>>(iv, is, wp, ws);
+    default: return false;
+  }
+}
+
+bool rk5_fold_layout_matches(int model, int n_port, int in_var, int in_port_stride,
+                             int w_param, int w_port_stride) {
+  switch (model) {
+    case MODEL_AEIF_COND_BETA_MS:
+      return fold_layout_ports<AeifCondBeta>(n_port, in_var, in_port_stride, w_param, w_port_stride);
+    case MODEL_AEIF_COND_ALPHA_MS:
+      return fold_layout_ports<AeifCondAlpha>(n_port, in_var, in_port_stride, w_param, w_port_stride);
+    case MODEL_IZH_COND_BETA:
+      return fold_layout_ports<IzhCondBeta>(n_port, in_var, in_port_stride, w_param, w_port_stride);
+    case MODEL_AEIF_PSC_EXP:
+      return n_port == 2 && fold_layout_is<AeifPscExp>(in_var, in_port_stride, w_param, w_port_stride);
+    default:
+      return false;
+  }
+}
+
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s) {
   if (a.n <= 0) return;
+  if (a.defer_cap > 0 && (!a.persistent || !a.lag || !a.pend))
+    throw std::runtime_error("RK5 split steps need the persistent kernel and the lag/pend buffers");
   switch (model) {
     case MODEL_AEIF_COND_BETA_MS: dispatch_ports<AeifCondBeta>(n_port, a, s); break;
     case MODEL_AEIF_COND_ALPHA_MS: dispatch_ports<AeifCondAlpha>(n_port, a, s); break;
@@ -876,7 +967,11 @@ __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
   const int stride = blockDim.x * kDeliverUnroll;
   unsigned long long n_ev = 0;
   for (int s = blockIdx.x; s < n_spk; s += gridDim.x) {
-    const int src = srcs[s];
+    // a lagged spike (emitted in the previous step) arrives one slot earlier
+    const unsigned int raw = (unsigned int)srcs[s];
+    const int src = (int)(raw & ~kSpikeLagBit);
+    int sbase = base - (int)(raw >> 31);
+    if (sbase < 0) sbase += a.n_slot;
     const float hgt = hts[s];
     const long long b = a.row_ptr[src], e = a.row_ptr[src + 1];
     n_ev += (unsigned long long)(e - b);
@@ -898,7 +993,7 @@ __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
 #pragma unroll
       for (int u = 0; u < kDeliverUnroll; ++u) {
         if (d[u] < 0) continue;
-        int slot = base + d[u];
+        int slot = sbase + d[u];
         if (slot >= a.n_slot) slot -= a.n_slot;
         unsafeAtomicAdd(&a.ring[(size_t)slot * a.n_ch + ch[u]], (double)(hgt * w[u]));
       }
@@ -929,27 +1024,15 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 // channel, so each lane sums its own edges (no atomics).  Contraction is off
 // so the double expression rounds exactly as the reference's.
 __global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
-#pragma clang fp contract(off)
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long total = (long long)a.n * a.n_port;
   if (idx >= total) return;
   const int port = (int)(idx / a.n);
   const int i = (int)(idx - (long long)port * a.n);
-  const size_t r = (size_t)a.ctx->slot * a.n_ch + (size_t)(a.ch_base + idx);
-  double acc = a.ring[r];
-  a.ring[r] = 0.0;
-  if (a.pe_off) {
-    const int b = a.pe_off[idx], e = a.pe_off[idx + 1];
-    const double t = a.ctx->t1;
-    const long long t_idx = a.ctx->t_idx;
-    for (int k = b; k < e; ++k) {
-      const int g = a.pe_gen[k];
-      const double t_rel = t - a.gen_origin[g] - a.pe_delay[k];
-      if (t_rel >= a.gen_start[g] && t_rel <= a.gen_stop[g]) {
-        const int cnt = poisson_draw(a.seed, a.pe_stream[k], t_idx, a.gen_chunks[g], a.gen_exp[g]);
-        if (cnt > 0) acc += (double)(a.pe_w[k] * (float)cnt);
-      }
-    }
+  const double acc = channel_input(a, idx, a.ctx->slot, a.ctx->t1, a.ctx->t_idx);
+  if (a.lag && a.lag[i]) {   // still inside this step: its RK5 launch adds the input later
+    a.pend[idx] = acc;
+    return;
   }
   if (acc == 0.0) return;   // input + 0*w == input exactly
   float w = 1.0f;
@@ -958,7 +1041,7 @@ __global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
     w = a.tab.is_per_neuron(j) ? a.param[(size_t)j * a.n + i] : a.tab.val[j];
   }
   float* in = a.var + (size_t)(a.in_var + port * a.in_port_stride) * a.n + i;
-  *in = (float)((double)(*in) + acc * (double)w);
+  *in = fold_add(*in, acc, w);
 }
 
 void launch_fold(const FoldArgs& a, hipStream_t s) {

@@ -60,3 +60,58 @@ def c1_network(seed=1, n=1000, k=100, t_rate=20000.0, poiss_w=None, iaf=False):
     si, ti, wi, di = random_edges(rng, n - ne, n, ki, ne, 0, 2.0, 6.0, 1, 20)
     return dict(n=n, ne=ne, exc=(se, te, we, de), inh=(si, ti, wi, di), rate=t_rate,
                 poiss_w=poiss_w)
+
+
+# Mixed-model network in the shape of the six-population batching workload,
+# scaled down: populations of every RK5 model plus iaf_psc_exp, fixed
+# indegree across all populations (80% of sources excitatory -> port 0, the
+# rest -> port 1), one Poisson generator per population.  Weights are per
+# target model (conductance models in nS with the inhibitory port at
+# E_rev = -85 mV, current models in pA).
+MIXED_POPS = [
+    # model, n, ports, status, exc weight range, inh weight range, Poisson rate (Hz), Poisson weight
+    ("izhikevich_cond_beta", 30, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [2.0, 2.0]},
+     (0.002, 0.006), (0.005, 0.015), 8000.0, 0.006),
+    ("izhikevich_cond_beta", 60, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [2.0, 2.0]},
+     (0.002, 0.006), (0.005, 0.015), 8000.0, 0.006),
+    ("aeif_psc_exp", 100, 2, {},
+     (20.0, 60.0), (60.0, 180.0), 20000.0, 170.0),
+    ("aeif_cond_alpha_multisynapse", 200, 4,
+     {"E_rev": [0.0, -85.0, 0.0, -85.0], "tau_syn": [1.0, 1.0, 5.0, 5.0]},
+     (0.3, 1.0), (1.0, 3.0), 20000.0, 0.25),
+    ("aeif_cond_beta_multisynapse", 300, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [1.0, 1.0]},
+     (0.5, 1.5), (2.0, 6.0), 20000.0, 0.3),
+    ("iaf_psc_exp", 150, 2, {},
+     (60.0, 120.0), (-240.0, -120.0), 8000.0, 22.0),
+]
+
+
+def mixed_network(seed=5, k=50, d_hi=20):
+    """Description of the mixed network: population list with first ids,
+    and per target population the (src, tgt, port, w, d) edge arrays."""
+    rng = np.random.default_rng(seed)
+    pops = []
+    i0 = 0
+    for (model, n, npt, st, we, wi, rate, pw) in MIXED_POPS:
+        pops.append(dict(model=model, n=n, n_port=npt, status=st, i0=i0, we=we, wi=wi,
+                         rate=rate, pw=pw))
+        i0 += n
+    n_tot = i0
+    n_exc = int(0.8 * n_tot)     # global ids below n_exc are excitatory sources
+    edges = []
+    for p in pops:
+        n = p["n"]
+        src = np.empty(n * k, np.int32)
+        for t in range(n):
+            src[t * k:(t + 1) * k] = rng.choice(n_tot, size=k, replace=False)
+        tgt = np.repeat(np.arange(n, dtype=np.int32), k) + p["i0"]
+        exc = src < n_exc
+        port = np.where(exc, 0, 1).astype(np.uint8)
+        w = np.where(exc, rng.uniform(*p["we"], size=n * k),
+                     rng.uniform(*p["wi"], size=n * k)).astype(np.float32)
+        d = (rng.integers(1, d_hi + 1, size=n * k) * 0.1).astype(np.float32)
+        edges.append((src, tgt, port, w, d))
+    return dict(pops=pops, n=n_tot, edges=edges)

@@ -292,3 +292,116 @@ def test_persistent_rk5_matches_lane_per_neuron_kernel(ngpu):
     np.testing.assert_array_equal(n1, n0)
     np.testing.assert_array_equal(v1, v0)
     assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
+
+
+def _mixed_gpu(ngpu, net, sim_ms):
+    pops = []
+    for p in net["pops"]:
+        nrn = ngpu.Create(p["model"], p["n"], p["n_port"])
+        assert nrn[0] == p["i0"]
+        if p["status"]:
+            ngpu.SetStatus(nrn, p["status"])
+        pops.append(nrn)
+    for p, nrn in zip(net["pops"], pops):
+        pg = ngpu.Create("poisson_generator")
+        ngpu.SetStatus(pg, "rate", p["rate"])
+        ngpu.Connect(pg, nrn, {"rule": "all_to_all"},
+                     {"weight": p["pw"], "delay": 0.1, "receptor": 0})
+    for (s, t, port, w, d) in net["edges"]:
+        for q in (0, 1):
+            m = port == q
+            netbuild.connect_edges_fast(ngpu, s[m], t[m], q, w[m], d[m])
+    for nrn in pops:
+        ngpu.ActivateRecSpikeTimes(nrn, 5000)
+    ngpu.Simulate(sim_ms)
+    t_idx, node = [], []
+    for nrn in pops:
+        for i, ts in enumerate(ngpu.GetRecSpikeTimes(nrn)):
+            for tt in ts:
+                t_idx.append(int(round(tt / 0.1)))
+                node.append(i + nrn[0])
+    t_idx = np.array(t_idx, np.int64)
+    node = np.array(node, np.int32)
+    o = np.lexsort((node, t_idx))
+    return t_idx[o], node[o]
+
+
+def _mixed_oracle(net, sim_ms):
+    from oracle.oracle import Oracle
+    o = Oracle(prec=0)
+    for p in net["pops"]:
+        n0 = o.create(p["model"], p["n"], p["n_port"])
+        assert n0 == p["i0"]
+        for k, v in p["status"].items():
+            for q, vv in enumerate(v):
+                o.set(n0, p["n"], k, [vv] * p["n"], port=q)
+    for p in net["pops"]:
+        pg = o.create("poisson_generator", 1, 0)
+        o.set(pg, 1, "rate", [p["rate"]])
+        o.connect(np.full(p["n"], pg, np.int32), np.arange(p["n"], dtype=np.int32) + p["i0"], 0,
+                  p["pw"], 0.1)
+    for (s, t, port, w, d) in net["edges"]:
+        for q in (0, 1):
+            m = port == q
+            o.connect(s[m], t[m], q, w[m], d[m])
+    o.simulate(sim_ms)
+    return o.events()
+
+
+@pytest.mark.parametrize("split", ["default", "2"])
+def test_mixed_model_network(ngpu, split):
+    # six populations of five models (izhikevich_cond_beta x2, aeif_psc_exp,
+    # aeif_cond_alpha_multisynapse with 4 ports, aeif_cond_beta_multisynapse,
+    # iaf_psc_exp), fixed indegree 50 across them, delays 0.1-2 ms: the
+    # raster over 100 ms equals the oracle's spike for spike.  "2" splits
+    # every neuron-step longer than 2 attempts over two launches.
+    net = netbuild.mixed_network()
+    if split != "default":
+        os.environ["NGPU_RK5_DEFER"] = split
+    try:
+        ngpu.ResetKernel()
+        ngpu.SetVerbosityLevel(0)
+        tg, ng = _mixed_gpu(ngpu, net, 100.0)
+        st = ngpu.GetStepStats()
+    finally:
+        os.environ.pop("NGPU_RK5_DEFER", None)
+    to, no = _mixed_oracle(net, 100.0)
+    print("mixed spikes gpu", len(tg), "oracle", len(to), "split steps", st["split_steps"])
+    for p in net["pops"]:
+        m = (no >= p["i0"]) & (no < p["i0"] + p["n"])
+        assert m.sum() > 20, p["model"]
+    if split != "default":
+        assert st["split_steps"] > 1000
+    np.testing.assert_array_equal(tg, to)
+    np.testing.assert_array_equal(ng, no)
+
+
+def test_split_steps_match_unsplit(ngpu):
+    # a neuron-step needing more attempts than the cap is stored mid-step and
+    # finished first in the next launch (its fold input applied then, its
+    # spike delivered with its own emission time): rasters, final state and
+    # attempt counts equal those of the unsplit run
+    net = netbuild.c1_network(seed=7)
+    res = []
+    for cap in ("3", "0"):
+        os.environ["NGPU_RK5_DEFER"] = cap
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            t, n = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", 60.0,
+                                poiss_w=netbuild.C1_POISSON_W, status=netbuild.C1_RECEPTORS)
+            nodes = ngpu.NodeSeq(0, net["n"])
+            v = np.array(ngpu.GetStatus(nodes, "V_m"))
+            g1 = np.array(ngpu.GetStatus(nodes, "g1"))
+            res.append((t, n, v, g1, ngpu.GetStepStats()))
+        finally:
+            os.environ.pop("NGPU_RK5_DEFER", None)
+    (t1, n1, v1, g11, s1), (t0, n0, v0, g10, s0) = res
+    print("split steps", s1["split_steps"], "spikes", len(t1))
+    assert s1["split_steps"] > 1000 and s0["split_steps"] == 0
+    assert len(t1) > 50
+    np.testing.assert_array_equal(t1, t0)
+    np.testing.assert_array_equal(n1, n0)
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(g11, g10)
+    assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
