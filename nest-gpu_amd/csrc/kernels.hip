@@ -51,7 +51,10 @@ using ck::kMaxAttempts;
 
 // ---------------------------------------------------------------------------
 // Neuron models (right-hand sides and post-step updates).
-// Parameter and variable indices follow the reference's name tables.
+// Parameter and variable indices follow the reference's name tables.  Each
+// product-plus-sum of the reference's expressions (a*b + c) is one fma, as
+// the reference's compiler contracts them; the oracle evaluates the same
+// sequence.
 //
 // Every division of a right-hand side is by a parameter (M::divisor(j)).
 // D::div(x, p, r, j) returns x / p[j]: DivIEEE divides; DivRecip, for
@@ -91,12 +94,12 @@ struct AeifCondBeta {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
-    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
-    const float V_spike = p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+    for (int i = 0; i < NP; ++i) I_syn = fmaf(y[2 + 2 * i], p[14 + 4 * i] - V, I_syn);
+    const float V_spike = p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
     dy[0] = (p[ref] > 0)
                 ? 0.0f
-                : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e], p, r, C_m);
-    dy[1] = D::div(p[a] * (V - p[E_L]) - y[1], p, r, tau_w);
+                : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, r, C_m);
+    dy[1] = D::div(fmaf(p[a], V - p[E_L], -y[1]), p, r, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 4 * i + 1);
@@ -125,12 +128,12 @@ struct AeifCondAlpha {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
-    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
-    const float V_spike = p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+    for (int i = 0; i < NP; ++i) I_syn = fmaf(y[2 + 2 * i], p[14 + 3 * i] - V, I_syn);
+    const float V_spike = p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
     dy[0] = (p[ref] > 0)
                 ? 0.0f
-                : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] + p[I_e], p, r, C_m);
-    dy[1] = D::div(p[a] * (V - p[E_L]) - y[1], p, r, tau_w);
+                : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, r, C_m);
+    dy[1] = D::div(fmaf(p[a], V - p[E_L], -y[1]), p, r, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 3 * i + 1);
@@ -161,11 +164,11 @@ struct AeifPscExp {
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[V_m], p[V_peak]);
     const float V_spike = p[Delta_T] == 0.0f
                               ? 0.0f
-                              : p[Delta_T] * exp_ieee(D::div(V - p[V_th], p, r, Delta_T));
+                              : p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
     dy[V_m] = (p[ref] > 0)
                   ? 0.0f
-                  : D::div(-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot - y[w] + p[I_e], p, r, C_m);
-    dy[w] = D::div(p[a] * (V - p[E_L]) - y[w], p, r, tau_w);
+                  : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) - y[w] + p[I_e], p, r, C_m);
+    dy[w] = D::div(fmaf(p[a], V - p[E_L], -y[w]), p, r, tau_w);
     dy[I_ex] = D::div(-y[I_ex], p, r, tau_ex);
     dy[I_in] = D::div(-y[I_in], p, r, tau_in);
   }
@@ -188,11 +191,11 @@ struct IzhCondBeta {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[c] : y[0];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) I_syn += y[2 + 2 * i] * (p[9 + 4 * i] - V);
+    for (int i = 0; i < NP; ++i) I_syn = fmaf(y[2 + 2 * i], p[9 + 4 * i] - V, I_syn);
     // evaluated in double, as the reference's double literals make it
     dy[0] = (p[ref] > 0) ? 0.0f
                          : (float)(0.04 * V * V + 5.0 * V + 140.0 - y[1] + I_syn + p[I_e]);
-    dy[1] = p[a] * (p[b] * V - y[1]);
+    dy[1] = p[a] * fmaf(p[b], V, -y[1]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 9 + 4 * i + 1);
@@ -290,6 +293,64 @@ __device__ inline void add_wave_counters(unsigned long long* ctr, int slot, int 
 }
 
 // ---------------------------------------------------------------------------
+// One Cash-Karp attempt of step h from (y, k1), src/rk5.h:88-131: the five
+// further stages, the 5th-order solution yn and the error ratio
+// max_j |h sum_i e_i k_i[j]| / ys[j] (before the division by eps).  Each
+// stage's sum_i a_i k_i is accumulated with fma and added to y with one more
+// (y + h*(...) as the reference's compiler contracts it).
+template <class M, class F>
+__device__ inline float rk5_try(const float* y, const float* k1, const float* ys, float h,
+                                float* yn, F&& deriv) {
+  float k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV];
+  const float h21 = h * ck::a21;
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) yn[j] = fmaf(h21, k1[j], y[j]);
+  deriv(yn, k2);
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) yn[j] = fmaf(h, fmaf(ck::a32, k2[j], ck::a31 * k1[j]), y[j]);
+  deriv(yn, k3);
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) {
+    float t = fmaf(ck::a42, k2[j], ck::a41 * k1[j]);
+    t = fmaf(ck::a43, k3[j], t);
+    yn[j] = fmaf(h, t, y[j]);
+  }
+  deriv(yn, k4);
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) {
+    float t = fmaf(ck::a52, k2[j], ck::a51 * k1[j]);
+    t = fmaf(ck::a53, k3[j], t);
+    t = fmaf(ck::a54, k4[j], t);
+    yn[j] = fmaf(h, t, y[j]);
+  }
+  deriv(yn, k5);
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) {
+    float t = fmaf(ck::a62, k2[j], ck::a61 * k1[j]);
+    t = fmaf(ck::a63, k3[j], t);
+    t = fmaf(ck::a64, k4[j], t);
+    t = fmaf(ck::a65, k5[j], t);
+    yn[j] = fmaf(h, t, y[j]);
+  }
+  deriv(yn, k6);
+  float err = 0.0f;
+#pragma unroll
+  for (int j = 0; j < M::NV; ++j) {
+    float t = fmaf(ck::b3, k3[j], ck::b1 * k1[j]);
+    t = fmaf(ck::b4, k4[j], t);
+    t = fmaf(ck::b6, k6[j], t);
+    yn[j] = fmaf(h, t, y[j]);
+    float e = fmaf(ck::e3, k3[j], ck::e1 * k1[j]);
+    e = fmaf(ck::e4, k4[j], e);
+    e = fmaf(ck::e5, k5[j], e);
+    e = fmaf(ck::e6, k6[j], e);
+    const float v = (h * e) / ys[j];
+    err = fmaxf(err, fabsf(v));
+  }
+  return err;
+}
+
+// ---------------------------------------------------------------------------
 // Adaptive RK5 update of one population, src/rk5.h:69-207: one lane per
 // neuron, IEEE divisions (the reference's arithmetic operation by operation).
 template <class M>
@@ -318,7 +379,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
     while (!end_step) {
       const float h_max = (float)(x1 - x);
       // ---- RK5Step ----
-      float k1[M::NV], k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV];
+      float k1[M::NV];
       float yn[M::NV], ys[M::NV];
       M::template deriv<DivIEEE>(y, k1, p, nullptr);
 #pragma unroll
@@ -327,36 +388,9 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
       for (;;) {
         if (h > h_max) h = h_max;
         if (h < h_min) h = h_min;
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
-        M::template deriv<DivIEEE>(yn, k2, p, nullptr);
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
-        M::template deriv<DivIEEE>(yn, k3, p, nullptr);
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j)
-          yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
-        M::template deriv<DivIEEE>(yn, k4, p, nullptr);
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j)
-          yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
-        M::template deriv<DivIEEE>(yn, k5, p, nullptr);
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j)
-          yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
-                              ck::a64 * k4[j] + ck::a65 * k5[j]);
-        M::template deriv<DivIEEE>(yn, k6, p, nullptr);
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j)
-          yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
-        err = 0.0f;
-#pragma unroll
-        for (int j = 0; j < M::NV; ++j) {
-          float v = h * (ck::e1 * k1[j] + ck::e3 * k3[j] + ck::e4 * k4[j] + ck::e5 * k5[j] +
-                         ck::e6 * k6[j]);
-          v /= ys[j];
-          err = fmaxf(err, fabsf(v));
-        }
+        err = rk5_try<M>(y, k1, ys, h, yn, [&](const float* yy, float* dd) {
+          M::template deriv<DivIEEE>(yy, dd, p, nullptr);
+        });
         err /= ck::eps;
         ++n_att;
         if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)
@@ -584,37 +618,10 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
         clip = true;
       }
       if (h < h_min) h = h_min;
-      float k2[M::NV], k3[M::NV], k4[M::NV], k5[M::NV], k6[M::NV], yn[M::NV];
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * ck::a21 * k1[j];
-      M::template deriv<D>(yn, k2, p, r);
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j) yn[j] = y[j] + h * (ck::a31 * k1[j] + ck::a32 * k2[j]);
-      M::template deriv<D>(yn, k3, p, r);
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j)
-        yn[j] = y[j] + h * (ck::a41 * k1[j] + ck::a42 * k2[j] + ck::a43 * k3[j]);
-      M::template deriv<D>(yn, k4, p, r);
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j)
-        yn[j] = y[j] + h * (ck::a51 * k1[j] + ck::a52 * k2[j] + ck::a53 * k3[j] + ck::a54 * k4[j]);
-      M::template deriv<D>(yn, k5, p, r);
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j)
-        yn[j] = y[j] + h * (ck::a61 * k1[j] + ck::a62 * k2[j] + ck::a63 * k3[j] +
-                            ck::a64 * k4[j] + ck::a65 * k5[j]);
-      M::template deriv<D>(yn, k6, p, r);
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j)
-        yn[j] = y[j] + h * (ck::b1 * k1[j] + ck::b3 * k3[j] + ck::b4 * k4[j] + ck::b6 * k6[j]);
-      float err = 0.0f;
-#pragma unroll
-      for (int j = 0; j < M::NV; ++j) {
-        float v = h * (ck::e1 * k1[j] + ck::e3 * k3[j] + ck::e4 * k4[j] + ck::e5 * k5[j] +
-                       ck::e6 * k6[j]);
-        v /= ys[j];
-        err = fmaxf(err, fabsf(v));
-      }
+      float yn[M::NV];
+      float err = rk5_try<M>(y, k1, ys, h, yn, [&](const float* yy, float* dd) {
+        M::template deriv<D>(yy, dd, p, r);
+      });
       err = div_rn(err, ck::eps, r_eps);
       ++n_att;
       if (++att_i > kMaxAttempts) {   // non-finite state: end the step (never hit by finite runs)

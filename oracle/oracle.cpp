@@ -153,7 +153,11 @@ inline double exp_model(double x);
 
 // ---------------------------------------------------------------------------
 // Model right-hand sides.  `y` and `p` follow the reference index layout
-// (scalar block, then `port_var`/`port_param` interleaved per port).
+// (scalar block, then `port_var`/`port_param` interleaved per port).  A
+// product plus a sum (a*b + c) of the reference's expressions is one fused
+// multiply-add, as the reference's CUDA compiler contracts them by default
+// (no -fmad=false in its build); the CUDA compiler's exact choice of which
+// products to fuse is not observable here, so this restatement fixes one.
 template <class T>
 struct AeifCondBeta {  // src/aeif_cond_beta_multisynapse_kernel.h:165-188
   static void deriv(int np, const T* y, T* dy, const T* p) {
@@ -161,12 +165,12 @@ struct AeifCondBeta {  // src/aeif_cond_beta_multisynapse_kernel.h:165-188
            t_ref, ref, den };
     T I_syn = 0;
     T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
-    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 4 * i] - V);
+    for (int i = 0; i < np; ++i) I_syn = std::fma(y[2 + 2 * i], p[14 + 4 * i] - V, I_syn);
     T V_spike = p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[0] = (p[ref] > 0) ? T(0)
-                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
+                         : (std::fma(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] +
                             p[I_e]) / p[C_m];
-    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    dy[1] = std::fma(p[a], V - p[E_L], -y[1]) / p[tau_w];
     for (int i = 0; i < np; ++i) {
       dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 4 * i + 1];
       dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 4 * i + 2];
@@ -181,12 +185,12 @@ struct AeifCondAlpha {  // src/aeif_cond_alpha_multisynapse_kernel.h:163-186
            t_ref, ref, den };
     T I_syn = 0;
     T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
-    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[14 + 3 * i] - V);
+    for (int i = 0; i < np; ++i) I_syn = std::fma(y[2 + 2 * i], p[14 + 3 * i] - V, I_syn);
     T V_spike = p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[0] = (p[ref] > 0) ? T(0)
-                         : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn - y[1] +
+                         : (std::fma(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] +
                             p[I_e]) / p[C_m];
-    dy[1] = (p[a] * (V - p[E_L]) - y[1]) / p[tau_w];
+    dy[1] = std::fma(p[a], V - p[E_L], -y[1]) / p[tau_w];
     for (int i = 0; i < np; ++i) {
       dy[3 + 2 * i] = -y[3 + 2 * i] / p[14 + 3 * i + 1];
       dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[14 + 3 * i + 1];
@@ -206,9 +210,9 @@ struct AeifPscExp {  // src/aeif_psc_exp_kernel.h:145-162
     T V_spike = p[Delta_T] == 0. ? T(0.)
                                  : p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
     dy[V_m] = (p[ref] > 0) ? T(0)
-                           : (-p[g_L] * (V - p[E_L] - V_spike) + I_syn_tot -
+                           : (std::fma(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) -
                               y[w] + p[I_e]) / p[C_m];
-    dy[w] = (p[a] * (V - p[E_L]) - y[w]) / p[tau_w];
+    dy[w] = std::fma(p[a], V - p[E_L], -y[w]) / p[tau_w];
     dy[I_ex] = -y[I_ex] / p[tau_ex];
     dy[I_in] = -y[I_in] / p[tau_in];
   }
@@ -220,13 +224,13 @@ struct IzhCondBeta {  // src/izhikevich_cond_beta_kernel.h:152-170
     enum { V_th, a, b, c, d, I_e, t_ref, ref, den };
     T I_syn = 0;
     T V = (p[ref] > 0) ? p[c] : y[0];
-    for (int i = 0; i < np; ++i) I_syn += y[2 + 2 * i] * (p[9 + 4 * i] - V);
+    for (int i = 0; i < np; ++i) I_syn = std::fma(y[2 + 2 * i], p[9 + 4 * i] - V, I_syn);
     // The reference evaluates this polynomial in double (double literals)
     // and stores the result in the state type.
     dy[0] = (p[ref] > 0) ? T(0)
                          : T(0.04 * V * V + 5.0 * V + 140.0 - y[1] + I_syn +
                              p[I_e]);
-    dy[1] = p[a] * (p[b] * V - y[1]);
+    dy[1] = p[a] * std::fma(p[b], V, -y[1]);
     for (int i = 0; i < np; ++i) {
       dy[3 + 2 * i] = -y[3 + 2 * i] / p[9 + 4 * i + 1];
       dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[9 + 4 * i + 2];
@@ -281,22 +285,23 @@ inline double inv_root_ieee(float x, int n) {
   }
   return std::ldexp(z * c5[r], -q);
 }
-// e^x of the AdEx exponential term: the float path evaluates it in double
-// (Cody-Waite reduction + degree-11 Taylor polynomial) and rounds to float,
-// i.e. the correctly rounded expf up to double-rounding ties; pinned against
-// exp in tests/test_oracle.py.  The FP64 path uses the library exp.
+// e^x of the AdEx exponential term, float path: Cody-Waite reduction
+// x = k ln2 + r with ln2 as a 15-bit float plus a float correction, the
+// degree-7 Taylor polynomial of e^r in Horner form with fma, and the exact
+// scaling by 2^k (one rounding where the result is subnormal).  Within one
+// ulp of exp (pinned against the library exp in tests/test_oracle.py).  The
+// FP64 path uses the library exp.
 inline float exp_model(float x) {
   if (!(x == x)) return x;
-  if (x > 89.0f) return (float)INFINITY;
-  if (x < -104.0f) return 0.0f;
-  double xd = x;
-  double k = std::rint(xd * 1.4426950408889634);
-  double r = xd - k * 0.6931471805599453;
-  const double c[12] = {1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0,
-                        1.0 / 5040.0, 1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
-  double p = c[0];
-  for (int i = 1; i < 12; ++i) p = p * r + c[i];
-  return (float)std::ldexp(p, (int)k);
+  if (x > 100.0f) return (float)INFINITY;
+  if (x < -110.0f) return 0.0f;
+  const float k = std::rint(x * 1.44269504f);
+  float r = std::fma(-k, 0.693145752f, x);
+  r = std::fma(-k, 1.42860677e-06f, r);
+  const float c[7] = {1.0f / 720.0f, 1.0f / 120.0f, 1.0f / 24.0f, 1.0f / 6.0f, 0.5f, 1.0f, 1.0f};
+  float p = 1.0f / 5040.0f;
+  for (int i = 0; i < 7; ++i) p = std::fma(p, r, c[i]);
+  return std::ldexp(p, (int)k);
 }
 inline double exp_model(double x) { return std::exp(x); }
 
@@ -330,30 +335,46 @@ void rk5_step(int np, int nv, double& x, T* y, T& h, T h_min, T h_max,
   for (;;) {
     if (h > h_max) h = h_max;
     if (h < h_min) h = h_min;
-    for (int i = 0; i < nv; ++i) y_new[i] = y[i] + h * K::a21 * k1[i];
+    // y + h * (sum_i a_i k_i): the sum accumulated left to right with fma
+    // from the first product, then one fma for y + h * sum
+    const T h21 = h * K::a21;
+    for (int i = 0; i < nv; ++i) y_new[i] = std::fma(h21, k1[i], y[i]);
     M::deriv(np, y_new, k2, p);
     for (int i = 0; i < nv; ++i)
-      y_new[i] = y[i] + h * (K::a31 * k1[i] + K::a32 * k2[i]);
+      y_new[i] = std::fma(h, std::fma(K::a32, k2[i], K::a31 * k1[i]), y[i]);
     M::deriv(np, y_new, k3, p);
-    for (int i = 0; i < nv; ++i)
-      y_new[i] = y[i] + h * (K::a41 * k1[i] + K::a42 * k2[i] + K::a43 * k3[i]);
+    for (int i = 0; i < nv; ++i) {
+      T s = std::fma(K::a42, k2[i], K::a41 * k1[i]);
+      s = std::fma(K::a43, k3[i], s);
+      y_new[i] = std::fma(h, s, y[i]);
+    }
     M::deriv(np, y_new, k4, p);
-    for (int i = 0; i < nv; ++i)
-      y_new[i] = y[i] + h * (K::a51 * k1[i] + K::a52 * k2[i] +
-                             K::a53 * k3[i] + K::a54 * k4[i]);
+    for (int i = 0; i < nv; ++i) {
+      T s = std::fma(K::a52, k2[i], K::a51 * k1[i]);
+      s = std::fma(K::a53, k3[i], s);
+      s = std::fma(K::a54, k4[i], s);
+      y_new[i] = std::fma(h, s, y[i]);
+    }
     M::deriv(np, y_new, k5, p);
-    for (int i = 0; i < nv; ++i)
-      y_new[i] = y[i] + h * (K::a61 * k1[i] + K::a62 * k2[i] +
-                             K::a63 * k3[i] + K::a64 * k4[i] + K::a65 * k5[i]);
+    for (int i = 0; i < nv; ++i) {
+      T s = std::fma(K::a62, k2[i], K::a61 * k1[i]);
+      s = std::fma(K::a63, k3[i], s);
+      s = std::fma(K::a64, k4[i], s);
+      s = std::fma(K::a65, k5[i], s);
+      y_new[i] = std::fma(h, s, y[i]);
+    }
     M::deriv(np, y_new, k6, p);
-    for (int i = 0; i < nv; ++i)
-      y_new[i] = y[i] + h * (K::a71 * k1[i] + K::a73 * k3[i] +
-                             K::a74 * k4[i] + K::a76 * k6[i]);
     err = 0.0;
     for (int i = 0; i < nv; ++i) {
-      T val = h * (K::e1 * k1[i] + K::e3 * k3[i] + K::e4 * k4[i] +
-                   K::e5 * k5[i] + K::e6 * k6[i]);
-      val /= y_scal[i];
+      T s = std::fma(K::a73, k3[i], K::a71 * k1[i]);
+      s = std::fma(K::a74, k4[i], s);
+      s = std::fma(K::a76, k6[i], s);
+      y_new[i] = std::fma(h, s, y[i]);
+      T e = std::fma(K::e3, k3[i], K::e1 * k1[i]);
+      e = std::fma(K::e4, k4[i], e);
+      e = std::fma(K::e5, k5[i], e);
+      e = std::fma(K::e6, k6[i], e);
+      T val = (h * e) / y_scal[i];
       err = std::max(err, (T)std::fabs(val));
     }
     err /= K::eps;
@@ -1253,6 +1274,9 @@ int orc_poisson(unsigned long long seed, unsigned int stream, long long t_idx, d
 // test hooks for the step factors
 float orc_step_factor_dec(float e) { return step_factor_dec(e); }
 float orc_exp_model(float x) { return exp_model(x); }
+void orc_exp_model_n(const float* x, float* out, long long n) {
+  for (long long i = 0; i < n; ++i) out[i] = exp_model(x[i]);
+}
 float orc_step_factor_inc(float e) { return step_factor_inc(e); }
 void orc_step_factors(const float* x, long long n, float* dec, float* inc) {
 #pragma omp parallel for schedule(static)

@@ -66,6 +66,9 @@ def lib():
         L.orc_step_factor_inc.argtypes = [ctypes.c_float]
         L.orc_exp_model.restype = ctypes.c_float
         L.orc_exp_model.argtypes = [ctypes.c_float]
+        L.orc_exp_model_n.restype = None
+        L.orc_exp_model_n.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                      ctypes.c_longlong]
         L.orc_mlp_n_params.restype = ctypes.c_int
         L.orc_mlp_forward.restype = None
         L.orc_mlp_forward.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int, vp]
@@ -240,4 +243,12 @@ def mlp_loss_grad(params, raw, target, cens, quant=1):
 
 
 def exp_model(x):
-    return lib().orc_exp_model(x)
+    """Float-path exponential of the AdEx models: a float for a scalar, an
+    array for an array."""
+    if np.isscalar(x):
+        return lib().orc_exp_model(x)
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(a)
+    lib().orc_exp_model_n(a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size)
+    return out

@@ -233,14 +233,30 @@ def test_step_factors_strided_sweep_of_all_positive_floats():
     assert np.isnan(d0[2]) and np.isnan(i0[2])
 
 
-def test_exp_model_is_correctly_rounded_exp():
+def test_exp_model_within_one_ulp_of_exp():
     # the AdEx exponential term (src/aeif_cond_beta_multisynapse_kernel.h:177)
-    # evaluated in double and rounded: equals float32(exp(float64 x)) except
-    # at double-rounding ties, which a random sample does not hit
-    rng = np.random.default_rng(11)
-    xs = np.concatenate([rng.uniform(-30, 10, 30000), rng.uniform(-103, 88, 5000),
-                         [0.0, -0.0, 1.0, -1.0, 88.5, -103.5]]).astype(np.float32)
-    for x in xs:
-        assert O.exp_model(float(x)) == np.float32(np.exp(np.float64(x))), x
-    assert O.exp_model(100.0) == float("inf")
-    assert O.exp_model(-200.0) == 0.0
+    # in float (Cody-Waite + degree-7 polynomial): every float in a strided
+    # sweep of [-110, 100] within 1 ulp of exp evaluated in float64, almost
+    # all correctly rounded (a dropped or wrong polynomial term breaks both);
+    # overflow, underflow and the special values of the IEEE exp
+    u = np.arange(0, 0x80000000, 1009, dtype=np.uint64).astype(np.uint32)
+    x = np.concatenate([u.view(np.float32), (u | np.uint32(0x80000000)).view(np.float32)])
+    x = x[np.isfinite(x) & (x > -110) & (x < 100)]
+    got = O.exp_model(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    fin = ref < np.float64(np.finfo(np.float32).max)
+    # ulp of the float result (subnormal spacing below the normal range)
+    r32 = ref[fin].astype(np.float32)
+    ulp = np.spacing(np.maximum(np.abs(r32), np.float32(np.finfo(np.float32).tiny))).astype(np.float64)
+    err = np.abs(got[fin] - ref[fin]) / ulp
+    print("exp: %d points, max error %.3f ulp, correctly rounded %.4f"
+          % (err.size, err.max(), np.mean(got[fin] == r32)))
+    assert err.max() < 1.0
+    assert np.mean(got[fin] == r32) > 0.99
+    assert np.all(np.isinf(got[~fin]))
+    assert O.exp_model(0.0) == 1.0 and O.exp_model(-0.0) == 1.0
+    assert O.exp_model(1.0) == np.float32(np.e)
+    assert O.exp_model(88.73) == float("inf") and O.exp_model(100.0) == float("inf")
+    assert O.exp_model(-104.0) == 0.0 and O.exp_model(-200.0) == 0.0
+    assert 0.0 < O.exp_model(-103.0) < np.finfo(np.float32).tiny   # subnormal result
+    assert np.isnan(O.exp_model(float("nan")))
