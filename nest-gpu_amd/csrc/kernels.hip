@@ -57,19 +57,29 @@ using ck::kMaxAttempts;
 // sequence.
 //
 // Every division of a right-hand side is by a parameter (M::divisor(j)).
-// D::div(x, p, r, j) returns x / p[j]: DivIEEE divides; DivRecip, for
-// populations whose divisor parameters are uniform with a usable reciprocal,
-// multiplies by r[j] = RN(1/p[j]) and applies Markstein's remainder
-// correction (div_rn), which gives the same float as the IEEE division with
-// 3 multiply/fma instead of the scaled reciprocal-iteration sequence.
+// dv.div(x, p, j) returns x / p[j].  DivIEEE divides.  DivFast, for
+// populations whose divisor parameters are uniform with a usable
+// reciprocal, multiplies by r[j] = RN(1/p[j]) and applies Markstein's
+// remainder correction, 3 multiply/fma giving the IEEE quotient for
+// dividends with magnitude in [2^-64, 2^64) and +0 (div_rn); any other
+// dividend (tiny, huge, inf, NaN, -0) only sets `bad`, and rhs() evaluates
+// the whole right-hand side again with DivIEEE when a lane of the wave has
+// it set: one wave-uniform branch, normally not taken, per evaluation
+// instead of a branch per division.  A negated dividend -x/d is written
+// -(x/d) (the same IEEE result) so that x = +0 stays on the fast path.
 struct DivIEEE {
-  __device__ static inline float div(float x, const float* p, const float*, int j) {
-    return x / p[j];
-  }
+  __device__ inline float div(float x, const float* p, int j) const { return x / p[j]; }
 };
-struct DivRecip {
-  __device__ static inline float div(float x, const float* p, const float* r, int j) {
-    return div_rn(x, p[j], r[j]);
+struct DivFast {
+  const float* r;
+  bool bad;
+  __device__ inline float div(float x, const float* p, int j) {
+    unsigned int u;
+    __builtin_memcpy(&u, &x, sizeof(u));
+    bad |= ((u & 0x7fffffffu) - 0x1f800000u >= 0x40000000u) & (u != 0u);
+    const float q = x * r[j];
+    const float rem = fmaf(-q, p[j], x);
+    return fmaf(rem, r[j], q);
   }
 };
 
@@ -90,20 +100,20 @@ struct AeifCondBeta {
   static constexpr int in_y(int k) { return 3 + 2 * k; }
   static constexpr int in_w(int k) { return 14 + 4 * k + 3; }
   template <class D>
-  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) I_syn = fmaf(y[2 + 2 * i], p[14 + 4 * i] - V, I_syn);
-    const float V_spike = p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
+    const float V_spike = p[Delta_T] * exp_fma(dv.div(V - p[V_th], p, Delta_T));
     dy[0] = (p[ref] > 0)
                 ? 0.0f
-                : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, r, C_m);
-    dy[1] = D::div(fmaf(p[a], V - p[E_L], -y[1]), p, r, tau_w);
+                : dv.div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, C_m);
+    dy[1] = dv.div(fmaf(p[a], V - p[E_L], -y[1]), p, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 4 * i + 1);
-      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 14 + 4 * i + 2);
+      dy[3 + 2 * i] = -dv.div(y[3 + 2 * i], p, 14 + 4 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - dv.div(y[2 + 2 * i], p, 14 + 4 * i + 2);
     }
   }
 };
@@ -124,20 +134,20 @@ struct AeifCondAlpha {
   static constexpr int in_y(int k) { return 3 + 2 * k; }
   static constexpr int in_w(int k) { return 14 + 3 * k + 2; }
   template <class D>
-  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[0], p[V_peak]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) I_syn = fmaf(y[2 + 2 * i], p[14 + 3 * i] - V, I_syn);
-    const float V_spike = p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
+    const float V_spike = p[Delta_T] * exp_fma(dv.div(V - p[V_th], p, Delta_T));
     dy[0] = (p[ref] > 0)
                 ? 0.0f
-                : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, r, C_m);
-    dy[1] = D::div(fmaf(p[a], V - p[E_L], -y[1]), p, r, tau_w);
+                : dv.div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn) - y[1] + p[I_e], p, C_m);
+    dy[1] = dv.div(fmaf(p[a], V - p[E_L], -y[1]), p, tau_w);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 14 + 3 * i + 1);
-      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 14 + 3 * i + 1);
+      dy[3 + 2 * i] = -dv.div(y[3 + 2 * i], p, 14 + 3 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - dv.div(y[2 + 2 * i], p, 14 + 3 * i + 1);
     }
   }
 };
@@ -158,19 +168,19 @@ struct AeifPscExp {
   }
   __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
   template <class D>
-  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn_tot = 0.0f;
     I_syn_tot += y[I_ex] - y[I_in];
     const float V = (p[ref] > 0) ? p[V_reset] : fminf(y[V_m], p[V_peak]);
     const float V_spike = p[Delta_T] == 0.0f
                               ? 0.0f
-                              : p[Delta_T] * exp_fma(D::div(V - p[V_th], p, r, Delta_T));
+                              : p[Delta_T] * exp_fma(dv.div(V - p[V_th], p, Delta_T));
     dy[V_m] = (p[ref] > 0)
                   ? 0.0f
-                  : D::div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) - y[w] + p[I_e], p, r, C_m);
-    dy[w] = D::div(fmaf(p[a], V - p[E_L], -y[w]), p, r, tau_w);
-    dy[I_ex] = D::div(-y[I_ex], p, r, tau_ex);
-    dy[I_in] = D::div(-y[I_in], p, r, tau_in);
+                  : dv.div(fmaf(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) - y[w] + p[I_e], p, C_m);
+    dy[w] = dv.div(fmaf(p[a], V - p[E_L], -y[w]), p, tau_w);
+    dy[I_ex] = -dv.div(y[I_ex], p, tau_ex);
+    dy[I_in] = -dv.div(y[I_in], p, tau_in);
   }
 };
 
@@ -187,7 +197,7 @@ struct IzhCondBeta {
   static constexpr bool divisor(int j) { return j >= 9 && (j - 9) % 4 >= 1 && (j - 9) % 4 <= 2; }
   __device__ static inline float hot_v(const float*) { return -45.0f; }
   template <class D>
-  __device__ static inline void deriv(const float* y, float* dy, const float* p, const float* r) {
+  __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn = 0.0f;
     const float V = (p[ref] > 0) ? p[c] : y[0];
 #pragma unroll
@@ -198,8 +208,8 @@ struct IzhCondBeta {
     dy[1] = p[a] * fmaf(p[b], V, -y[1]);
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      dy[3 + 2 * i] = D::div(-y[3 + 2 * i], p, r, 9 + 4 * i + 1);
-      dy[2 + 2 * i] = y[3 + 2 * i] - D::div(y[2 + 2 * i], p, r, 9 + 4 * i + 2);
+      dy[3 + 2 * i] = -dv.div(y[3 + 2 * i], p, 9 + 4 * i + 1);
+      dy[2 + 2 * i] = y[3 + 2 * i] - dv.div(y[2 + 2 * i], p, 9 + 4 * i + 2);
     }
   }
 };
@@ -293,6 +303,25 @@ __device__ inline void add_wave_counters(unsigned long long* ctr, int slot, int 
 }
 
 // ---------------------------------------------------------------------------
+// Right-hand side of a population: with uniform divisor parameters (UNI)
+// through DivFast, evaluated again with IEEE divisions when a lane of the
+// wave had a dividend outside DivFast's exact range.
+template <class M, bool UNI>
+__device__ inline void rhs(const float* y, float* dy, const float* p, const float* r) {
+  if constexpr (UNI) {
+    DivFast dv{r, false};
+    M::deriv(y, dy, p, dv);
+    if (__builtin_expect(__any(dv.bad), 0)) {
+      DivIEEE di;
+      M::deriv(y, dy, p, di);
+    }
+  } else {
+    DivIEEE di;
+    M::deriv(y, dy, p, di);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // One Cash-Karp attempt of step h from (y, k1), src/rk5.h:88-131: the five
 // further stages, the 5th-order solution yn and the error ratio
 // max_j |h sum_i e_i k_i[j]| / ys[j] (before the division by eps).  Each
@@ -381,7 +410,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
       // ---- RK5Step ----
       float k1[M::NV];
       float yn[M::NV], ys[M::NV];
-      M::template deriv<DivIEEE>(y, k1, p, nullptr);
+      rhs<M, false>(y, k1, p, nullptr);
 #pragma unroll
       for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
       float err;
@@ -389,7 +418,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
         if (h > h_max) h = h_max;
         if (h < h_min) h = h_min;
         err = rk5_try<M>(y, k1, ys, h, yn, [&](const float* yy, float* dd) {
-          M::template deriv<DivIEEE>(yy, dd, p, nullptr);
+          rhs<M, false>(yy, dd, p, nullptr);
         });
         err /= ck::eps;
         ++n_att;
@@ -455,7 +484,6 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
 template <class M, bool UNI, int MODE>
 __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
-  using D = typename std::conditional<UNI, DivRecip, DivIEEE>::type;
   static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * (blockDim.x / kWave) +
@@ -605,7 +633,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
       // ---- start of a substep: k1 and the error scale (src/rk5.h:83-86) ----
       if (need_k1) {
         if constexpr (MODE == 1) h_sub = h;
-        M::template deriv<D>(y, k1, p, r);
+        rhs<M, UNI>(y, k1, p, r);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
         need_k1 = false;
@@ -620,7 +648,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
       if (h < h_min) h = h_min;
       float yn[M::NV];
       float err = rk5_try<M>(y, k1, ys, h, yn, [&](const float* yy, float* dd) {
-        M::template deriv<D>(yy, dd, p, r);
+        rhs<M, UNI>(yy, dd, p, r);
       });
       err = div_rn(err, ck::eps, r_eps);
       ++n_att;

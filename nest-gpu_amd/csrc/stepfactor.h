@@ -107,11 +107,11 @@ NGPU_HD inline float div_rn(float a, float d, float r) {
 // subnormal, overflowing to +inf above ~88.72).  Error < 1 ulp over the
 // whole float range (0.93 ulp at most; 99.5% of results correctly rounded).
 NGPU_HD inline float exp_fma(float x) {
-  if (!(x == x)) return x;
-  if (x > 100.0f) return (float)INFINITY;
-  if (x < -110.0f) return 0.0f;
-  const float k = rintf(x * 1.44269504f);
-  float r = fmaf(-k, 0.693145752f, x);
+  // branch-free: the polynomial runs on the clamped argument and the special
+  // cases are selected afterwards (NaN -> x, above 100 -> +inf, below -110 -> 0)
+  const float xc = fminf(fmaxf(x, -110.0f), 100.0f);
+  const float k = rintf(xc * 1.44269504f);
+  float r = fmaf(-k, 0.693145752f, xc);
   r = fmaf(-k, 1.42860677e-06f, r);
   float p = 1.0f / 5040.0f;
   p = fmaf(p, r, 1.0f / 720.0f);
@@ -121,7 +121,10 @@ NGPU_HD inline float exp_fma(float x) {
   p = fmaf(p, r, 0.5f);
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
-  return ldexpf(p, (int)k);
+  float e = ldexpf(p, (int)k);
+  e = x > 100.0f ? (float)INFINITY : e;
+  e = x < -110.0f ? 0.0f : e;
+  return x == x ? e : x;
 }
 
 }  // namespace ngpu
