@@ -138,6 +138,24 @@ Engine::~Engine() {
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+// Host waits for all queued work of the simulator's stream.
+void Engine::sync_streams() {
+  if (stream_) HIPCHK(hipStreamSynchronize(stream_));
+}
+
+// Time of step `it` of the current Simulate() call, src/nestgpu.cu:486-490
+// (the same double expressions as before each step of the reference).
+StepCtx Engine::step_ctx(long long it) const {
+  StepCtx c;
+  c.it = it;
+  c.t_idx = (long long)(int)std::round(neur_t0_ / dt_) + it + 1;
+  c.t0 = neur_t0_;
+  c.t1 = neur_t0_ + (double)dt_ * (double)(it + 1);
+  c.slot = (int)(c.t_idx % n_slot_);
+  c.parity = (int)(c.t_idx & 1);
+  return c;
+}
+
 void Engine::check_uncalibrated(const char* msg) const {
   if (calibrated_) throw Error(msg);
 }
@@ -754,7 +772,7 @@ int* Engine::GetIntVar(int i_node, int n, const std::string& name) {
   out.assign(n, 0);
   if (calibrated_) {
     DBuf<int>& d = (name == "spike_count") ? g.spike_count : g.n_rec;
-    HIPCHK(hipStreamSynchronize(stream_));
+    sync_streams();
     HIPCHK(hipMemcpy(out.data(), d.p + rel0, n * sizeof(int), hipMemcpyDeviceToHost));
   }
   return out.data();
@@ -1195,7 +1213,7 @@ int Engine::GetNRecSpikeTimes(int node) {
   if (g.max_rec <= 0) throw Error("Spike times recording was not activated");
   if (!calibrated_) return 0;
   int v;
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   HIPCHK(hipMemcpy(&v, g.n_rec.p + rel, sizeof(int), hipMemcpyDeviceToHost));
   return v;
 }
@@ -1204,7 +1222,7 @@ int Engine::GetNRecSpikeTimes(int node) {
 // (src/base_neuron.cu:1488-1526).
 void Engine::buffer_rec_spike_times(Group& g) {
   if (!calibrated_ || g.max_rec <= 0) return;
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   std::vector<int> cnt(g.n);
   HIPCHK(hipMemcpy(cnt.data(), g.n_rec.p, g.n * sizeof(int), hipMemcpyDeviceToHost));
   long long tot = 0;
@@ -1381,7 +1399,7 @@ int Engine::Calibrate() {
   upload(d_syn_ch_, ch, stream_);
   upload(d_syn_w_, c_w_, stream_);
   upload(d_syn_d_, dd, stream_);
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   ch.clear(); ch.shrink_to_fit(); dd.clear(); dd.shrink_to_fit();
   d_ring_.alloc((size_t)n_slot_ * std::max<long long>(n_ch_, 1));
   HIPCHK(hipMemsetAsync(d_ring_.p, 0, d_ring_.n * sizeof(double), stream_));
@@ -1397,7 +1415,6 @@ int Engine::Calibrate() {
   upload(d_last_idx_, guard, stream_);
   d_last_h_.alloc(std::max(n_nodes_, 1));
   HIPCHK(hipMemsetAsync(d_last_h_.p, 0, d_last_h_.n * sizeof(float), stream_));
-  d_ctx_.alloc(1);
   {
     // persistent RK5 grid: 4 waves per SIMD on every CU
     int dev = 0, cus = 256;
@@ -1511,7 +1528,7 @@ int Engine::Calibrate() {
     }
     if (g->max_rec > 0) g->rec_times.alloc((size_t)g->n * g->max_rec);
   }
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   for (size_t k = 0; k < groups_.size(); ++k) {
     stage_var_[k].clear(); stage_var_[k].shrink_to_fit();
     stage_param_[k].clear(); stage_param_[k].shrink_to_fit();
@@ -1547,7 +1564,7 @@ void Engine::upload_poisson_tables() {
   HIPCHK(hipMemcpyAsync(d_gen_origin_.p, org.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
   HIPCHK(hipMemcpyAsync(d_gen_start_.p, sta.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
   HIPCHK(hipMemcpyAsync(d_gen_stop_.p, sto.data(), n_gen_ * sizeof(float), hipMemcpyHostToDevice, stream_));
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
 }
 
 static ParamTable make_table(const Group& g) {
@@ -1570,7 +1587,7 @@ FoldArgs Engine::make_fold_args(const Group& g) {
   fa.param = g.param.p; fa.tab = make_table(g);
   fa.w_param = fl.w_param;
   fa.w_port_stride = fl.w_port_stride;
-  fa.ring = d_ring_.p; fa.n_ch = n_ch_; fa.ctx = d_ctx_.p;
+  fa.ring = d_ring_.p; fa.n_ch = n_ch_;
   if (g.has_pe) {
     fa.pe_off = g.pe_off.p; fa.pe_gen = g.pe_gen.p; fa.pe_w = g.pe_w.p; fa.pe_delay = g.pe_delay.p;
     fa.pe_stream = g.pe_stream.p; fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
@@ -1581,6 +1598,7 @@ FoldArgs Engine::make_fold_args(const Group& g) {
 }
 
 void Engine::launch_step() {
+  const StepCtx sc = step_ctx(it_);
   SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
   for (size_t gi = 0; gi < groups_.size(); ++gi) {
     Group* g = groups_[gi];
@@ -1593,7 +1611,7 @@ void Engine::launch_step() {
       case MODEL_IZH_COND_BETA: {
         RK5Args a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
-        a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
         a.ctr = d_ctr_.p;
         a.persistent = rk5_persistent_ ? 1 : 0;
         a.max_blocks = rk5_max_blocks_;
@@ -1629,14 +1647,14 @@ void Engine::launch_step() {
       case MODEL_IAF_PSC_EXP: {
         IafArgs a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.tab = make_table(*g);
-        a.dt = dt_; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
         timed(KT_IAF, [&] { launch_iaf(a, stream_); });
         break;
       }
       case MODEL_SPIKE_GEN: {
         SpikeGenArgs a{};
         a.n = g->n; a.i0 = g->i0; a.off = g->sg_off.p; a.t_idx = g->sg_tidx.p; a.height = g->sg_h.p;
-        a.next = g->sg_next.p; a.ctx = d_ctx_.p; a.sl = sl; a.sb = sb;
+        a.next = g->sg_next.p; a.ctx = sc; a.sl = sl; a.sb = sb;
         timed(KT_OTHER, [&] { launch_spike_gen(a, stream_); });
         break;
       }
@@ -1650,27 +1668,28 @@ void Engine::launch_step() {
     }
   }
   if (rec_ncol_ > 0) {
-    RecordArgs ra{d_rec_cols_.p, rec_ncol_, d_rec_buf_.p, d_ctx_.p, d_last_idx_.p, d_last_h_.p,
+    RecordArgs ra{d_rec_cols_.p, rec_ncol_, d_rec_buf_.p, sc, d_last_idx_.p, d_last_h_.p,
                   rec_row_base_, rec_buf_rows_};
     timed(KT_OTHER, [&] { launch_record(ra, stream_); });
   }
   DeliverArgs da{};
-  da.ctx = d_ctx_.p; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
-  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_; da.n_slot = n_slot_;
-  da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
+  da.ctx = sc; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
+  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_;
+  da.n_slot = n_slot_; da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
   timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
     FoldArgs fa = make_fold_args(*g);
+    fa.ctx = sc;
     fa.lag = g->lag.p;
     fa.pend = g->pend.p;
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
-  timed(KT_OTHER, [&] { launch_end_step(d_ctx_.p, dt_, n_slot_, stream_); });
   if (timing_ && ev_pending_.size() >= 4096) flush_timing();
 }
 
-void Engine::timed(int cat, const std::function<void()>& launch) {
+void Engine::timed(int cat, const std::function<void()>& launch, hipStream_t st) {
+  if (!st) st = stream_;
   if (!timing_ || (timing_rk5_only_ && cat != KT_RK5)) {
     launch();
     HIPCHK(hipGetLastError());   // a launch that cannot run fails the step loudly
@@ -1687,16 +1706,16 @@ void Engine::timed(int cat, const std::function<void()>& launch) {
     return e;
   };
   EvPair p{get(), get(), cat};
-  HIPCHK(hipEventRecord(p.a, stream_));
+  HIPCHK(hipEventRecord(p.a, st));
   launch();
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(p.b, stream_));
+  HIPCHK(hipEventRecord(p.b, st));
   ev_pending_.push_back(p);
 }
 
 void Engine::flush_timing() {
   if (ev_pending_.empty()) return;
-  HIPCHK(hipEventSynchronize(ev_pending_.back().b));
+  sync_streams();
   for (auto& p : ev_pending_) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
@@ -1724,7 +1743,7 @@ int Engine::StartSimulation() {
   if (!calibrated_) Calibrate();
   if (first_sim_) {
     // record row at the initial time (src/nestgpu.cu:383-388)
-    HIPCHK(hipStreamSynchronize(stream_));
+    sync_streams();
     for (auto& r : records_) {
       std::vector<float> row{(float)neural_time_};
       for (size_t k = 0; k < r.var.size(); ++k) {
@@ -1750,14 +1769,6 @@ int Engine::StartSimulation() {
   it_ = 0;
   nt_ = (long long)std::round(sim_time_ / dt_);
   upload_poisson_tables();
-  StepCtx c;
-  c.it = 0;
-  c.t_idx = (long long)(int)std::round(neur_t0_ / dt_) + 1;
-  c.t0 = neur_t0_;
-  c.t1 = neur_t0_ + (double)dt_ * 1.0;
-  c.slot = (int)(c.t_idx % n_slot_);
-  c.parity = (int)(c.t_idx & 1);
-  HIPCHK(hipMemcpyAsync(d_ctx_.p, &c, sizeof(c), hipMemcpyHostToDevice, stream_));
   // record columns and buffer for this call
   std::vector<RecCol> cols;
   for (auto* g : groups_) g->recorded = false;
@@ -1790,7 +1801,7 @@ int Engine::SimulationStep() {
   if (first_sim_ || !calibrated_) StartSimulation();
   if (rec_ncol_ > 0 && rec_rows_pending_ >= rec_buf_rows_) {
     // record buffer full: move its rows to the host and restart it
-    HIPCHK(hipStreamSynchronize(stream_));
+    sync_streams();
     fetch_records(rec_rows_pending_);
     rec_rows_pending_ = 0;
     rec_row_base_ = it_;
@@ -1831,7 +1842,7 @@ void Engine::fetch_records(long long n_rows) {
 }
 
 int Engine::EndSimulation() {
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   if (timing_) flush_timing();
   fetch_records(rec_rows_pending_);
   rec_rows_pending_ = 0;
@@ -1858,7 +1869,7 @@ int Engine::Simulate() {
 void Engine::Stats(unsigned long long* out) {
   for (int k = 0; k < 8; ++k) out[k] = 0;
   if (!calibrated_) return;
-  HIPCHK(hipStreamSynchronize(stream_));
+  sync_streams();
   std::vector<unsigned long long> rows(d_ctr_.n);
   HIPCHK(hipMemcpy(rows.data(), d_ctr_.p, rows.size() * sizeof(unsigned long long),
                    hipMemcpyDeviceToHost));
@@ -1874,7 +1885,7 @@ void Engine::Stats(unsigned long long* out) {
 }
 
 void Engine::Synchronize() {
-  if (stream_) HIPCHK(hipStreamSynchronize(stream_));
+  if (stream_) sync_streams();
 }
 
 template struct DBuf<float>;

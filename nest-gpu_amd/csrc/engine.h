@@ -241,7 +241,9 @@ class Engine {
   int param_idx(const Group& g, const std::string& name, int port) const;
   int var_idx(const Group& g, const std::string& name, int port) const;
   void random_block(uint32_t out[4]);
-  void timed(int cat, const std::function<void()>& launch);
+  void timed(int cat, const std::function<void()>& launch, hipStream_t st = nullptr);
+  void sync_streams();
+  StepCtx step_ctx(long long it) const;
   void flush_timing();
 
   // kernel params
@@ -283,7 +285,6 @@ class Engine {
   int spk_cap_ = 0;
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
-  DBuf<StepCtx> d_ctx_;
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
   int rk5_defer_cap_ = 24;             // attempts after which a step is split over two launches (0: never)
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)

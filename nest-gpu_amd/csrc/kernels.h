@@ -42,7 +42,7 @@ struct FoldArgs {
   int w_port_stride;
   double* ring;
   long long n_ch;
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   // poisson direct connections into this population, CSR by local channel
   const int* pe_off;     // [n*n_port+1] or null
   const int* pe_gen;     // generator index
@@ -72,7 +72,7 @@ struct RK5Args {
   ParamTable tab;        // uniform parameters
   float h_min;
   float dt;
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
   SpikeBook sb;
   unsigned long long* ctr;    // per-wave counters [slot][kCtrCols] or null
@@ -116,7 +116,7 @@ struct IafArgs {
   float* param;
   ParamTable tab;
   float dt;
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
   SpikeBook sb;
 };
@@ -127,7 +127,7 @@ struct SpikeGenArgs {
   const int* t_idx;      // spike step indexes (relative to Simulate start)
   const float* height;
   int* next;             // [n] next spike to emit
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
   SpikeBook sb;
 };
@@ -139,7 +139,7 @@ struct DetectorArgs {
 };
 
 struct DeliverArgs {
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
   const long long* row_ptr;    // [n_nodes+1] synapse segment per source
   const unsigned int* syn_ch;  // [n_syn] target channel
@@ -163,7 +163,7 @@ struct RecordArgs {
   const RecCol* cols;
   int ncol;
   float* buf;            // [rows][ncol+1]
-  const StepCtx* ctx;
+  StepCtx ctx;           // this step's time (by value)
   const long long* last_idx;
   const float* last_height;
   long long row_base;    // step index (ctx->it) stored in buffer row 0
@@ -180,6 +180,5 @@ void launch_detector(const DetectorArgs& a, hipStream_t s);
 void launch_deliver(const DeliverArgs& a, hipStream_t s);   // kDeliverBlocks x 256
 void launch_fold(const FoldArgs& a, hipStream_t s);
 void launch_record(const RecordArgs& a, hipStream_t s);
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s);
 
 }  // namespace ngpu

@@ -397,9 +397,9 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
       p[j] = a.tab.is_per_neuron(j) ? a.param[(size_t)j * n + i] : a.tab.val[j];
     double x = a.x[i];
     float h = a.h[i];
-    const double x1 = a.ctx->t1;
-    const long long t_idx = a.ctx->t_idx;
-    const int parity = a.ctx->parity;
+    const double x1 = a.ctx.t1;
+    const long long t_idx = a.ctx.t_idx;
+    const int parity = a.ctx.parity;
     const float h_min = a.h_min;
     const float ref_in = p[M::i_ref];
 
@@ -490,12 +490,12 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
                    __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_waves = gridDim.x * (blockDim.x / kWave);
   const int n = a.n;
-  const double x1 = a.ctx->t1;
-  const long long t_idx = a.ctx->t_idx;
-  const int parity = a.ctx->parity;
+  const double x1 = a.ctx.t1;
+  const long long t_idx = a.ctx.t_idx;
+  const int parity = a.ctx.parity;
   // end of the previous step (the end_step_kernel formula one step back): a
   // lagged neuron integrates up to it before starting this step
-  const double x1_prev = a.ctx->t0 + (double)a.dt * (double)a.ctx->it;
+  const double x1_prev = a.ctx.t0 + (double)a.dt * (double)a.ctx.it;
   // the wave's own hot list first (neurons of its chunk that ended the last
   // step about to spike, or split; ~1 per wave, so each takes a lane of its
   // own and the long attempt chains start together in the first iteration),
@@ -926,7 +926,7 @@ __global__ void __launch_bounds__(256) iaf_kernel(IafArgs a) {
   iex *= P(P11ex);
   iin *= P(P11in);
   if (v >= P(Theta)) {
-    emit_spike(a.sl, a.sb, a.ctx->parity, a.i0 + i, i, 1.0f, a.ctx->t_idx, a.ctx->t1);
+    emit_spike(a.sl, a.sb, a.ctx.parity, a.i0 + i, i, 1.0f, a.ctx.t_idx, a.ctx.t1);
     v = P(V_reset);
     ref = (float)(int)roundf(P(t_ref) / a.dt);
   }
@@ -950,8 +950,8 @@ __global__ void spike_gen_kernel(SpikeGenArgs a) {
   const int b = a.off[i], e = a.off[i + 1];
   if (b == e) return;
   const int is = b + a.next[i];
-  if (is < e && (long long)a.t_idx[is] == a.ctx->it) {
-    emit_spike(a.sl, a.sb, a.ctx->parity, a.i0 + i, i, a.height[is], a.ctx->t_idx, a.ctx->t1);
+  if (is < e && (long long)a.t_idx[is] == a.ctx.it) {
+    emit_spike(a.sl, a.sb, a.ctx.parity, a.i0 + i, i, a.height[is], a.ctx.t_idx, a.ctx.t1);
     a.next[i]++;
   }
 }
@@ -993,10 +993,10 @@ void launch_detector(const DetectorArgs& a, hipStream_t s) {
 constexpr int kDeliverUnroll = 4;
 
 __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
-  const int parity = a.ctx->parity;
+  const int parity = a.ctx.parity;
   int n_spk = a.sl.count[parity];
   if (n_spk > a.sl.cap) n_spk = a.sl.cap;
-  const int base = (int)((a.ctx->t_idx + 1) % a.n_slot);
+  const int base = (int)((a.ctx.t_idx + 1) % a.n_slot);
   const int* srcs = a.sl.src + (size_t)parity * a.sl.cap;
   const float* hts = a.sl.height + (size_t)parity * a.sl.cap;
   const int stride = blockDim.x * kDeliverUnroll;
@@ -1064,7 +1064,7 @@ __global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
   if (idx >= total) return;
   const int port = (int)(idx / a.n);
   const int i = (int)(idx - (long long)port * a.n);
-  const double acc = channel_input(a, idx, a.ctx->slot, a.ctx->t1, a.ctx->t_idx);
+  const double acc = channel_input(a, idx, a.ctx.slot, a.ctx.t1, a.ctx.t_idx);
   if (a.lag && a.lag[i]) {   // still inside this step: its RK5 launch adds the input later
     a.pend[idx] = acc;
     return;
@@ -1093,17 +1093,17 @@ void launch_fold(const FoldArgs& a, hipStream_t s) {
 __global__ void record_kernel(RecordArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c > a.ncol) return;
-  const long long row = a.ctx->it - a.row_base;
+  const long long row = a.ctx.it - a.row_base;
   if (row < 0 || row >= a.max_rows) return;
   float v;
   if (c == 0) {
-    v = (float)a.ctx->t1;
+    v = (float)a.ctx.t1;
   } else {
     const RecCol rc = a.cols[c - 1];
     if (rc.ptr) {
       v = *rc.ptr;
     } else {
-      v = (rc.has_out && a.last_idx[rc.node] == a.ctx->t_idx) ? a.last_height[rc.node] : 0.0f;
+      v = (rc.has_out && a.last_idx[rc.node] == a.ctx.t_idx) ? a.last_height[rc.node] : 0.0f;
     }
   }
   a.buf[(size_t)row * (a.ncol + 1) + c] = v;
@@ -1111,21 +1111,6 @@ __global__ void record_kernel(RecordArgs a) {
 
 void launch_record(const RecordArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(record_kernel, dim3((a.ncol + 1 + 255) / 256), dim3(256), 0, s, a);
-}
-
-// Advance the step context (src/nestgpu.cu:486-490 semantics).
-__global__ void end_step_kernel(StepCtx* ctx, float dt, int n_slot) {
-  StepCtx c = *ctx;
-  c.it += 1;
-  c.t_idx += 1;
-  c.t1 = c.t0 + (double)dt * (double)(c.it + 1);
-  c.slot = (int)(c.t_idx % n_slot);
-  c.parity = (int)(c.t_idx & 1);
-  *ctx = c;
-}
-
-void launch_end_step(StepCtx* ctx, float dt, int n_slot, hipStream_t s) {
-  hipLaunchKernelGGL(end_step_kernel, dim3(1), dim3(1), 0, s, ctx, dt, n_slot);
 }
 
 }  // namespace ngpu
