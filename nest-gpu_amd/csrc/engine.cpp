@@ -1423,6 +1423,8 @@ int Engine::Calibrate() {
     rk5_max_blocks_ = cus * 4;
   }
   if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_RK5_HOT_MARGIN")) rk5_hot_margin_ = (float)atof(env);
+  if (const char* env = getenv("NGPU_RK5_HOT_ONLY")) rk5_hot_only_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
     rk5_defer_env_ = true;
@@ -1629,6 +1631,8 @@ void Engine::launch_step() {
           a.hot_flag = g->hot_flag.p;
           a.hot_list = g->hot_list.p;
           a.hot_count = g->hot_count.p;
+          a.hot_margin = rk5_hot_margin_;
+          a.hot_only = rk5_hot_only_ ? 1 : 0;
         }
         if (g->lag.p) {
           a.lag = g->lag.p;

@@ -288,6 +288,8 @@ class Engine {
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
   int rk5_defer_cap_ = 24;             // attempts after which a step is split over two launches (0: never)
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
+  float rk5_hot_margin_ = 2.5f;        // hot list from V_th + margin * Delta_T (AdEx models)
+  bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]

@@ -94,6 +94,8 @@ struct RK5Args {
   unsigned char* hot_flag;         // [2][n] neuron listed hot for the step (by t_idx parity), or null
   int* hot_list;                   // [3][n]
   int* hot_count;                  // [3]
+  float hot_margin;                // AdEx models: listed from V >= V_th + hot_margin * Delta_T
+  int hot_only;                    // waves holding hot entries take no bulk share
   // split steps: a neuron still integrating after defer_cap attempts of a
   // step (at a substep boundary) is stored with lag = 1, finishes that step
   // first in the next launch, then adds that step's input (stashed in pend

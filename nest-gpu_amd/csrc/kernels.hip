@@ -94,8 +94,9 @@ struct AeifCondBeta {
   static constexpr bool divisor(int j) {
     return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 4 >= 1 && (j - 14) % 4 <= 2);
   }
-  // membrane potential from which a neuron is scheduled first (spike upswing)
-  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
+  // membrane potential from which a neuron is scheduled first (spike
+  // upswing): V_th + m Delta_T
+  __device__ static inline float hot_v(const float* p, float m) { return fmaf(m, p[Delta_T], p[V_th]); }
   // input of port k: variable g1_k, weight g0_k (src/aeif_cond_beta_multisynapse.cu:151-159)
   static constexpr int in_y(int k) { return 3 + 2 * k; }
   static constexpr int in_w(int k) { return 14 + 4 * k + 3; }
@@ -129,7 +130,7 @@ struct AeifCondAlpha {
   static constexpr bool divisor(int j) {
     return j == Delta_T || j == C_m || j == tau_w || (j >= 14 && (j - 14) % 3 == 1);
   }
-  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
+  __device__ static inline float hot_v(const float* p, float m) { return fmaf(m, p[Delta_T], p[V_th]); }
   // input of port k: variable g1_k, weight g0_k
   static constexpr int in_y(int k) { return 3 + 2 * k; }
   static constexpr int in_w(int k) { return 14 + 3 * k + 2; }
@@ -166,7 +167,7 @@ struct AeifPscExp {
   static constexpr bool divisor(int j) {
     return j == tau_ex || j == tau_in || j == Delta_T || j == C_m || j == tau_w;
   }
-  __device__ static inline float hot_v(const float* p) { return p[V_th] + 2.5f * p[Delta_T]; }
+  __device__ static inline float hot_v(const float* p, float m) { return fmaf(m, p[Delta_T], p[V_th]); }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn_tot = 0.0f;
@@ -195,7 +196,7 @@ struct IzhCondBeta {
   static constexpr int iV = 0, iw = 1, i_reset = c, i_thr = V_th, i_jump = d,
                        i_tref = t_ref, i_ref = ref;
   static constexpr bool divisor(int j) { return j >= 9 && (j - 9) % 4 >= 1 && (j - 9) % 4 <= 2; }
-  __device__ static inline float hot_v(const float*) { return -45.0f; }
+  __device__ static inline float hot_v(const float*, float) { return -45.0f; }
   template <class D>
   __device__ static inline void deriv(const float* y, float* dy, const float* p, D& dv) {
     float I_syn = 0.0f;
@@ -526,9 +527,17 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
                      : (int)((((long long)n_hot + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1));
   const long long hs = (long long)wave * hp;
   const int n_hw = hs < n_hot ? (int)((hs + hp < n_hot ? hs + hp : n_hot) - hs) : 0;
-  const long long bchunk = (((long long)n + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1);
-  const long long b0 = (long long)wave * bchunk;
-  const int n_bw = b0 < n ? (int)((b0 + bchunk < n ? b0 + bchunk : n) - b0) : 0;
+  // while the hot entries fill less than half the grid, the waves holding
+  // them take no share of the bulk (a wave's bulk refills would stall its
+  // long chains on memory); the other waves split all neurons
+  const int w_hot = (int)(((long long)n_hot + hp - 1) / hp);
+  const bool hot_only = a.hot_only && w_hot > 0 && 2 * w_hot <= n_waves;
+  const int n_bulk_waves = hot_only ? n_waves - w_hot : n_waves;
+  const long long bwave = hot_only ? (long long)wave - w_hot : (long long)wave;
+  const long long bchunk =
+      (((long long)n + n_bulk_waves - 1) / n_bulk_waves + kWave - 1) & ~(long long)(kWave - 1);
+  const long long b0 = bwave * bchunk;
+  const int n_bw = (bwave >= 0 && b0 < n) ? (int)((b0 + bchunk < n ? b0 + bchunk : n) - b0) : 0;
   long long next = 0;
   const long long end = (long long)n_hw + n_bw;
   // the waves holding the chains issue first; the others fill the free slots
@@ -726,7 +735,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
           a.h[i] = h;
           if constexpr (MODE != 0) a.err_last[i] = el;
           if (p[M::i_ref] != ref_in) a.param[(size_t)M::i_ref * n + i] = p[M::i_ref];
-          pend = 1 | ((!end_step || (p[M::i_ref] <= 0.0f && y[M::iV] >= M::hot_v(p))) ? 2 : 0);
+          pend = 1 | ((!end_step || (p[M::i_ref] <= 0.0f && y[M::iV] >= M::hot_v(p, a.hot_margin))) ? 2 : 0);
           pend_i = i;
           i = -1;
         }
