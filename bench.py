@@ -153,7 +153,9 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
                           "bytes_per_launch": b / cnt}
     ms, cnt = kt["fold_poisson"]
     if cnt:
-        b = (n * n_port * FOLD_BYTES_PER_CHANNEL + n_poisson_edges * POISSON_BYTES_PER_EDGE) * steps
+        # one launch per step, or only the last step of a Simulate() call when
+        # the persistent RK5 launches do the fold themselves
+        b = (n * n_port * FOLD_BYTES_PER_CHANNEL + n_poisson_edges * POISSON_BYTES_PER_EDGE) * cnt
         a = b / (ms * 1e-3) / 1e9
         out["fold_poisson"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
                                "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
