@@ -238,7 +238,10 @@ int NESTGPU_AMD_Synchronize();
 int NESTGPU_AMD_SetStepPredictorMode(int mode);
 /* Allocates (and clears) the log: capacity records of 10 floats (8 raw
  * inputs, target log2(h/dt), censored flag), every substep of the neurons
- * whose index within their population is a multiple of stride. */
+ * whose index within their population is a multiple of stride.  A record
+ * is censored (its target only a lower bound of a good proposal) when the
+ * accepted step was cut by the step end or accepted at the first attempt;
+ * after a rejection the accepted step is the target itself. */
 int NESTGPU_AMD_StartPredictorLog(long long capacity, int stride);
 long long NESTGPU_AMD_GetPredictorLogCount();
 int NESTGPU_AMD_GetPredictorLog(float *out, long long first, long long n);

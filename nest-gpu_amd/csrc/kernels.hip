@@ -558,6 +558,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
   float el = 1.0f;      // last accepted error ratio (predictor input)
   float h_sub = 0.0f;   // controller proposal at the start of the substep
   bool clip = false;    // accepted attempt was cut by the end of the step
+  bool rej_sub = false; // the current substep had a rejected attempt
   __shared__ PredLDS s_pred;
   if constexpr (MODE == 2) {
     pred_stage(s_pred, a.pred_params);
@@ -641,7 +642,10 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
     if (i >= 0) {
       // ---- start of a substep: k1 and the error scale (src/rk5.h:83-86) ----
       if (need_k1) {
-        if constexpr (MODE == 1) h_sub = h;
+        if constexpr (MODE == 1) {
+          h_sub = h;
+          rej_sub = false;
+        }
         rhs<M, UNI>(y, k1, p, r);
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) ys[j] = fabsf(y[j]) + fabsf(k1[j] * h) + ck::scal_min;
@@ -667,6 +671,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
       }
       if (!(err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5))) {
         ++n_rej;   // rejected: shrink h, retry with the same k1 and scale
+        if constexpr (MODE == 1) rej_sub = true;
         const float h_new = h * ck::safety * pow_m025(err);
         h = (float)fmax((double)h_new, 0.1 * (double)h);
       } else {
@@ -681,7 +686,11 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
               for (int k = 0; k < kPI; ++k) rec[k] = raw[k];
               rec[kPI] = __log2f(h / a.dt);
-              rec[kPI + 1] = clip ? 1.0f : 0.0f;   // h cut by the end of the step: lower bound
+              // lower bound only (censored): h cut by the end of the step,
+              // or accepted at the first try (a longer step might have
+              // passed too); exact after a rejection, where the controller
+              // overshot and the accepted h is what should have been proposed
+              rec[kPI + 1] = (clip || !rej_sub) ? 1.0f : 0.0f;
             }
           }
         }
@@ -792,30 +801,57 @@ long long rk5_waves(int n, int persistent, int max_blocks) {
                     : 4LL * ((n + 255) / 256);
 }
 
+// Blocks of one persistent RK5 instantiation that are resident on the device
+// at once (its register and LDS use decide; every block must be resident,
+// since each wave owns a static chunk of the population).
+template <class M, bool UNI, int MODE>
+static int resident_blocks() {
+  static int blocks = 0;
+  if (blocks == 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rk5_persistent_kernel<M, UNI, MODE>,
+                                                     kRK5Block, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu <= 0 || cus <= 0)
+      throw std::runtime_error("cannot query the occupancy of the persistent RK5 kernel");
+    blocks = per_cu * cus;
+  }
+  return blocks;
+}
+
+template <class M, bool UNI, int MODE>
+static void launch_persistent(const RK5Args& a, hipStream_t s) {
+  // enough waves to fill the GPU with every block resident, at most one
+  // lane per neuron
+  int cap = resident_blocks<M, UNI, MODE>();
+  if (a.max_blocks > 0 && a.max_blocks < cap) cap = a.max_blocks;
+  const int grid = persistent_grid(a.n, cap);
+  hipLaunchKernelGGL((rk5_persistent_kernel<M, UNI, MODE>), dim3(grid), dim3(kRK5Block), 0, s, a);
+}
+
 template <class M>
 static void launch_rk5_t(const RK5Args& a, hipStream_t s) {
   const int block = kRK5Block;
   if (a.persistent) {
-    // persistent grid: enough waves to fill the GPU, at most one lane per neuron
-    const int grid = persistent_grid(a.n, a.max_blocks);
     const uint64_t rest = a.tab.per_neuron[0] & ~(1ull << M::i_ref);
     bool uni = rest == 0 && a.tab.per_neuron[1] == 0;
     for (int j = 0; j < M::NPAR && uni; ++j)
       if (M::divisor(j) && recip_rn(a.tab.val[j]) == 0.0f) uni = false;
     if constexpr (PredCapable<M>::value) {
       if (uni && a.pred_mode == 1) {
-        hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 1>), dim3(grid), dim3(block), 0, s, a);
+        launch_persistent<M, true, 1>(a, s);
         return;
       }
       if (uni && a.pred_mode == 2) {
-        hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 2>), dim3(grid), dim3(block), 0, s, a);
+        launch_persistent<M, true, 2>(a, s);
         return;
       }
     }
     if (uni)
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, true, 0>), dim3(grid), dim3(block), 0, s, a);
+      launch_persistent<M, true, 0>(a, s);
     else
-      hipLaunchKernelGGL((rk5_persistent_kernel<M, false, 0>), dim3(grid), dim3(block), 0, s, a);
+      launch_persistent<M, false, 0>(a, s);
     return;
   }
   const int grid = (a.n + block - 1) / block;
