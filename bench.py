@@ -54,6 +54,8 @@ DELIVER_BYTES_PER_EVENT = 4 + 4 + 2 + 16
 # fold per (neuron, port) channel = ring slot read + clear (16 B) + input
 # variable read and write (8 B); Poisson edges (20 B each) counted separately
 FOLD_BYTES_PER_CHANNEL = 24
+# iaf_psc_exp update: 4 state floats read and written per neuron-step
+IAF_BYTES_PER_NEURON = 32
 POISSON_BYTES_PER_EDGE = 20
 
 
@@ -144,6 +146,13 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
                              "frac": a / PEAK_FP32_TFLOPS, "avg_launch_us": ms * 1e3 / cnt,
                              "flops_per_launch": flops / cnt,
                              "note": "FP32 vector peak (the RK5 arithmetic is FP32 VALU work)"}
+    ms, cnt = kt["iaf_update"]
+    if cnt:
+        b = n * IAF_BYTES_PER_NEURON * steps
+        a = b / (ms * 1e-3) / 1e9
+        out["iaf_update"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
+                             "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
+                             "bytes_per_launch": b / cnt}
     ms, cnt = kt["deliver"]
     if cnt:
         b = st["syn_events"] * DELIVER_BYTES_PER_EVENT
@@ -199,7 +208,8 @@ def main():
         print("bench.py: this build runs on one GPU only (multi-GPU spike exchange is out of "
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(0)
-    seed = args.seed if args.seed is not None else (12345 if args.workload == "scale_adex" else 200)
+    mc = args.workload == "microcircuit"
+    seed = args.seed if args.seed is not None else {"scale_adex": 12345, "microcircuit": 7}.get(args.workload, 200)
     pred_path = args.predictor
     if pred_path is None:
         p = os.path.join(ROOT, "nest-gpu_amd", "predictors", args.workload + ".json")
@@ -221,13 +231,16 @@ def main():
     import nestgpu as ngpu
     ngpu.SetVerbosityLevel(0)
     ngpu.SetStepPredictorMode("plain")
-    n = args.neurons
-    n_port = 2 if args.workload == "scale_adex" else 4
+    n = sum(workloads.MC_SIZES) if mc else args.neurons
+    n_port = 4 if args.workload == "predictor_adex" else 2
     n_poisson_edges = n
     t0 = time.perf_counter()
     neurons = workloads.WORKLOADS[args.workload](ngpu, n, seed)
+    n_syn = workloads.microcircuit.n_syn if mc else n * 1000
     log("built %s: %d neurons, %d synapses + %d Poisson edges in %.1f s"
-        % (args.workload, n, n * 1000, n, time.perf_counter() - t0))
+        % (args.workload, n, n_syn, n, time.perf_counter() - t0))
+    # no RK5 in the microcircuit: its timed windows bracket every launch
+    win_timing = 1 if mc else 2
     if predictor is not None:
         ngpu.ActivateRecSpikeTimes(neurons, 150)
     t0 = time.perf_counter()
@@ -237,7 +250,7 @@ def main():
     ngpu.Synchronize()
     log("warmup %d steps done" % args.warmup)
     t_plain0 = (args.warmup) * DT
-    wp = timed_window(ngpu, args.steps)
+    wp = timed_window(ngpu, args.steps, timing=win_timing)
     sp = window_summary(wp, args.steps, n)
     log("plain: %d steps, wall %.3f s, RTF %.4f, kernel ms %s" % (args.steps, wp["wall"], sp["rtf"], wp["kt"]))
 
@@ -294,11 +307,12 @@ def main():
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
     roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_edges).get(dom, roof_all.get(dom, {})))
     kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
-             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel"}.get(dom, dom)
+             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel",
+             "iaf_update": "iaf_kernel"}.get(dom, dom)
     roof["kernel"] = kname
     roof["traffic"] = None
     tpath = os.path.join(ROOT, "profiles", "%s_traffic.json" % args.workload)
-    if os.path.exists(tpath) and n == N_FULL:
+    if os.path.exists(tpath) and (n == N_FULL or mc):
         with open(tpath) as f:
             tr = json.load(f)
         if kname in tr["kernels"]:
@@ -324,11 +338,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%s: aeif_cond_beta_multisynapse %d neurons, %d ports, indegree 1000 "
-                               "(%.2g synapses), delays U{1..50}x0.1 ms, Poisson input, plain RK5 "
-                               "(eps 1e-6), 1 s bio per window" % (args.workload, n, n_port, n * 1000.0),
-                   "neurons": n, "synapses": n * 1000, "dt_ms": DT, "seed": seed,
-                   "method": "plain RK5 (value); learned step-size proposal in 'learned'"},
+        "config": {"workload": ("%s: iaf_psc_exp %d neurons in 8 populations, fixed_total_number per "
+                                "pair (p ~ U(0.01,0.15)), %.3g synapses, normal weights and delays, "
+                                "Poisson 8 Hz x K_ext, exact propagator; 1 s bio per window"
+                                % (args.workload, n, n_syn)) if mc else
+                               ("%s: aeif_cond_beta_multisynapse %d neurons, %d ports, indegree 1000 "
+                                "(%.2g synapses), delays U{1..50}x0.1 ms, Poisson input, plain RK5 "
+                                "(eps 1e-6), 1 s bio per window" % (args.workload, n, n_port, n_syn)),
+                   "neurons": n, "synapses": n_syn, "dt_ms": DT, "seed": seed,
+                   "method": "exact iaf_psc_exp update" if mc else
+                             "plain RK5 (value); learned step-size proposal in 'learned'"},
         "synaptic_events_per_s": sp["synaptic_events_per_s"],
         "neuron_updates_per_s": sp["neuron_updates_per_s"],
         "mean_rate_hz": sp["mean_rate_hz"],

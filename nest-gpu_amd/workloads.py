@@ -13,6 +13,17 @@ predictor_adex (the learned step-size benchmark): the same neuron with 4
   2, 150 inh -> port 1, 50 inh -> port 3; slow-port weights x0.2), and one
   Poisson generator per neuron with rate U(5,40) kHz at 0.2 nS on port 0
   (mean ~9 Hz, a mix of silent and fast-firing neurons).
+
+microcircuit (the delivery-bound iaf_psc_exp benchmark): 8 populations of
+  the cortical microcircuit's sizes (77,169 neurons; microcircuit neuron
+  parameters of python/Potjans_2014_s/network_params.py: E_L -65 mV, V_th
+  -50 mV, V_reset -65 mV, C_m 250 pF, tau_m 10 ms, tau_syn 0.5 ms, t_ref
+  2 ms); each population pair connected by fixed_total_number with
+  K = round(p N_pre N_post), p ~ U(0.01, 0.15) per pair (64 seeded draws);
+  weights N(87.8, 8.78) pA from excitatory populations, x -4 from
+  inhibitory ones (kept on their sign); delays N(1.5, 0.75) ms exc,
+  N(0.75, 0.375) ms inh, at least 0.1 ms; Poisson input 8 Hz x K_ext per
+  neuron (K_ext = 1600, 1500, 2100, 1900, 2000, 1900, 2900, 2100) at 87.8 pA.
 """
 import ctypes
 
@@ -92,4 +103,53 @@ def predictor_adex(ngpu, n=1_000_000, seed=200):
     return neurons
 
 
-WORKLOADS = {"scale_adex": scale_adex, "predictor_adex": predictor_adex}
+MC_SIZES = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]
+MC_K_EXT = [1600, 1500, 2100, 1900, 2000, 1900, 2900, 2100]
+MC_PARAMS = {"E_L": -65.0, "Theta_rel": 15.0, "V_reset_rel": 0.0, "C_m": 250.0, "tau_m": 10.0,
+             "tau_ex": 0.5, "tau_in": 0.5, "t_ref": 2.0}
+
+
+def microcircuit(ngpu, n=None, seed=7):
+    """Build the microcircuit-shaped network (n is ignored: the population
+    sizes are fixed); returns the NodeSeq of all its neurons."""
+    rng = np.random.default_rng(seed)
+    ngpu.SetRandomSeed(seed)
+    pops = [ngpu.Create("iaf_psc_exp", m, 2) for m in MC_SIZES]
+    for pop in pops:
+        ngpu.SetStatus(pop, MC_PARAMS)
+        v0 = rng.uniform(0.0, 15.0, pop.n).astype(np.float32)
+        ngpu.SetStatus(pop, "V_m_rel", {"array": [float(v) for v in v0]})
+    prob = rng.uniform(0.01, 0.15, (8, 8))
+    lib = ngpu._nestgpu
+    n_syn = 0
+    for j, pre in enumerate(pops):            # presynaptic population
+        exc = j % 2 == 0
+        for i, post in enumerate(pops):
+            k = int(round(prob[i, j] * pre.n * post.n))
+            if k == 0:
+                continue
+            if exc:
+                w = np.maximum(rng.normal(87.8, 8.78, k), 0.0).astype(np.float32)
+                d = np.maximum(rng.normal(1.5, 0.75, k), DT).astype(np.float32)
+            else:
+                w = np.minimum(rng.normal(-4 * 87.8, 4 * 8.78, k), 0.0).astype(np.float32)
+                d = np.maximum(rng.normal(0.75, 0.375, k), DT).astype(np.float32)
+            ngpu.ConnSpecInit()
+            ngpu.SynSpecInit()
+            ngpu.SetConnSpecParam("rule", 2)       # fixed_total_number
+            ngpu.SetConnSpecParam("total_num", k)
+            ngpu.SetSynSpecIntParam("receptor", 0 if exc else 1)
+            lib.NESTGPU_SetSynSpecFloatPtParam(b"weight_array", w.ctypes.data_as(ctypes.c_void_p))
+            lib.NESTGPU_SetSynSpecFloatPtParam(b"delay_array", d.ctypes.data_as(ctypes.c_void_p))
+            lib.NESTGPU_ConnectSeqSeq(pre.i0, pre.n, post.i0, post.n)
+            ngpu._check()
+            n_syn += k
+    for pop, kx in zip(pops, MC_K_EXT):
+        pg = ngpu.Create("poisson_generator")
+        ngpu.SetStatus(pg, "rate", 8.0 * kx)
+        ngpu.Connect(pg, pop, {"rule": "all_to_all"}, {"weight": 87.8, "delay": 1.5, "receptor": 0})
+    microcircuit.n_syn = n_syn
+    return ngpu.NodeSeq(pops[0].i0, sum(MC_SIZES))
+
+
+WORKLOADS = {"scale_adex": scale_adex, "predictor_adex": predictor_adex, "microcircuit": microcircuit}
