@@ -52,7 +52,10 @@ def rk5_flops_per_attempt(n_port):
 # (u32 channel + f32 weight + u16 delay) + FP64 accumulator read and write
 DELIVER_BYTES_PER_EVENT = 4 + 4 + 2 + 16
 # fold per (neuron, port) channel = ring slot read + clear (16 B) + input
-# variable read and write (8 B); Poisson edges (20 B each) counted separately
+# variable read and write (8 B); Poisson edges kept as per-channel CSR add
+# 20 B each, edges of a uniform run (all-to-all or one-to-one from
+# generators, as in every workload here) none: weight, delay and stream are
+# launch arguments and the generator's table entry stays in cache
 FOLD_BYTES_PER_CHANNEL = 24
 # iaf_psc_exp update: 4 state floats read and written per neuron-step
 IAF_BYTES_PER_NEURON = 32
@@ -162,8 +165,7 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
                           "bytes_per_launch": b / cnt}
     ms, cnt = kt["fold_poisson"]
     if cnt:
-        # one launch per step, or only the last step of a Simulate() call when
-        # the persistent RK5 launches do the fold themselves
+        # one launch per population with ports and step
         b = (n * n_port * FOLD_BYTES_PER_CHANNEL + n_poisson_edges * POISSON_BYTES_PER_EDGE) * cnt
         a = b / (ms * 1e-3) / 1e9
         out["fold_poisson"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
@@ -233,7 +235,8 @@ def main():
     ngpu.SetStepPredictorMode("plain")
     n = sum(workloads.MC_SIZES) if mc else args.neurons
     n_port = 4 if args.workload == "predictor_adex" else 2
-    n_poisson_edges = n
+    n_poisson_edges = n          # Poisson edges built (one per neuron)
+    n_poisson_csr_edges = 0      # of which kept as CSR (each workload's are one run per population)
     t0 = time.perf_counter()
     neurons = workloads.WORKLOADS[args.workload](ngpu, n, seed)
     n_syn = workloads.microcircuit.n_syn if mc else n * 1000
@@ -299,13 +302,13 @@ def main():
     nb = min(args.steps, 2000)
     wb = timed_window(ngpu, nb, timing=1)
     log("breakdown: %d steps, wall %.3f s, kernel ms %s" % (nb, wb["wall"], wb["kt"]))
-    roof_all = rooflines(wb, n, n_port, nb, n_poisson_edges)
+    roof_all = rooflines(wb, n, n_port, nb, n_poisson_csr_edges)
     breakdown = {k: v[0] * 1e3 / nb for k, v in wb["kt"].items()}
     # the dominant kernel's roofline from the plain timed window (its RK5
     # launches were bracketed by events there); HBM traffic per launch from
     # the committed FETCH_SIZE/WRITE_SIZE summary of this workload, if any
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
-    roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_edges).get(dom, roof_all.get(dom, {})))
+    roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges).get(dom, roof_all.get(dom, {})))
     kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
              "deliver": "deliver_kernel", "fold_poisson": "fold_kernel",
              "iaf_update": "iaf_kernel"}.get(dom, dom)

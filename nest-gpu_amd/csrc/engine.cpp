@@ -1425,6 +1425,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_HOT_MARGIN")) rk5_hot_margin_ = (float)atof(env);
   if (const char* env = getenv("NGPU_RK5_HOT_ONLY")) rk5_hot_only_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
     rk5_defer_env_ = true;
@@ -1465,9 +1466,36 @@ int Engine::Calibrate() {
     long long lo = g->ch_base, hi = g->ch_base + (long long)g->n * g->n_port;
     auto b = std::lower_bound(edges.begin(), edges.end(), lo, [](const Edge& e, long long v) { return e.ch < v; });
     auto e = std::lower_bound(edges.begin(), edges.end(), hi, [](const Edge& e, long long v) { return e.ch < v; });
-    if (b == e) { g->has_pe = false; continue; }
+    g->has_pe = false;
+    g->pe_run = false;
+    if (b == e) continue;
     g->has_pe = true;
     size_t ne = e - b;
+    // one uniform run: consecutive channels with one edge each, one weight
+    // and delay, generator constant or stepping by one, stream stepping by one
+    // (an all-to-all or one-to-one connection from generators)
+    if (poisson_runs_) {
+      const Edge& f = *b;
+      const int gstep = ne > 1 ? (b + 1)->gen - f.gen : 0;
+      bool run = gstep == 0 || gstep == 1;
+      for (size_t k = 0; run && k < ne; ++k) {
+        const Edge& x = *(b + k);
+        run = x.ch == f.ch + (long long)k && x.gen == f.gen + gstep * (int)k && x.w == f.w &&
+              x.delay == f.delay && x.stream == f.stream + (unsigned int)k;
+      }
+      if (run) {
+        g->has_pe = false;
+        g->pe_run = true;
+        g->pu_b = f.ch - lo;
+        g->pu_e = f.ch - lo + (long long)ne;
+        g->pu_gen0 = f.gen;
+        g->pu_gstep = gstep;
+        g->pu_w = f.w;
+        g->pu_delay = f.delay;
+        g->pu_stream0 = f.stream;
+        continue;
+      }
+    }
     std::vector<int> off((size_t)g->n * g->n_port + 1, 0), gen(ne);
     std::vector<float> w(ne), dl(ne);
     std::vector<unsigned int> st(ne);
@@ -1592,7 +1620,13 @@ FoldArgs Engine::make_fold_args(const Group& g) {
   fa.ring = d_ring_.p; fa.n_ch = n_ch_;
   if (g.has_pe) {
     fa.pe_off = g.pe_off.p; fa.pe_gen = g.pe_gen.p; fa.pe_w = g.pe_w.p; fa.pe_delay = g.pe_delay.p;
-    fa.pe_stream = g.pe_stream.p; fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
+    fa.pe_stream = g.pe_stream.p;
+  } else if (g.pe_run) {
+    fa.pu_b = g.pu_b; fa.pu_e = g.pu_e; fa.pu_gen0 = g.pu_gen0; fa.pu_gstep = g.pu_gstep;
+    fa.pu_w = g.pu_w; fa.pu_delay = g.pu_delay; fa.pu_stream0 = g.pu_stream0;
+  }
+  if (g.has_pe || g.pe_run) {
+    fa.gen_chunks = d_gen_chunks_.p; fa.gen_exp = d_gen_exp_.p;
     fa.gen_origin = d_gen_origin_.p; fa.gen_start = d_gen_start_.p; fa.gen_stop = d_gen_stop_.p;
     fa.seed = kernel_seed_;
   }
@@ -1642,7 +1676,7 @@ void Engine::launch_step() {
           // (state is complete between calls)
           // learned proposals shorten the spike upswing's attempt chain, so
           // the split point that balances two launches comes earlier
-          const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 16 : rk5_defer_cap_);
+          const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 14 : rk5_defer_cap_);
           a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
         }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });

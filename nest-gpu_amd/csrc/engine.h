@@ -86,6 +86,12 @@ struct Group {
   DBuf<float> pe_w, pe_delay;
   DBuf<unsigned int> pe_stream;
   bool has_pe = false;
+  // or, when every edge fits one uniform run (see FoldArgs), that run
+  bool pe_run = false;
+  long long pu_b = 0, pu_e = 0;
+  int pu_gen0 = 0, pu_gstep = 0;
+  float pu_w = 0.0f, pu_delay = 0.0f;
+  unsigned int pu_stream0 = 0;
   // poisson generator: index of the first generator in the global tables
   int gen_base = 0;
 };
@@ -290,6 +296,7 @@ class Engine {
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
   float rk5_hot_margin_ = 2.5f;        // hot list from V_th + margin * Delta_T (AdEx models)
   bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
+  bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]

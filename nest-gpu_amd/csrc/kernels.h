@@ -43,7 +43,14 @@ struct FoldArgs {
   double* ring;
   long long n_ch;
   StepCtx ctx;           // this step's time (by value)
-  // poisson direct connections into this population, CSR by local channel
+  // poisson direct connections into this population, either one uniform
+  // run (one edge per channel of [pu_b, pu_e), same weight and delay,
+  // generator pu_gen0 + pu_gstep * r and stream pu_stream0 + r for the
+  // channel r places into the run; pu_e > pu_b) or CSR by local channel
+  long long pu_b, pu_e;
+  int pu_gen0, pu_gstep;
+  float pu_w, pu_delay;
+  unsigned int pu_stream0;
   const int* pe_off;     // [n*n_port+1] or null
   const int* pe_gen;     // generator index
   const float* pe_w;

@@ -245,19 +245,34 @@ __device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
   }
 }
 
-// Input of channel idx (= port * n + local neuron) for the step with time
-// index t_idx ending at t1: the ring slot (cleared for reuse) plus the
-// Poisson draws of the generators connected to it, src/get_spike.cu:142-166
-// and src/poiss_gen.cu:55-86.  Poisson edges are sorted by channel, so each
-// lane sums its own edges (no atomics).  Contraction is off: the double
+// Poisson input of channel idx (= port * n + local neuron) for the step with
+// time index t_idx ending at t1, added to acc (the channel's ring value) edge
+// by edge, src/poiss_gen.cu:55-86.  Poisson edges are sorted by channel, so
+// each lane sums its own edges (no atomics).  Contraction is off: the double
 // expressions round exactly as the reference's.
-__device__ inline double channel_input(const FoldArgs& f, long long idx, int slot, double t1,
-                                       long long t_idx) {
+__device__ inline double poisson_input(const FoldArgs& f, long long idx, double acc, double t1,
+                                       long long t_idx, bool run_here) {
 #pragma clang fp contract(off)
-  double* rp = f.ring + (size_t)slot * f.n_ch + (size_t)(f.ch_base + idx);
-  double acc = *rp;
-  *rp = 0.0;
-  if (f.pe_off) {
+  if (f.pu_e > f.pu_b) {
+    if (!run_here) return acc;
+    // uniform run: nothing to look up but the generator's own table, read
+    // by every lane (a lane outside the run reads the first generator's) so
+    // the loads issue together with the caller's, without a branch between
+    const bool in_run = idx >= f.pu_b && idx < f.pu_e;
+    const int r = in_run ? (int)(idx - f.pu_b) : 0;
+    const int g = f.pu_gen0 + f.pu_gstep * r;
+    const float origin = f.gen_origin[g], start = f.gen_start[g], stop = f.gen_stop[g];
+    const int chunks = f.gen_chunks[g];
+    const double e_chunk = f.gen_exp[g];
+    // one wait for all five (the compiler would otherwise sink each load to
+    // its first use and wait for them one by one)
+    asm volatile("" ::"v"(origin), "v"(start), "v"(stop), "v"(chunks), "v"(e_chunk));
+    const double t_rel = t1 - origin - f.pu_delay;
+    if (in_run & (t_rel >= start) & (t_rel <= stop)) {
+      const int cnt = poisson_draw(f.seed, f.pu_stream0 + (unsigned int)r, t_idx, chunks, e_chunk);
+      if (cnt > 0) acc += (double)(f.pu_w * (float)cnt);
+    }
+  } else if (f.pe_off) {
     const int b = f.pe_off[idx], e = f.pe_off[idx + 1];
     for (int k = b; k < e; ++k) {
       const int g = f.pe_gen[k];
@@ -1100,34 +1115,70 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 // (src/poiss_gen.cu:55-86): for each (port, neuron) channel,
 //   acc   = ring[slot][ch] + sum_e (double)(w_e * Poisson(lambda_e))
 //   input = (float)((double)input + acc * port_weight)
-// and the ring slot is cleared for reuse.  Poisson edges are sorted by target
+// and the ring slot is cleared for reuse.  One lane per neuron handles its
+// NP ports, so the port and its weight are uniform over the wave (scalar
+// loads) and the split flag is read once.  Poisson edges are sorted by target
 // channel, so each lane sums its own edges (no atomics).  Contraction is off
 // so the double expression rounds exactly as the reference's.
+// split flag read by populations without split steps
+__device__ const unsigned char kNoLag = 0;
+
+template <int NP>
 __global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = (long long)a.n * a.n_port;
-  if (idx >= total) return;
-  const int port = (int)(idx / a.n);
-  const int i = (int)(idx - (long long)port * a.n);
-  const double acc = channel_input(a, idx, a.ctx.slot, a.ctx.t1, a.ctx.t_idx);
-  if (a.lag && a.lag[i]) {   // still inside this step: its RK5 launch adds the input later
-    a.pend[idx] = acc;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  // every load that depends on nothing else is issued first (ring slots,
+  // split flag, input variables), and the slots are cleared last, so one
+  // wait covers them all; an input is read whatever acc turns out to be: a
+  // 128-byte line of it is almost always needed by some lane of the wave
+  const bool lagged = *(a.lag ? a.lag + i : &kNoLag) != 0;
+  double* rp[NP];
+  double acc[NP];
+  float* in[NP];
+  float in0[NP], w[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    rp[p] = a.ring + (size_t)a.ctx.slot * a.n_ch + (size_t)(a.ch_base + (long long)p * a.n + i);
+    acc[p] = *rp[p];
+    in[p] = a.var + (size_t)(a.in_var + p * a.in_port_stride) * a.n + i;
+    in0[p] = *in[p];
+    w[p] = 1.0f;
+    if (a.w_param >= 0) {
+      const int j = a.w_param + p * a.w_port_stride;
+      w[p] = a.tab.is_per_neuron(j) ? a.param[(size_t)j * a.n + i] : a.tab.val[j];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const long long lo = (long long)p * a.n;
+    acc[p] = poisson_input(a, lo + i, acc[p], a.ctx.t1, a.ctx.t_idx,
+                           a.pu_b < lo + a.n && a.pu_e > lo);
+    *rp[p] = 0.0;
+  }
+  if (lagged) {   // still inside this step: its RK5 launch adds the input later
+#pragma unroll
+    for (int p = 0; p < NP; ++p) a.pend[(size_t)p * a.n + i] = acc[p];
     return;
   }
-  if (acc == 0.0) return;   // input + 0*w == input exactly
-  float w = 1.0f;
-  if (a.w_param >= 0) {
-    const int j = a.w_param + port * a.w_port_stride;
-    w = a.tab.is_per_neuron(j) ? a.param[(size_t)j * a.n + i] : a.tab.val[j];
-  }
-  float* in = a.var + (size_t)(a.in_var + port * a.in_port_stride) * a.n + i;
-  *in = fold_add(*in, acc, w);
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+    if (acc[p] != 0.0) *in[p] = fold_add(in0[p], acc[p], w[p]);   // input + 0*w == input exactly
 }
 
 void launch_fold(const FoldArgs& a, hipStream_t s) {
-  const long long total = (long long)a.n * a.n_port;
-  if (total <= 0) return;
-  hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+  if (a.n <= 0 || a.n_port <= 0) return;
+  const dim3 grid((unsigned)((a.n + 255) / 256)), block(256);
+  switch (a.n_port) {
+    case 1: hipLaunchKernelGGL(fold_kernel<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(fold_kernel<2>, grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(fold_kernel<3>, grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(fold_kernel<4>, grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(fold_kernel<5>, grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL(fold_kernel<6>, grid, block, 0, s, a); break;
+    case 7: hipLaunchKernelGGL(fold_kernel<7>, grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL(fold_kernel<8>, grid, block, 0, s, a); break;
+    default: throw std::runtime_error("fold: more than " + std::to_string(kMaxPort) + " ports");
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -64,14 +64,26 @@ __device__ inline pred_s4 pred_pack4(unsigned int lo, unsigned int hi) {
   v[3] = (short)(hi >> 16);
   return v;
 }
+// two floats rounded to bfloat16 (nearest even, as bf16_bits) and packed,
+// one v_cvt_pk_bf16_f32 on gfx950
+typedef __bf16 pred_bf2 __attribute__((ext_vector_type(2)));
 __device__ inline unsigned int pred_pair(float a, float b) {
-  return bf16_bits(a) | (bf16_bits(b) << 16);
+  pred_bf2 v;
+  v[0] = (__bf16)a;
+  v[1] = (__bf16)b;
+  return __builtin_bit_cast(unsigned int, v);
 }
 
 // Output log2(h/dt) for the neuron held by this lane.  Every lane of the
 // wave must call it together (the matrix instructions read all 64 lanes);
 // lanes without a neuron pass any finite inputs and ignore the result.
-__device__ inline float pred_infer_wave(const PredLDS& s, const float* raw) {
+__device__ inline float pred_infer_wave(const PredLDS& s_in, const float* raw) {
+  // the parameters are read from LDS at every call: an opaque zero offset
+  // keeps the compiler from hoisting ~70 loop-invariant values out of a
+  // caller's loop into registers for the whole kernel
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  const PredLDS& s = *(&s_in + z);
   const int lane = threadIdx.x & 63;
   const int half = lane >> 5, col = lane & 31;
   float x[kPI];

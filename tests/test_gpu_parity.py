@@ -184,9 +184,12 @@ def _gpu_network(ngpu, net, model, sim_ms, poiss_w=1.0, status=None):
     return t_idx[o], node[o]
 
 
-def test_iaf_psc_exp_network_bitexact(ngpu):
+@pytest.mark.parametrize("runs", ["1", "0"])
+def test_iaf_psc_exp_network_bitexact(ngpu, runs, monkeypatch):
     # iaf_psc_exp + Poisson + delivery: the same float operations in the same
-    # order on both sides, so spike rasters must be identical
+    # order on both sides, so spike rasters must be identical; the shared
+    # generator's edges as one uniform run (runs=1) and as per-channel CSR
+    monkeypatch.setenv("NGPU_POISSON_RUNS", runs)
     net = netbuild.c1_network(seed=3, n=600, k=60, t_rate=8000.0)
     net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2] * 60.0, net["exc"][3])
     net["inh"] = (net["inh"][0], net["inh"][1], net["inh"][2] * -60.0, net["inh"][3])
@@ -196,6 +199,38 @@ def test_iaf_psc_exp_network_bitexact(ngpu):
     assert len(to) > 500
     np.testing.assert_array_equal(tg, to)
     np.testing.assert_array_equal(ng, no)
+
+
+def _iaf_one_to_one_poisson(ngpu, net, rates):
+    nrn = ngpu.Create("iaf_psc_exp", net["n"], 2)
+    pg = ngpu.Create("poisson_generator", net["n"])
+    ngpu.SetStatus(pg, {"rate": {"array": [float(r) for r in rates]}})
+    ngpu.Connect(pg, nrn, {"rule": "one_to_one"}, {"weight": 30.0, "delay": 0.1, "receptor": 0})
+    se, te, we, de = net["exc"]
+    si, ti, wi, di = net["inh"]
+    netbuild.connect_edges_fast(ngpu, se + nrn[0], te + nrn[0], 0, we * 60.0, de)
+    netbuild.connect_edges_fast(ngpu, si + nrn[0], ti + nrn[0], 1, wi * -60.0, di)
+    ngpu.ActivateRecSpikeTimes(nrn, 5000)
+    ngpu.Simulate(200.0)
+    return [list(ts) for ts in ngpu.GetRecSpikeTimes(nrn)]
+
+
+def test_poisson_one_to_one_run_matches_csr(ngpu, monkeypatch):
+    # one generator per neuron (generator index stepping with the channel):
+    # the uniform-run fold and the per-channel CSR fold must give the same
+    # raster bit for bit (the CSR path is pinned to the oracle above)
+    net = netbuild.c1_network(seed=5, n=400, k=40, t_rate=8000.0)
+    rates = np.random.default_rng(9).uniform(4000.0, 12000.0, net["n"])
+    out = {}
+    for runs in ("1", "0"):
+        monkeypatch.setenv("NGPU_POISSON_RUNS", runs)
+        ngpu.ResetKernel()
+        ngpu.SetVerbosityLevel(0)
+        out[runs] = _iaf_one_to_one_poisson(ngpu, net, rates)
+    n_spk = sum(len(t) for t in out["1"])
+    print("one-to-one poisson spikes", n_spk)
+    assert n_spk > 100
+    assert out["1"] == out["0"]
 
 
 def test_aeif_network_config1(ngpu):
