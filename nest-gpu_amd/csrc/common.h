@@ -27,6 +27,9 @@ constexpr int kWave = 64;
 // leaves the rest unlisted, which only delays them)
 constexpr int kRK5Block = 256;
 constexpr int kHotLds = 128;
+// persistent RK5 kernel: LDS entries per wave for its spikes of one launch
+// (a wave with more adds the rest to the spike list one atomic each)
+constexpr int kSpkLds = 64;
 
 // Time of the step being computed, kept in device memory so the step's
 // kernels can be replayed (HIP graphs) without host arguments.

@@ -223,19 +223,14 @@ template <int NP> struct PredCapable<AeifCondBeta<NP>> : std::true_type {};
 // ---------------------------------------------------------------------------
 // Spike emission: append to this step's spike list and do the per-node
 // bookkeeping of src/spike_buffer.cu:162-217.
-__device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
-                                  int parity, int node, int rel, float height,
-                                  long long t_idx, double t1, bool lag = false) {
-  int pos = atomicAdd(&sl.count[parity], 1);
-  if (pos < sl.cap) {
-    sl.src[(size_t)parity * sl.cap + pos] = lag ? (int)((unsigned int)node | kSpikeLagBit) : node;
-    sl.height[(size_t)parity * sl.cap + pos] = height;
-  } else {
-    atomicAdd(sl.overflow, 1);
-  }
+// Spike bookkeeping of the emitting node: plain stores and a counter add
+// whose result is not used, so nothing here waits for memory (the
+// spike-time recorder's read of its fill count aside).
+__device__ inline void spike_book(const SpikeBook& sb, int node, int rel, float height,
+                                  long long t_idx, double t1) {
   sb.last_idx[node] = t_idx;
   sb.last_height[node] = height;
-  if (sb.spike_count) sb.spike_count[rel]++;
+  if (sb.spike_count) atomicAdd(&sb.spike_count[rel], 1);
   if (sb.max_rec > 0) {
     int k = sb.n_rec[rel];
     if (k < sb.max_rec - 1) {
@@ -243,6 +238,24 @@ __device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
       sb.n_rec[rel] = k + 1;
     }
   }
+}
+
+// Appends one source (node | kSpikeLagBit for a lagged spike) to the spike list.
+__device__ inline void spike_list_add(const SpikeList& sl, int parity, int src, float height) {
+  int pos = atomicAdd(&sl.count[parity], 1);
+  if (pos < sl.cap) {
+    sl.src[(size_t)parity * sl.cap + pos] = src;
+    sl.height[(size_t)parity * sl.cap + pos] = height;
+  } else {
+    atomicAdd(sl.overflow, 1);
+  }
+}
+
+__device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
+                                  int parity, int node, int rel, float height,
+                                  long long t_idx, double t1, bool lag = false) {
+  spike_list_add(sl, parity, lag ? (int)((unsigned int)node | kSpikeLagBit) : node, height);
+  spike_book(sb, node, rel, height, t_idx, t1);
 }
 
 // Poisson input of channel idx (= port * n + local neuron) for the step with
@@ -498,8 +511,12 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // that of rk5_kernel.  UNI: every parameter except refractory_step is
 // uniform over the population, so parameters come from the kernel-argument
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
+// The learned-proposal kernel of a small model is held to 168 VGPRs (3 waves
+// per SIMD); the compiler would otherwise take a few more and drop to 2.
 template <class M, bool UNI, int MODE>
-__global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
+__global__ void __launch_bounds__(kRK5Block)
+__attribute__((amdgpu_waves_per_eu(MODE == 2 && M::NV <= 6 ? 3 : 1)))
+rk5_persistent_kernel(RK5Args a) {
   static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * (blockDim.x / kWave) +
@@ -557,6 +574,13 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
   const long long end = (long long)n_hw + n_bw;
   // the waves holding the chains issue first; the others fill the free slots
   if (n_hw > 0) __builtin_amdgcn_s_setprio(3);
+  // spikes of the wave, appended to the step's list with one atomic when the
+  // wave exits (a returning atomic per spike would stall the long chains)
+  __shared__ int s_spk[kRK5Block / kWave][kSpkLds];
+  int* my_spk = s_spk[threadIdx.x / kWave];
+  int scnt = 0;         // spikes of this wave (wave-uniform)
+  bool spk = false;     // the lane's neuron spiked in this iteration
+  int spk_src = 0;
   int hcnt = 0;         // entries of this wave's hot list for the next step
   int pend = 0;         // bit 0: neuron finished this iteration, bit 1: it is hot
   int pend_i = 0;
@@ -566,6 +590,9 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
   double x = 0.0;
   double cx1 = 0.0;     // end of the step the lane's neuron is in (x1, or x1_prev if lagged)
   bool lagged = false;
+  // a lagged neuron's stashed input, loaded when the lane takes it, so
+  // resuming a split step needs no memory round trip inside the chain
+  double pend_v[M::NPORT];
   float h = 0.0f, ref_in = 0.0f;
   int i = -1;
   int att_i = 0;   // attempts of the current neuron in this step
@@ -621,9 +648,32 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
           h = a.h[i];
           if constexpr (MODE != 0) el = a.err_last[i];
           lagged = a.lag ? a.lag[i] != 0 : false;
+          if (lagged) {
+#pragma unroll
+            for (int k = 0; k < M::NPORT; ++k) pend_v[k] = a.pend[(size_t)k * n + i];
+          }
           cx1 = lagged ? x1_prev : x1;
           need_k1 = true;
           att_i = 0;
+          // the loads complete here: left pending, the compiler waits for them
+          // where the paths join (the k1 evaluation of every later
+          // iteration), and that wait, counting stores too, would hold the
+          // chain on the stores of finished neurons
+#pragma unroll
+          for (int j = 0; j < M::NV; ++j) asm volatile("" ::"v"(y[j]));
+          if (UNI) {
+            asm volatile("" ::"v"(p[M::i_ref]));
+          } else {
+#pragma unroll
+            for (int j = 0; j < M::NPAR; ++j)
+              if (a.tab.is_per_neuron(j)) asm volatile("" ::"v"(p[j]));
+          }
+          asm volatile("" ::"v"(x), "v"(h));
+          if constexpr (MODE != 0) asm volatile("" ::"v"(el));
+          if (lagged) {
+#pragma unroll
+            for (int k = 0; k < M::NPORT; ++k) asm volatile("" ::"v"(pend_v[k]));
+          }
         }
       }
       next += __popcll(m);
@@ -723,8 +773,9 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
           y[M::iV] = p[M::i_reset];
           if (end_step) p[M::i_ref] -= 1.0f;
         } else if (y[M::iV] >= p[M::i_thr]) {
-          emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, lagged ? t_idx - 1 : t_idx, cx1,
-                     lagged);
+          spike_book(a.sb, a.i0 + i, i, 1.0f, lagged ? t_idx - 1 : t_idx, cx1);
+          spk = true;
+          spk_src = lagged ? (int)((unsigned int)(a.i0 + i) | kSpikeLagBit) : a.i0 + i;
           y[M::iV] = p[M::i_reset];
           y[M::iw] += p[M::i_jump];
           const float rr = (float)(int)roundf(p[M::i_tref] / a.dt);
@@ -735,7 +786,7 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
           // step's fold launch, same arithmetic), then go on with this step
 #pragma unroll
           for (int k = 0; k < M::NPORT; ++k) {
-            const double acc = a.pend[(size_t)k * n + i];
+            const double acc = pend_v[k];
             if (acc != 0.0) {
               const int wj = M::in_w(k);
               const float w = wj >= 0 ? p[wj < 0 ? 0 : wj] : 1.0f;
@@ -765,6 +816,18 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
         }
       }
     }
+    // spikes of this iteration into the wave's buffer (the list itself when
+    // the buffer is full)
+    {
+      const unsigned long long ms = __ballot(spk);
+      if (spk) {
+        const int pos = scnt + __popcll(ms & ((1ull << lane) - 1ull));
+        if (pos < kSpkLds) my_spk[pos] = spk_src;
+        else spike_list_add(a.sl, parity, spk_src, 1.0f);
+        spk = false;
+      }
+      scnt += __popcll(ms);
+    }
     // neurons that finished the step: next step's hot list and flags
     if (a.hot_flag) {
       const unsigned long long mh = __ballot(pend == 3);
@@ -781,6 +844,25 @@ __global__ void __launch_bounds__(kRK5Block) rk5_persistent_kernel(RK5Args a) {
         pend = 0;
       }
       hcnt += __popcll(mh);
+    }
+  }
+  {
+    // the wave's spikes into the step's list (order within a list does not
+    // matter: delivery sums exactly)
+    const int listed = scnt < kSpkLds ? scnt : kSpkLds;
+    if (listed > 0) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(a.sl.count + parity, listed);
+      base = __shfl(base, 0, kWave);
+      for (int e = lane; e < listed; e += kWave) {
+        const int pos = base + e;
+        if (pos < a.sl.cap) {
+          a.sl.src[(size_t)parity * a.sl.cap + pos] = my_spk[e];
+          a.sl.height[(size_t)parity * a.sl.cap + pos] = 1.0f;
+        } else {
+          atomicAdd(a.sl.overflow, 1);
+        }
+      }
     }
   }
   if (a.hot_flag) {

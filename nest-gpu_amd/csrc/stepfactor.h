@@ -39,7 +39,10 @@ NGPU_HD inline float pow_m025(float x) {
   const double m = split_binade(x, &e);
   int q = e / 4, r = e % 4;
   if (r < 0) { r += 4; q -= 1; }
-  const double c4[4] = {1.0, 0.8408964152537145, 0.7071067811865476, 0.5946035575013605};
+  // 2^(-r/4) by selects, not a table: a table read would be a memory load
+  // (and wait) inside the RK5 loop
+  const double c4 = r == 0 ? 1.0 : r == 1 ? 0.8408964152537145 : r == 2 ? 0.7071067811865476
+                                                                          : 0.5946035575013605;
   double z = 1.2852 + m * (-0.35326 + m * 0.06590);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -47,7 +50,7 @@ NGPU_HD inline float pow_m025(float x) {
     const double z4 = z2 * z2;
     z = z * ((5.0 - m * z4) * 0.25);
   }
-  return (float)ldexp(z * c4[r], -q);
+  return (float)ldexp(z * c4, -q);
 }
 
 // x^(-1/5) for x >= 0, the same construction: x = m 2^(5q+r), a quadratic
@@ -61,8 +64,11 @@ NGPU_HD inline float pow_m02(float x) {
   const double m = split_binade(x, &e);
   int q = e / 5, r = e % 5;
   if (r < 0) { r += 5; q -= 1; }
-  const double c5[5] = {1.0, 0.8705505632961241, 0.757858283255199, 0.6597539553864471,
-                        0.5743491774985174};
+  const double c5 = r == 0   ? 1.0
+                    : r == 1 ? 0.8705505632961241
+                    : r == 2 ? 0.757858283255199
+                    : r == 3 ? 0.6597539553864471
+                             : 0.5743491774985174;
   double z = 1.2284 + m * (-0.28173 + m * 0.05166);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -71,7 +77,7 @@ NGPU_HD inline float pow_m02(float x) {
     const double z5 = z4 * z;
     z = z * ((6.0 - m * z5) * 0.2);
   }
-  return (float)ldexp(z * c5[r], -q);
+  return (float)ldexp(z * c5, -q);
 }
 
 // RN(1/d) for a divisor usable by div_rn (normal, |d| in [2^-32, 2^32]),
