@@ -310,7 +310,7 @@ def main():
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
     roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges).get(dom, roof_all.get(dom, {})))
     kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
-             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel",
+             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel<%d>" % n_port,
              "iaf_update": "iaf_kernel"}.get(dom, dom)
     roof["kernel"] = kname
     roof["traffic"] = None
