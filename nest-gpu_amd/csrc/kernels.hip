@@ -224,18 +224,20 @@ template <int NP> struct PredCapable<AeifCondBeta<NP>> : std::true_type {};
 // Spike emission: append to this step's spike list and do the per-node
 // bookkeeping of src/spike_buffer.cu:162-217.
 // Spike bookkeeping of the emitting node: plain stores and a counter add
-// whose result is not used, so nothing here waits for memory (the
-// spike-time recorder's read of its fill count aside).
+// whose result is not used, so nothing here waits for memory.  rec_k: the
+// node's spike-time fill count, held by the caller (read when it took the
+// node); null: read here.
 __device__ inline void spike_book(const SpikeBook& sb, int node, int rel, float height,
-                                  long long t_idx, double t1) {
+                                  long long t_idx, double t1, int* rec_k = nullptr) {
   sb.last_idx[node] = t_idx;
   sb.last_height[node] = height;
   if (sb.spike_count) atomicAdd(&sb.spike_count[rel], 1);
   if (sb.max_rec > 0) {
-    int k = sb.n_rec[rel];
+    const int k = rec_k ? *rec_k : sb.n_rec[rel];
     if (k < sb.max_rec - 1) {
       sb.rec_times[(size_t)rel * sb.max_rec + k] = (float)t1;
       sb.n_rec[rel] = k + 1;
+      if (rec_k) *rec_k = k + 1;
     }
   }
 }
@@ -593,6 +595,7 @@ rk5_persistent_kernel(RK5Args a) {
   // a lagged neuron's stashed input, loaded when the lane takes it, so
   // resuming a split step needs no memory round trip inside the chain
   double pend_v[M::NPORT];
+  int rec_k = 0;        // spike-time fill count of the lane's neuron (recording on)
   float h = 0.0f, ref_in = 0.0f;
   int i = -1;
   int att_i = 0;   // attempts of the current neuron in this step
@@ -652,6 +655,7 @@ rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
             for (int k = 0; k < M::NPORT; ++k) pend_v[k] = a.pend[(size_t)k * n + i];
           }
+          if (a.sb.max_rec > 0) rec_k = a.sb.n_rec[i];
           cx1 = lagged ? x1_prev : x1;
           need_k1 = true;
           att_i = 0;
@@ -674,6 +678,7 @@ rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
             for (int k = 0; k < M::NPORT; ++k) asm volatile("" ::"v"(pend_v[k]));
           }
+          if (a.sb.max_rec > 0) asm volatile("" ::"v"(rec_k));
         }
       }
       next += __popcll(m);
@@ -773,7 +778,7 @@ rk5_persistent_kernel(RK5Args a) {
           y[M::iV] = p[M::i_reset];
           if (end_step) p[M::i_ref] -= 1.0f;
         } else if (y[M::iV] >= p[M::i_thr]) {
-          spike_book(a.sb, a.i0 + i, i, 1.0f, lagged ? t_idx - 1 : t_idx, cx1);
+          spike_book(a.sb, a.i0 + i, i, 1.0f, lagged ? t_idx - 1 : t_idx, cx1, &rec_k);
           spk = true;
           spk_src = lagged ? (int)((unsigned int)(a.i0 + i) | kSpikeLagBit) : a.i0 + i;
           y[M::iV] = p[M::i_reset];
