@@ -1426,6 +1426,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_RK5_HOT_MARGIN")) rk5_hot_margin_ = (float)atof(env);
   if (const char* env = getenv("NGPU_RK5_HOT_ONLY")) rk5_hot_only_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
     rk5_defer_env_ = true;
@@ -1667,6 +1668,7 @@ void Engine::launch_step() {
           a.hot_count = g->hot_count.p;
           a.hot_margin = rk5_hot_margin_;
           a.hot_only = rk5_hot_only_ ? 1 : 0;
+          a.hot_lanes = rk5_hot_lanes_;
         }
         if (g->lag.p) {
           a.lag = g->lag.p;

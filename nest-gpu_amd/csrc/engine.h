@@ -297,6 +297,7 @@ class Engine {
   float rk5_hot_margin_ = 3.5f;        // hot list from V_th + margin * Delta_T (AdEx models)
   bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
   bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
+  int rk5_hot_lanes_ = 64;             // hot entries per wave
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]

@@ -103,6 +103,7 @@ struct RK5Args {
   int* hot_count;                  // [3]
   float hot_margin;                // AdEx models: listed from V >= V_th + hot_margin * Delta_T
   int hot_only;                    // waves holding hot entries take no bulk share
+  int hot_lanes;                   // hot entries per wave (1..64) while they fit the grid
   // split steps: a neuron still integrating after defer_cap attempts of a
   // step (at a substep boundary) is stored with lag = 1, finishes that step
   // first in the next launch, then adds that step's input (stashed in pend

@@ -556,8 +556,9 @@ rk5_persistent_kernel(RK5Args a) {
   // wave w takes hot entries [w*hp, w*hp + hp) (one per lane, so the long
   // attempt chains start together in the first iteration, packed into few
   // waves), then its equal share of all neurons
-  const int hp = n_hot <= n_waves * kWave
-                     ? kWave
+  const int hl = a.hot_lanes < 1 ? 1 : (a.hot_lanes > kWave ? kWave : a.hot_lanes);
+  const int hp = n_hot <= (long long)n_waves * hl
+                     ? hl
                      : (int)((((long long)n_hot + n_waves - 1) / n_waves + kWave - 1) & ~(long long)(kWave - 1));
   const long long hs = (long long)wave * hp;
   const int n_hw = hs < n_hot ? (int)((hs + hp < n_hot ? hs + hp : n_hot) - hs) : 0;
