@@ -1426,6 +1426,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_RK5_HOT_MARGIN")) rk5_hot_margin_ = (float)atof(env);
   if (const char* env = getenv("NGPU_RK5_HOT_ONLY")) rk5_hot_only_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_FOLD_IN_RK5")) fold_in_rk5_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
@@ -1680,6 +1681,11 @@ void Engine::launch_step() {
           // the split point that balances two launches comes earlier
           const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 14 : rk5_defer_cap_);
           a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
+          if (g->fold_pending) {
+            a.prefold = 1;
+            a.pf = make_fold_args(*g);
+            a.pf.ctx = step_ctx(it_ - 1);
+          }
         }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
@@ -1719,6 +1725,12 @@ void Engine::launch_step() {
   timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
+    // a split-step RK5 population's fold is done by its next RK5 launch, as
+    // each lane takes its neuron (that launch comes after this delivery, so
+    // spikes of split steps arriving in this slot are in), except after the
+    // last step of a Simulate() call: state is complete between calls
+    g->fold_pending = fold_in_rk5_ && g->lag.p && rk5_persistent_ && defer_ok_;
+    if (g->fold_pending) continue;
     FoldArgs fa = make_fold_args(*g);
     fa.ctx = sc;
     fa.lag = g->lag.p;

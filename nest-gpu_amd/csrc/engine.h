@@ -88,6 +88,8 @@ struct Group {
   bool has_pe = false;
   // or, when every edge fits one uniform run (see FoldArgs), that run
   bool pe_run = false;
+  // the last step's fold was left to the next RK5 launch of this population
+  bool fold_pending = false;
   long long pu_b = 0, pu_e = 0;
   int pu_gen0 = 0, pu_gstep = 0;
   float pu_w = 0.0f, pu_delay = 0.0f;
@@ -298,6 +300,7 @@ class Engine {
   bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
   bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
   int rk5_hot_lanes_ = 64;             // hot entries per wave
+  bool fold_in_rk5_ = true;            // split-step RK5 populations fold inside the next RK5 launch
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]

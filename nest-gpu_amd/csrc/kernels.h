@@ -112,6 +112,11 @@ struct RK5Args {
   unsigned char* lag;              // [n] or null (no split steps)
   double* pend;                    // [n_port][n]
   int defer_cap;                   // 0: no new split in this launch
+  // the previous step's fold of this population, done by the lane that
+  // takes each neuron (prefold = 1; pf.ctx is the previous step's): the
+  // separate fold launch of that step was skipped
+  int prefold;
+  FoldArgs pf;
 };
 
 // Whether the persistent RK5 kernel's compile-time input layout of a model

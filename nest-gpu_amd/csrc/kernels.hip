@@ -513,11 +513,13 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // that of rk5_kernel.  UNI: every parameter except refractory_step is
 // uniform over the population, so parameters come from the kernel-argument
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
-// The learned-proposal kernel of a small model is held to 168 VGPRs (3 waves
-// per SIMD); the compiler would otherwise take a few more and drop to 2.
+// Register budget of a small model's kernel (at most 6 state variables): the
+// plain kernel is held to 128 VGPRs (4 waves per SIMD) and the
+// learned-proposal kernel to 168 (3 waves); the compiler would otherwise take
+// a few more and drop a wave per SIMD.
 template <class M, bool UNI, int MODE>
 __global__ void __launch_bounds__(kRK5Block)
-__attribute__((amdgpu_waves_per_eu(MODE == 2 && M::NV <= 6 ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu(M::NV > 6 ? 1 : MODE == 2 ? 3 : MODE == 0 ? 4 : 1)))
 rk5_persistent_kernel(RK5Args a) {
   static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
@@ -652,7 +654,27 @@ rk5_persistent_kernel(RK5Args a) {
           h = a.h[i];
           if constexpr (MODE != 0) el = a.err_last[i];
           lagged = a.lag ? a.lag[i] != 0 : false;
-          if (lagged) {
+          if (a.prefold) {
+            // the previous step's fold of this neuron's channels (the fold
+            // kernel's arithmetic): added to the input variables, or kept
+            // as the stashed input of a neuron still inside that step
+            const FoldArgs& f = a.pf;
+#pragma unroll
+            for (int k = 0; k < M::NPORT; ++k) {
+              const long long lo = (long long)k * n;
+              double* rp = f.ring + (size_t)f.ctx.slot * f.n_ch + (size_t)(f.ch_base + lo + i);
+              double acc = poisson_input(f, lo + i, *rp, f.ctx.t1, f.ctx.t_idx,
+                                         f.pu_b < lo + n && f.pu_e > lo);
+              *rp = 0.0;
+              if (lagged) {
+                pend_v[k] = acc;
+              } else if (acc != 0.0) {
+                const int wj = M::in_w(k);
+                const float w = wj >= 0 ? p[wj < 0 ? 0 : wj] : 1.0f;
+                y[M::in_y(k)] = fold_add(y[M::in_y(k)], acc, w);
+              }
+            }
+          } else if (lagged) {
 #pragma unroll
             for (int k = 0; k < M::NPORT; ++k) pend_v[k] = a.pend[(size_t)k * n + i];
           }
