@@ -440,3 +440,47 @@ def test_split_steps_match_unsplit(ngpu):
     np.testing.assert_array_equal(v1, v0)
     np.testing.assert_array_equal(g11, g10)
     assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
+
+
+def test_fold_in_rk5_matches_fold_launch(ngpu):
+    # the fold of a step done by the next RK5 launch as each lane takes its
+    # neuron (default) against a fold launch after every step: with many
+    # split steps and over two Simulate() calls (the last step of a call
+    # folds by launch, the next call's first launch folds nothing), rasters,
+    # state and attempt counts are equal
+    net = netbuild.c1_network(seed=9)
+    res = []
+    for fold_in in ("1", "0"):
+        os.environ["NGPU_RK5_DEFER"] = "3"
+        os.environ["NGPU_FOLD_IN_RK5"] = fold_in
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            nrn = ngpu.Create("aeif_cond_beta_multisynapse", net["n"], 2)
+            ngpu.SetStatus(nrn, netbuild.C1_RECEPTORS)
+            pg = ngpu.Create("poisson_generator")
+            ngpu.SetStatus(pg, "rate", net["rate"])
+            ngpu.Connect(pg, nrn, {"rule": "all_to_all"},
+                         {"weight": netbuild.C1_POISSON_W, "delay": 0.1, "receptor": 0})
+            se, te, we, de = net["exc"]
+            si, ti, wi, di = net["inh"]
+            netbuild.connect_edges_fast(ngpu, se + nrn[0], te + nrn[0], 0, we, de)
+            netbuild.connect_edges_fast(ngpu, si + nrn[0], ti + nrn[0], 1, wi, di)
+            ngpu.ActivateRecSpikeTimes(nrn, 5000)
+            ngpu.Simulate(30.0)
+            ngpu.Simulate(30.0)
+            st = [list(x) for x in ngpu.GetRecSpikeTimes(nrn)]
+            v = np.array(ngpu.GetStatus(nrn, "V_m"))
+            g1 = np.array(ngpu.GetStatus(nrn, "g1"))
+            res.append((st, v, g1, ngpu.GetStepStats()))
+        finally:
+            os.environ.pop("NGPU_RK5_DEFER", None)
+            os.environ.pop("NGPU_FOLD_IN_RK5", None)
+    (st1, v1, g11, s1), (st0, v0, g10, s0) = res
+    n_spk = sum(len(x) for x in st1)
+    print("split steps", s1["split_steps"], "spikes", n_spk)
+    assert s1["split_steps"] > 500 and n_spk > 50
+    assert st1 == st0
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(g11, g10)
+    assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
