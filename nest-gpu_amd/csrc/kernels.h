@@ -106,9 +106,9 @@ struct RK5Args {
   int hot_lanes;                   // hot entries per wave (1..64) while they fit the grid
   // split steps: a neuron still integrating after defer_cap attempts of a
   // step (at a substep boundary) is stored with lag = 1, finishes that step
-  // first in the next launch, then adds that step's input (stashed in pend
-  // by the fold launch) and goes on; its spike is delivered with its own
-  // emission time
+  // first in the next launch, then adds that step's input (folded by that
+  // launch itself, or stashed in pend by a fold launch) and goes on; its
+  // spike is delivered with its own emission time
   unsigned char* lag;              // [n] or null (no split steps)
   double* pend;                    // [n_port][n]
   int defer_cap;                   // 0: no new split in this launch

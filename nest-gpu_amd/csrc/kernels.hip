@@ -810,8 +810,9 @@ rk5_persistent_kernel(RK5Args a) {
           p[M::i_ref] = rr < 0.0f ? 0.0f : rr;
         }
         if (end_step && lagged) {
-          // the previous step is complete: add its input (stashed by that
-          // step's fold launch, same arithmetic), then go on with this step
+          // the previous step is complete: add its input (computed when the
+          // lane took the neuron, or stashed by that step's fold launch; the
+          // fold's arithmetic), then go on with this step
 #pragma unroll
           for (int k = 0; k < M::NPORT; ++k) {
             const double acc = pend_v[k];
