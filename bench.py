@@ -8,17 +8,21 @@ step of the whole network.  The network is built through the nestgpu API, as a
 user script would; build and calibration are not timed (as the reference's
 "Time to simulate").
 
-Timed windows: `--steps` steps with the plain RK5 controller (the reported
-`value`), then, when a trained step-size predictor exists for the workload
-(nest-gpu_amd/predictors/<workload>.json or --predictor), the same number of
-steps with the learned proposal on the same network, with its RK5 statistics,
-spike rate and ISI CV beside the plain window's.
+Timing: at least 3000 untimed steps (300 ms of biological time, in which
+the network's rate settles) whatever --warmup asks, then exactly `--steps`
+timed steps with the plain RK5 controller (the reported `value`; a window
+that delivers no synaptic event is refused).  When a trained step-size
+predictor exists for the workload (nest-gpu_amd/predictors/<workload>.json or
+--predictor), three further 1 s windows follow on the same network: plain,
+learned, plain again, with the learned window's RK5 statistics, spike rate
+and ISI CV against the mean of the two plain windows around it.
 
 Prints ONE JSON line on stdout.  Progress goes to stderr.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -34,6 +38,11 @@ DT = workloads.DT
 N_FULL = 1_000_000
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# untimed steps before the first timed window: the network's rate settles
+# within ~300 ms of biological time
+SETTLE_STEPS = 3000
+# steps of each window of the learned-vs-plain comparison (1 s bio)
+LEARN_STEPS = 10000
 
 
 def log(*a):
@@ -62,12 +71,31 @@ IAF_BYTES_PER_NEURON = 32
 POISSON_BYTES_PER_EDGE = 20
 
 
-def cpu_baseline(n_sample, sim_ms, seed):
+def cpu_model():
+    """The host CPU's model line (lscpu 'Model name', else /proc/cpuinfo)."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def cpu_slice(n_sample, sim_ms, seed):
     """The oracle (host C++/OpenMP restatement of the reference step) on a
     slice of the scale_adex network: n_sample neurons with the full in-degree
-    1000 and the same per-neuron input; the per-neuron-step cost is scaled to
-    the 1M-neuron workload."""
-    from oracle.oracle import Oracle, threads
+    1000 and the same per-neuron input.  Returns (wall seconds per
+    neuron-step, synaptic events per wall second, wall seconds)."""
+    from oracle.oracle import Oracle
     rng = np.random.default_rng(seed)
     n = n_sample
     ne = int(0.8 * n)
@@ -95,13 +123,31 @@ def cpu_baseline(n_sample, sim_ms, seed):
     wall = time.perf_counter() - t0
     st1 = o.stats()
     steps = int(round(sim_ms / DT))
-    per_neuron_step = wall / (n * steps)
-    rtf_full = per_neuron_step * N_FULL * (1000.0 / DT)
+    return wall / (n * steps), (st1["syn_events"] - st0["syn_events"]) / wall, wall
+
+
+def cpu_baseline(n_sample, sim_ms, seed):
+    """CPU baseline of the 1M-neuron second: the oracle's per-neuron-step
+    cost on an n_sample slice, scaled to 1e6 neurons x 1 s.  A second slice
+    of half the size states how far the per-neuron-step cost moves with the
+    slice size (the extrapolation's error estimate: the full network's state
+    and synapse table no longer fit the host caches, so the scaled figure is
+    a lower bound of the full-size CPU time)."""
+    from oracle.oracle import threads
+    half = max(n_sample // 2, 1000)
+    c_half, _, w_half = cpu_slice(half, sim_ms, seed)
+    c_full, ev, wall = cpu_slice(n_sample, sim_ms, seed)
+    rtf_full = c_full * N_FULL * (1000.0 / DT)
     return {"value": rtf_full, "unit": "s_wall/s_bio", "cores": threads(), "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": "oracle on %d neurons x %d synapses/neuron for %.0f ms (%.1f s wall); "
-                      "per-neuron-step cost scaled to 1e6 neurons x 1 s" % (n, 1000, sim_ms, wall),
-            "neuron_updates_per_s": n * steps / wall,
-            "syn_events_per_s": (st1["syn_events"] - st0["syn_events"]) / wall}
+                      "per-neuron-step cost scaled to 1e6 neurons x 1 s" % (n_sample, 1000, sim_ms, wall),
+            "extrapolation_check": {"neurons": half, "wall_s": w_half,
+                                    "cost_ratio_full_vs_half": c_full / c_half,
+                                    "note": "per-neuron-step cost of the %d-neuron slice over that "
+                                            "of the %d-neuron slice" % (n_sample, half)},
+            "neuron_updates_per_s": 1.0 / c_full,
+            "syn_events_per_s": ev}
 
 
 def timed_window(ngpu, steps, timing=2):
@@ -145,7 +191,7 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
     if cnt:
         flops = st["attempts"] * rk5_flops_per_attempt(n_port)
         a = flops / (ms * 1e-3) / 1e12
-        out["rk5_update"] = {"bound": "mfma", "unit": "TFLOP/s", "achieved": a, "peak": PEAK_FP32_TFLOPS,
+        out["rk5_update"] = {"bound": "fp32_valu", "unit": "TFLOP/s", "achieved": a, "peak": PEAK_FP32_TFLOPS,
                              "frac": a / PEAK_FP32_TFLOPS, "avg_launch_us": ms * 1e3 / cnt,
                              "flops_per_launch": flops / cnt,
                              "note": "FP32 vector peak (the RK5 arithmetic is FP32 VALU work)"}
@@ -192,8 +238,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000, help="timed 0.1 ms steps per window (10000 = 1 s)")
-    ap.add_argument("--warmup", type=int, default=3000,
-                    help="untimed steps first (the network's rate settles within ~300 ms)")
+    ap.add_argument("--warmup", type=int, default=SETTLE_STEPS,
+                    help="untimed steps first; at least %d (300 ms, the network's rate settles "
+                         "within that) whatever is asked" % SETTLE_STEPS)
     ap.add_argument("--workload", default="scale_adex", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--neurons", type=int, default=N_FULL)
     ap.add_argument("--seed", type=int, default=None)
@@ -245,61 +292,80 @@ def main():
     # no RK5 in the microcircuit: its timed windows bracket every launch
     win_timing = 1 if mc else 2
     if predictor is not None:
-        ngpu.ActivateRecSpikeTimes(neurons, 150)
+        ngpu.ActivateRecSpikeTimes(neurons, 400)
     t0 = time.perf_counter()
     ngpu.Calibrate()
     log("calibrated in %.1f s" % (time.perf_counter() - t0))
-    ngpu.Simulate(args.warmup * DT)
+    # the network's rate settles within ~300 ms of biological time: whatever
+    # --warmup asks, at least that much runs untimed before any window
+    settle = max(args.warmup, SETTLE_STEPS)
+    ngpu.Simulate(settle * DT)
     ngpu.Synchronize()
-    log("warmup %d steps done" % args.warmup)
-    t_plain0 = (args.warmup) * DT
+    log("settled: %d untimed steps (%.0f ms bio)" % (settle, settle * DT))
+    t_plain0 = settle * DT
     wp = timed_window(ngpu, args.steps, timing=win_timing)
     sp = window_summary(wp, args.steps, n)
-    log("plain: %d steps, wall %.3f s, RTF %.4f, kernel ms %s" % (args.steps, wp["wall"], sp["rtf"], wp["kt"]))
+    log("plain: %d steps, wall %.3f s, RTF %.4f, events %d, kernel ms %s"
+        % (args.steps, wp["wall"], sp["rtf"], wp["stats"]["syn_events"], wp["kt"]))
+    if wp["stats"]["syn_events"] == 0:
+        # a window without synaptic events is not the workload: no value
+        log("bench.py: the timed window delivered no synaptic events (network inactive); "
+            "no measurement is reported")
+        sys.exit(3)
+    t_now = t_plain0 + args.steps * DT
 
     learned = None
     if predictor is not None:
-        # plain (above), learned, plain again: the learned window's spike
-        # statistics are compared with the mean of the two plain windows
-        # around it, so a slow drift of the network is not read as an effect
+        # plain, learned, plain again on fixed 1 s windows after the timed
+        # one (whatever --steps is): the learned window's spike statistics
+        # are compared with the mean of the two plain windows around it, so a
+        # slow drift of the network is not read as an effect
+        L = LEARN_STEPS
+        ngpu.Simulate(200 * DT)     # untimed gap
+        t_pa = t_now + 200 * DT
+        wa = timed_window(ngpu, L)
+        sa = window_summary(wa, L, n)
         ngpu.SetPredictorParams(np.asarray(predictor["params"], np.float32))
         ngpu.SetStepPredictorMode("learned")
         ngpu.Simulate(200 * DT)     # switch-over, untimed
-        t_learn0 = t_plain0 + (args.steps + 200) * DT
-        wl = timed_window(ngpu, args.steps)
+        t_learn0 = t_pa + (L + 200) * DT
+        wl = timed_window(ngpu, L)
         ngpu.SetStepPredictorMode("plain")
-        sl = window_summary(wl, args.steps, n)
+        sl = window_summary(wl, L, n)
         log("learned: wall %.3f s, RTF %.4f, kernel ms %s" % (wl["wall"], sl["rtf"], wl["kt"]))
         ngpu.Simulate(200 * DT)
-        t_plain2 = t_learn0 + (args.steps + 200) * DT
-        wp2 = timed_window(ngpu, args.steps)
-        sp2 = window_summary(wp2, args.steps, n)
-        log("plain again: wall %.3f s, RTF %.4f" % (wp2["wall"], sp2["rtf"]))
+        t_plain2 = t_learn0 + (L + 200) * DT
+        wp2 = timed_window(ngpu, L)
+        sp2 = window_summary(wp2, L, n)
+        log("plain before/after: RTF %.4f / %.4f" % (sa["rtf"], sp2["rtf"]))
         t0 = time.perf_counter()
         times = ngpu.GetRecSpikeTimes(neurons)
-        rp, cvp, ncp = window_spike_stats(times, t_plain0, t_plain0 + args.steps * DT, n)
-        rl, cvl, ncl = window_spike_stats(times, t_learn0, t_learn0 + args.steps * DT, n)
-        rp2, cvp2, _ = window_spike_stats(times, t_plain2, t_plain2 + args.steps * DT, n)
+        rp, cvp, ncp = window_spike_stats(times, t_pa, t_pa + L * DT, n)
+        rl, cvl, ncl = window_spike_stats(times, t_learn0, t_learn0 + L * DT, n)
+        rp2, cvp2, _ = window_spike_stats(times, t_plain2, t_plain2 + L * DT, n)
         log("spike statistics in %.1f s: plain rate %.3f CV %s, learned rate %.3f CV %s, plain again "
             "rate %.3f CV %s" % (time.perf_counter() - t0, rp, cvp, rl, cvl, rp2, cvp2))
-        sp["isi_cv"] = cvp
+        sa["isi_cv"] = cvp
         sl["isi_cv"] = cvl
         sp2["isi_cv"] = cvp2
         r_ref = 0.5 * (rp + rp2)
         cv_ref = 0.5 * (cvp + cvp2) if (cvp and cvp2) else None
         learned = dict(sl)
+        learned["window_steps"] = L
+        learned["plain_before"] = sa
         learned["plain_after"] = sp2
-        learned["speedup_vs_plain"] = 0.5 * (sp["rtf"] + sp2["rtf"]) / sl["rtf"]
-        learned["rk5_kernel_speedup_vs_plain"] = (0.5 * (wp["kt"]["rk5_update"][0] + wp2["kt"]["rk5_update"][0])
+        learned["speedup_vs_plain"] = 0.5 * (sa["rtf"] + sp2["rtf"]) / sl["rtf"]
+        learned["rk5_kernel_speedup_vs_plain"] = (0.5 * (wa["kt"]["rk5_update"][0] + wp2["kt"]["rk5_update"][0])
                                                   / max(wl["kt"]["rk5_update"][0], 1e-9))
         learned["rate_rel_diff"] = (rl - r_ref) / r_ref if r_ref else None
         learned["cv_rel_diff"] = (cvl - cv_ref) / cv_ref if (cv_ref and cvl) else None
+        learned["neurons_in_cv"] = ncl
         learned["predictor"] = {k: predictor[k] for k in predictor if k != "params"}
 
     # every launch timed (events around each kernel slow the step a little,
     # so this window is reported apart from the timed windows above): the
     # per-kernel split of a step and the rooflines of all kernels
-    nb = min(args.steps, 2000)
+    nb = 2000
     wb = timed_window(ngpu, nb, timing=1)
     log("breakdown: %d steps, wall %.3f s, kernel ms %s" % (nb, wb["wall"], wb["kt"]))
     roof_all = rooflines(wb, n, n_port, nb, n_poisson_csr_edges)
@@ -335,6 +401,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_steps": settle,
+        "timed_bio_ms": args.steps * DT,
         "ms_per_step": sp["ms_per_step"],
         "higher_is_better": False,
         "scaling": "weak",
@@ -343,11 +411,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": ("%s: iaf_psc_exp %d neurons in 8 populations, fixed_total_number per "
                                 "pair (p ~ U(0.01,0.15)), %.3g synapses, normal weights and delays, "
-                                "Poisson 8 Hz x K_ext, exact propagator; 1 s bio per window"
-                                % (args.workload, n, n_syn)) if mc else
+                                "Poisson 8 Hz x K_ext, exact propagator; timed %d steps = %.1f ms bio after "
+                                "%.0f ms bio settling" % (args.workload, n, n_syn, args.steps,
+                                                          args.steps * DT, settle * DT)) if mc else
                                ("%s: aeif_cond_beta_multisynapse %d neurons, %d ports, indegree 1000 "
                                 "(%.2g synapses), delays U{1..50}x0.1 ms, Poisson input, plain RK5 "
-                                "(eps 1e-6), 1 s bio per window" % (args.workload, n, n_port, n_syn)),
+                                "(eps 1e-6); timed %d steps = %.1f ms bio after %.0f ms bio settling"
+                                % (args.workload, n, n_port, n_syn, args.steps, args.steps * DT,
+                                   settle * DT)),
                    "neurons": n, "synapses": n_syn, "dt_ms": DT, "seed": seed,
                    "method": "exact iaf_psc_exp update" if mc else
                              "plain RK5 (value); learned step-size proposal in 'learned'"},

@@ -19,7 +19,12 @@
 
 namespace ngpu {
 
-constexpr int kMaxPort = 8;             // ports per RK5 population in this build
+// receptor ports per multisynapse population (the reference's MAX_PORT_NUM,
+// src/aeif_cond_beta_multisynapse.h:165); populations with up to kFastPort
+// ports get the persistent RK5 kernel's uniform-parameter and predictor
+// variants
+constexpr int kMaxPort = 20;
+constexpr int kFastPort = 8;
 constexpr int kMaxParam = 14 + 4 * kMaxPort;
 constexpr int kWave = 64;
 // persistent RK5 kernel: threads per block, and LDS entries per wave for the

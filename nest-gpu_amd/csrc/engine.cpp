@@ -53,12 +53,26 @@ static void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
   }
 }
 
+// Blocking copy ordered after all work queued on the simulator stream (a
+// plain hipMemcpy runs on the null stream, which does not wait for a
+// non-blocking stream's kernels).
+static void copy_on(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
 // ---------------------------------------------------------------------------
 // Model name tables.  Names and their order are the reference's interface
 // (src/*_kernel.h enums and name arrays) so Set/GetStatus indices match.
 static const std::vector<std::string> kAeifScal = {
     "V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
     "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"};
+
+static std::vector<std::string> cat(std::vector<std::string> a, const std::vector<std::string>& b) {
+  a.insert(a.end(), b.begin(), b.end());
+  return a;
+}
 
 static const std::vector<ModelDesc>& model_table() {
   static const std::vector<ModelDesc> t = {
@@ -76,6 +90,28 @@ static const std::vector<ModelDesc>& model_table() {
        {"V_th", "a", "b", "c", "d", "I_e", "t_ref", "refractory_step", "den_delay"},
        {"E_rev", "tau_rise", "tau_decay", "g0"}, {"h_min_rel", "h0_rel"}, {},
        -1, true, 1, true, 3},
+      // two fixed ports with ex/in names (src/aeif_cond_alpha_kernel.h:42-110,
+      // src/aeif_cond_beta_kernel.h:42-110, src/aeif_psc_alpha_kernel.h:42-110)
+      {"aeif_cond_alpha", MODEL_AEIF_COND_ALPHA, {"g_ex", "g_in", "g1_ex", "g1_in", "V_m", "w"}, {},
+       cat({"g0_ex", "g0_in", "E_rev_ex", "E_rev_in", "tau_syn_ex", "tau_syn_in"}, kAeifScal),
+       {}, {"h_min_rel", "h0_rel"}, {}, 2, true, 2, false, 0, 1, true},
+      {"aeif_cond_beta", MODEL_AEIF_COND_BETA, {"g_ex", "g_in", "g1_ex", "g1_in", "V_m", "w"}, {},
+       cat({"g0_ex", "g0_in", "E_rev_ex", "E_rev_in", "tau_rise_ex", "tau_rise_in",
+            "tau_decay_ex", "tau_decay_in"}, kAeifScal),
+       {}, {"h_min_rel", "h0_rel"}, {}, 2, true, 2, false, 0, 1, true},
+      {"aeif_psc_alpha", MODEL_AEIF_PSC_ALPHA,
+       {"I_syn_ex", "I_syn_in", "I1_syn_ex", "I1_syn_in", "V_m", "w"}, {},
+       cat({"tau_syn_ex", "tau_syn_in", "I0_ex", "I0_in"}, kAeifScal),
+       {}, {"h_min_rel", "h0_rel"}, {}, 2, true, 2, false, 2, 1, true},
+      // src/aeif_psc_alpha_multisynapse_kernel.h:42-110
+      {"aeif_psc_alpha_multisynapse", MODEL_AEIF_PSC_ALPHA_MS, {"V_m", "w"}, {"I_syn", "I1_syn"},
+       kAeifScal, {"tau_syn", "I0"}, {"h_min_rel", "h0_rel"}, {}, -1, true, 1, true, 1},
+      // src/aeif_psc_delta_kernel.h:42-100: every port jumps V_m, weight 1
+      {"aeif_psc_delta", MODEL_AEIF_PSC_DELTA, {"V_m", "w"}, {}, kAeifScal, {},
+       {"h_min_rel", "h0_rel"}, {}, -1, true, 0, false, -1, 0, false},
+      // src/aeif_psc_exp_multisynapse_kernel.h:42-100
+      {"aeif_psc_exp_multisynapse", MODEL_AEIF_PSC_EXP_MS, {"V_m", "w"}, {"I_syn"}, kAeifScal,
+       {"tau_syn"}, {"h_min_rel", "h0_rel"}, {}, -1, true, 0, true, -1},
       {"iaf_psc_exp", MODEL_IAF_PSC_EXP, {"I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"},
        {},
        {"tau_m", "C_m", "E_L", "I_e", "Theta_rel", "V_reset_rel", "tau_ex", "tau_in",
@@ -93,10 +129,9 @@ static const std::vector<ModelDesc>& model_table() {
 
 // Reference model names outside this build's scope (src/neuron_models.h:60-86).
 static const char* kOtherModels[] = {
-    "iaf_psc_exp_g", "iaf_psc_exp_hc", "iaf_psc_alpha", "ext_neuron", "aeif_cond_alpha",
-    "aeif_cond_beta", "aeif_psc_alpha", "aeif_psc_delta", "aeif_psc_exp_multisynapse",
-    "aeif_psc_alpha_multisynapse", "parrot_neuron", "izhikevich", "izhikevich_psc_exp_5s",
-    "izhikevich_psc_exp_2s", "izhikevich_psc_exp", "user_m1", "user_m2"};
+    "iaf_psc_exp_g", "iaf_psc_exp_hc", "iaf_psc_alpha", "ext_neuron", "parrot_neuron",
+    "izhikevich", "izhikevich_psc_exp_5s", "izhikevich_psc_exp_2s", "izhikevich_psc_exp",
+    "user_m1", "user_m2"};
 
 // Where a model's port input lands: variable index of port 0's input and the
 // stride between ports, parameter index of port 0's weight (-1: weight 1).
@@ -110,10 +145,18 @@ static FoldLayout fold_layout(const ModelDesc& m) {
     f.in_port_stride = (int)m.port_var.size();
   } else {
     f.in_var = m.in_var;
-    f.in_port_stride = 1;
+    f.in_port_stride = m.in_scal_stride;
   }
-  f.w_param = m.w_param >= 0 ? (int)m.scal_param.size() + m.w_param : -1;
-  f.w_port_stride = (int)m.port_param.size();
+  if (m.w_param < 0) {
+    f.w_param = -1;
+    f.w_port_stride = 0;
+  } else if (m.w_in_scal) {
+    f.w_param = m.w_param;
+    f.w_port_stride = 1;
+  } else {
+    f.w_param = (int)m.scal_param.size() + m.w_param;
+    f.w_port_stride = (int)m.port_param.size();
+  }
   return f;
 }
 
@@ -282,19 +325,43 @@ int Engine::Create(const std::string& model, int n, int n_port) {
     case MODEL_AEIF_COND_BETA_MS:
     case MODEL_AEIF_COND_ALPHA_MS:
     case MODEL_AEIF_PSC_EXP:
+    case MODEL_AEIF_COND_ALPHA:
+    case MODEL_AEIF_COND_BETA:
+    case MODEL_AEIF_PSC_ALPHA:
+    case MODEL_AEIF_PSC_ALPHA_MS:
+    case MODEL_AEIF_PSC_DELTA:
+    case MODEL_AEIF_PSC_EXP_MS:
       P("V_th", -50.4f); P("Delta_T", 2.0f); P("g_L", 30.0f); P("E_L", -70.6f);
       P("C_m", 281.0f); P("a", 4.0f); P("b", 80.5f); P("tau_w", 144.0f); P("I_e", 0.f);
       P("V_peak", 0.f); P("V_reset", -60.0f); P("t_ref", 0.f); P("den_delay", 0.f);
       P("refractory_step", 0.f);
       V("V_m", -70.6f); V("w", 0.f);
-      if (md->id == MODEL_AEIF_PSC_EXP) {
-        P("tau_syn_ex", 0.2f); P("tau_syn_in", 2.0f);
-      } else {
-        for (int q = 0; q < n_port; ++q) {
-          P("E_rev", 0.f, q);
-          if (md->id == MODEL_AEIF_COND_BETA_MS) { P("tau_decay", 20.f, q); P("tau_rise", 2.f, q); }
-          else P("tau_syn", 2.f, q);
-        }
+      switch (md->id) {
+        case MODEL_AEIF_PSC_EXP:
+        case MODEL_AEIF_PSC_ALPHA:   // src/aeif_psc_alpha.cu:64-65
+          P("tau_syn_ex", 0.2f); P("tau_syn_in", 2.0f);
+          break;
+        case MODEL_AEIF_COND_ALPHA:  // src/aeif_cond_alpha.cu:55-58
+          P("E_rev_ex", 0.f); P("E_rev_in", -85.f); P("tau_syn_ex", 0.2f); P("tau_syn_in", 2.0f);
+          break;
+        case MODEL_AEIF_COND_BETA:   // src/aeif_cond_beta.cu:56-61
+          P("E_rev_ex", 0.f); P("E_rev_in", 0.f); P("tau_decay_ex", 20.f); P("tau_decay_in", 20.f);
+          P("tau_rise_ex", 2.f); P("tau_rise_in", 2.f);
+          break;
+        case MODEL_AEIF_PSC_ALPHA_MS:   // src/aeif_psc_alpha_multisynapse.cu:61-65
+        case MODEL_AEIF_PSC_EXP_MS:     // src/aeif_psc_exp_multisynapse.cu:60-63
+          for (int q = 0; q < n_port; ++q) P("tau_syn", 0.2f, q);
+          break;
+        case MODEL_AEIF_COND_BETA_MS:
+        case MODEL_AEIF_COND_ALPHA_MS:
+          for (int q = 0; q < n_port; ++q) {
+            P("E_rev", 0.f, q);
+            if (md->id == MODEL_AEIF_COND_BETA_MS) { P("tau_decay", 20.f, q); P("tau_rise", 2.f, q); }
+            else P("tau_syn", 2.f, q);
+          }
+          break;
+        default:
+          break;
       }
       break;
     case MODEL_IZH_COND_BETA:
@@ -425,28 +492,29 @@ int Engine::GetVarSize(int node, const std::string& n) {
 }
 
 // Column j of a population: host staging before calibration, device after.
-static void col_read(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev, int j,
-                     int rel0, int cnt, float* out) {
+static void col_read(hipStream_t stream_, bool cal, Group& g, std::vector<float>& stage,
+                     DBuf<float>& dev, int j, int rel0, int cnt, float* out) {
   if (!cal) {
     std::memcpy(out, stage.data() + (size_t)j * g.n + rel0, cnt * sizeof(float));
   } else {
-    HIPCHK(hipMemcpy(out, dev.p + (size_t)j * g.n + rel0, cnt * sizeof(float), hipMemcpyDeviceToHost));
+    copy_on(stream_, out, dev.p + (size_t)j * g.n + rel0, cnt * sizeof(float), hipMemcpyDeviceToHost);
   }
 }
-static void col_write(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev, int j,
-                      int rel0, int cnt, const float* in) {
+static void col_write(hipStream_t stream_, bool cal, Group& g, std::vector<float>& stage,
+                      DBuf<float>& dev, int j, int rel0, int cnt, const float* in) {
   if (!cal) {
     std::memcpy(stage.data() + (size_t)j * g.n + rel0, in, cnt * sizeof(float));
   } else {
-    HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n + rel0, in, cnt * sizeof(float), hipMemcpyHostToDevice));
+    copy_on(stream_, dev.p + (size_t)j * g.n + rel0, in, cnt * sizeof(float), hipMemcpyHostToDevice);
   }
 }
 
 // Writes vals[k] at rows rel[k] of column j: into the host staging before
 // calibration (O(entries)), else to the device (one copy for a contiguous
 // ascending range, one read-modify-write of the column otherwise).
-static void col_write_entries(bool cal, Group& g, std::vector<float>& stage, DBuf<float>& dev,
-                              int j, const std::vector<int>& rel, const float* vals) {
+static void col_write_entries(hipStream_t stream_, bool cal, Group& g, std::vector<float>& stage,
+                              DBuf<float>& dev, int j, const std::vector<int>& rel,
+                              const float* vals) {
   if (rel.empty()) return;
   if (!cal) {
     float* c = stage.data() + (size_t)j * g.n;
@@ -456,14 +524,14 @@ static void col_write_entries(bool cal, Group& g, std::vector<float>& stage, DBu
   bool contiguous = true;
   for (size_t k = 1; k < rel.size() && contiguous; ++k) contiguous = rel[k] == rel[0] + (int)k;
   if (contiguous) {
-    HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n + rel[0], vals, rel.size() * sizeof(float),
-                     hipMemcpyHostToDevice));
+    copy_on(stream_, dev.p + (size_t)j * g.n + rel[0], vals, rel.size() * sizeof(float),
+                     hipMemcpyHostToDevice);
     return;
   }
   std::vector<float> col(g.n);
-  HIPCHK(hipMemcpy(col.data(), dev.p + (size_t)j * g.n, g.n * sizeof(float), hipMemcpyDeviceToHost));
+  copy_on(stream_, col.data(), dev.p + (size_t)j * g.n, g.n * sizeof(float), hipMemcpyDeviceToHost);
   for (size_t k = 0; k < rel.size(); ++k) col[rel[k]] = vals[k];
-  HIPCHK(hipMemcpy(dev.p + (size_t)j * g.n, col.data(), g.n * sizeof(float), hipMemcpyHostToDevice));
+  copy_on(stream_, dev.p + (size_t)j * g.n, col.data(), g.n * sizeof(float), hipMemcpyHostToDevice);
 }
 
 void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const float* vals,
@@ -472,14 +540,14 @@ void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const
   auto& stage = stage_param_[gi];
   if (whole) {
     std::vector<float> col(g.n, vals[0]);
-    col_write(calibrated_, g, stage, g.param, j, 0, g.n, col.data());
+    col_write(stream_, calibrated_, g, stage, g.param, j, 0, g.n, col.data());
     if (!(g.md->rk5 && j == param_idx(g, "refractory_step", 0))) {
       g.uni[j] = 1;
       g.uni_val[j] = vals[0];
     }
     return;
   }
-  col_write_entries(calibrated_, g, stage, g.param, j, rel, vals);
+  col_write_entries(stream_, calibrated_, g, stage, g.param, j, rel, vals);
   // stays uniform only if every written value equals the uniform value (a
   // column made equal again by partial writes is just read per neuron)
   if (g.uni[j]) {
@@ -494,7 +562,7 @@ void Engine::set_param_index(Group& g, int j, const std::vector<int>& rel, const
 void Engine::set_var_index(Group& g, int j, const std::vector<int>& rel, const float* vals) {
   int gi = group_index(groups_, &g);
   auto& stage = stage_var_[gi];
-  col_write_entries(calibrated_, g, stage, g.var, j, rel, vals);
+  col_write_entries(stream_, calibrated_, g, stage, g.var, j, rel, vals);
 }
 
 static std::vector<int> iota_rel(int rel0, int n) {
@@ -626,7 +694,7 @@ void Engine::SetIntVar(int i_node, int n, const std::string& name, int val) {
   if (!calibrated_) return;  // counters are created zeroed at calibration
   DBuf<int>& d = (name == "spike_count") ? g.spike_count : g.n_rec;
   std::vector<int> v(n, val);
-  HIPCHK(hipMemcpy(d.p + rel0, v.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  copy_on(stream_, d.p + rel0, v.data(), n * sizeof(int), hipMemcpyHostToDevice);
 }
 
 void Engine::SetIntVar(const int* nodes, int n, const std::string& name, int val) {
@@ -652,14 +720,14 @@ float* Engine::GetParam(int i_node, int n, const std::string& name) {
   std::vector<float>& out = ret_float(0);
   if (std::find(sp.begin(), sp.end(), name) != sp.end()) {
     out.resize(n);
-    col_read(calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, 0), rel0, n, out.data());
+    col_read(stream_, calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, 0), rel0, n, out.data());
     return out.data();
   }
   if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
     out.assign((size_t)n * g.n_port, 0.f);
     std::vector<float> col(n);
     for (int q = 0; q < g.n_port; ++q) {
-      col_read(calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, q), rel0, n, col.data());
+      col_read(stream_, calibrated_, g, stage_param_[gi], g.param, param_idx(g, name, q), rel0, n, col.data());
       for (int k = 0; k < n; ++k) out[(size_t)k * g.n_port + q] = col[k];
     }
     return out.data();
@@ -681,7 +749,7 @@ float* Engine::GetParam(const int* nodes, int n, const std::string& name) {
   std::vector<float> col(g->n);
   std::vector<float>& out = ret_float(0);
   if (std::find(sp.begin(), sp.end(), name) != sp.end()) {
-    col_read(calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, 0), 0, g->n, col.data());
+    col_read(stream_, calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, 0), 0, g->n, col.data());
     out.resize(n);
     for (int k = 0; k < n; ++k) out[k] = col[rel[k]];
     return out.data();
@@ -689,7 +757,7 @@ float* Engine::GetParam(const int* nodes, int n, const std::string& name) {
   if (std::find(pp.begin(), pp.end(), name) != pp.end()) {
     out.assign((size_t)n * g->n_port, 0.f);
     for (int q = 0; q < g->n_port; ++q) {
-      col_read(calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, q), 0, g->n, col.data());
+      col_read(stream_, calibrated_, *g, stage_param_[gi], g->param, param_idx(*g, name, q), 0, g->n, col.data());
       for (int k = 0; k < n; ++k) out[(size_t)k * g->n_port + q] = col[rel[k]];
     }
     return out.data();
@@ -719,14 +787,14 @@ float* Engine::GetVar(int i_node, int n, const std::string& name) {
   std::vector<float>& out = ret_float(1);
   if (std::find(sv.begin(), sv.end(), name) != sv.end()) {
     out.resize(n);
-    col_read(calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, 0), rel0, n, out.data());
+    col_read(stream_, calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, 0), rel0, n, out.data());
     return out.data();
   }
   if (std::find(pv.begin(), pv.end(), name) != pv.end()) {
     out.assign((size_t)n * g.n_port, 0.f);
     std::vector<float> col(n);
     for (int q = 0; q < g.n_port; ++q) {
-      col_read(calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, q), rel0, n, col.data());
+      col_read(stream_, calibrated_, g, stage_var_[gi], g.var, var_idx(g, name, q), rel0, n, col.data());
       for (int k = 0; k < n; ++k) out[(size_t)k * g.n_port + q] = col[k];
     }
     return out.data();
@@ -743,7 +811,7 @@ float* Engine::GetVar(const int* nodes, int n, const std::string& name) {
   std::vector<float> col(g->n);
   std::vector<float>& out = ret_float(1);
   if (std::find(sv.begin(), sv.end(), name) != sv.end()) {
-    col_read(calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, 0), 0, g->n, col.data());
+    col_read(stream_, calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, 0), 0, g->n, col.data());
     out.resize(n);
     for (int k = 0; k < n; ++k) out[k] = col[rel[k]];
     return out.data();
@@ -751,7 +819,7 @@ float* Engine::GetVar(const int* nodes, int n, const std::string& name) {
   if (std::find(pv.begin(), pv.end(), name) != pv.end()) {
     out.assign((size_t)n * g->n_port, 0.f);
     for (int q = 0; q < g->n_port; ++q) {
-      col_read(calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, q), 0, g->n, col.data());
+      col_read(stream_, calibrated_, *g, stage_var_[gi], g->var, var_idx(*g, name, q), 0, g->n, col.data());
       for (int k = 0; k < n; ++k) out[(size_t)k * g->n_port + q] = col[rel[k]];
     }
     return out.data();
@@ -773,7 +841,7 @@ int* Engine::GetIntVar(int i_node, int n, const std::string& name) {
   if (calibrated_) {
     DBuf<int>& d = (name == "spike_count") ? g.spike_count : g.n_rec;
     sync_streams();
-    HIPCHK(hipMemcpy(out.data(), d.p + rel0, n * sizeof(int), hipMemcpyDeviceToHost));
+    copy_on(stream_, out.data(), d.p + rel0, n * sizeof(int), hipMemcpyDeviceToHost);
   }
   return out.data();
 }
@@ -1214,7 +1282,7 @@ int Engine::GetNRecSpikeTimes(int node) {
   if (!calibrated_) return 0;
   int v;
   sync_streams();
-  HIPCHK(hipMemcpy(&v, g.n_rec.p + rel, sizeof(int), hipMemcpyDeviceToHost));
+  copy_on(stream_, &v, g.n_rec.p + rel, sizeof(int), hipMemcpyDeviceToHost);
   return v;
 }
 
@@ -1224,16 +1292,17 @@ void Engine::buffer_rec_spike_times(Group& g) {
   if (!calibrated_ || g.max_rec <= 0) return;
   sync_streams();
   std::vector<int> cnt(g.n);
-  HIPCHK(hipMemcpy(cnt.data(), g.n_rec.p, g.n * sizeof(int), hipMemcpyDeviceToHost));
+  copy_on(stream_, cnt.data(), g.n_rec.p, g.n * sizeof(int), hipMemcpyDeviceToHost);
   long long tot = 0;
   for (int c : cnt) tot += c;
   if (tot == 0) return;
   std::vector<float> t((size_t)g.n * g.max_rec);
-  HIPCHK(hipMemcpy(t.data(), g.rec_times.p, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+  copy_on(stream_, t.data(), g.rec_times.p, t.size() * sizeof(float), hipMemcpyDeviceToHost);
   for (int i = 0; i < g.n; ++i)
     g.rec_host[i].insert(g.rec_host[i].end(), t.begin() + (size_t)i * g.max_rec,
                          t.begin() + (size_t)i * g.max_rec + cnt[i]);
-  HIPCHK(hipMemset(g.n_rec.p, 0, g.n * sizeof(int)));
+  // stream-ordered: the next step's kernels read the fill counts
+  HIPCHK(hipMemsetAsync(g.n_rec.p, 0, g.n * sizeof(int), stream_));
 }
 
 int Engine::GetRecSpikeTimes(int i_node, int n, int** n_pt, float*** t_pt) {
@@ -1267,37 +1336,67 @@ void Engine::calibrate_group(Group& g) {
     g.h0 = g.h0_rel * dt_;
     float* ref = col(param_idx(g, "refractory_step", 0));
     std::fill(ref, ref + n, 0.f);
-    if (m.id == MODEL_AEIF_COND_BETA_MS || m.id == MODEL_IZH_COND_BETA) {
-      // beta normalisation, src/aeif_cond_beta_multisynapse.cu:72-99
-      for (int q = 0; q < g.n_port; ++q) {
-        int jr = ns + q * npp + 1, jd = ns + q * npp + 2, jg = ns + q * npp + 3;
-        auto g0_of = [](float tr, float td) -> float {
-          float denom1 = td - tr, denom2 = 0;
-          if (denom1 != 0) {
-            float t_p = td * tr * std::log(td / tr) / denom1;
-            denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
-          }
-          if (denom2 == 0) return (float)(M_E / td);
-          return (float)((1. / tr - 1. / td) / denom2);
-        };
-        float* gc = col(jg);
-        const float* trc = col(jr);
-        const float* tdc = col(jd);
-        for (int i = 0; i < n; ++i) gc[i] = g0_of(trc[i], tdc[i]);
-        g.uni[jg] = g.uni[jr] && g.uni[jd];
-        g.uni_val[jg] = gc[0];
-      }
-    } else if (m.id == MODEL_AEIF_COND_ALPHA_MS) {
-      for (int q = 0; q < g.n_port; ++q) {
-        int js = ns + q * npp + 1, jg = ns + q * npp + 2;
-        float* gc = col(jg);
-        const float* ts = col(js);
-        for (int i = 0; i < n; ++i) gc[i] = (float)(M_E / ts[i]);
-        g.uni[jg] = g.uni[js];
-        g.uni_val[jg] = gc[0];
-      }
-    } else if (m.id == MODEL_AEIF_PSC_EXP) {
-      int jdt = param_idx(g, "Delta_T", 0), jth = param_idx(g, "V_th", 0), jpk = param_idx(g, "V_peak", 0);
+    // beta normalisation, src/aeif_cond_beta_multisynapse.cu:72-99
+    // (src/aeif_cond_beta.cu:81-113 per ex/in port)
+    auto beta_g0 = [&](int jr, int jd, int jg) {
+      auto g0_of = [](float tr, float td) -> float {
+        float denom1 = td - tr, denom2 = 0;
+        if (denom1 != 0) {
+          float t_p = td * tr * std::log(td / tr) / denom1;
+          denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
+        }
+        if (denom2 == 0) return (float)(M_E / td);
+        return (float)((1. / tr - 1. / td) / denom2);
+      };
+      float* gc = col(jg);
+      const float* trc = col(jr);
+      const float* tdc = col(jd);
+      for (int i = 0; i < n; ++i) gc[i] = g0_of(trc[i], tdc[i]);
+      g.uni[jg] = g.uni[jr] && g.uni[jd];
+      g.uni_val[jg] = gc[0];
+    };
+    // alpha normalisation e / tau (src/aeif_cond_alpha_multisynapse.cu:79,
+    // src/aeif_cond_alpha.cu, src/aeif_psc_alpha.cu:78-79)
+    auto alpha_g0 = [&](int js, int jg) {
+      float* gc = col(jg);
+      const float* ts = col(js);
+      for (int i = 0; i < n; ++i) gc[i] = (float)(M_E / ts[i]);
+      g.uni[jg] = g.uni[js];
+      g.uni_val[jg] = gc[0];
+    };
+    auto P = [&](const char* nm, int port = 0) { return param_idx(g, nm, port); };
+    switch (m.id) {
+      case MODEL_AEIF_COND_BETA_MS:
+      case MODEL_IZH_COND_BETA:
+        for (int q = 0; q < g.n_port; ++q)
+          beta_g0(ns + q * npp + 1, ns + q * npp + 2, ns + q * npp + 3);
+        break;
+      case MODEL_AEIF_COND_ALPHA_MS:
+        for (int q = 0; q < g.n_port; ++q) alpha_g0(ns + q * npp + 1, ns + q * npp + 2);
+        break;
+      case MODEL_AEIF_COND_BETA:
+        beta_g0(P("tau_rise_ex"), P("tau_decay_ex"), P("g0_ex"));
+        beta_g0(P("tau_rise_in"), P("tau_decay_in"), P("g0_in"));
+        break;
+      case MODEL_AEIF_COND_ALPHA:
+        alpha_g0(P("tau_syn_ex"), P("g0_ex"));
+        alpha_g0(P("tau_syn_in"), P("g0_in"));
+        break;
+      case MODEL_AEIF_PSC_ALPHA:
+        alpha_g0(P("tau_syn_ex"), P("I0_ex"));
+        alpha_g0(P("tau_syn_in"), P("I0_in"));
+        break;
+      case MODEL_AEIF_PSC_ALPHA_MS:
+        for (int q = 0; q < g.n_port; ++q) alpha_g0(P("tau_syn", q), P("I0", q));
+        break;
+      default:
+        break;
+    }
+    if (m.id == MODEL_AEIF_PSC_EXP || m.id == MODEL_AEIF_PSC_ALPHA ||
+        m.id == MODEL_AEIF_PSC_ALPHA_MS || m.id == MODEL_AEIF_PSC_DELTA ||
+        m.id == MODEL_AEIF_PSC_EXP_MS) {
+      // Delta_T <= 0: threshold at V_th (src/aeif_psc_exp.cu, src/aeif_psc_alpha.cu:75-77)
+      int jdt = P("Delta_T"), jth = P("V_th"), jpk = P("V_peak");
       bool changed = false;
       for (int i = 0; i < n; ++i)
         if (col(jdt)[i] <= 0.0f) { col(jpk)[i] = col(jth)[i]; changed = true; }
@@ -1580,7 +1679,7 @@ void Engine::upload_poisson_tables() {
   for (auto* g : groups_) {
     if (g->md->id != MODEL_POISSON_GEN) continue;
     std::vector<float> p((size_t)g->np * g->n);
-    HIPCHK(hipMemcpy(p.data(), g->param.p, p.size() * sizeof(float), hipMemcpyDeviceToHost));
+    copy_on(stream_, p.data(), g->param.p, p.size() * sizeof(float), hipMemcpyDeviceToHost);
     for (int i = 0; i < g->n; ++i) {
       float lambda = time_step * p[(size_t)0 * g->n + i];
       int m = poisson_chunks_host(lambda);
@@ -1646,7 +1745,13 @@ void Engine::launch_step() {
       case MODEL_AEIF_COND_BETA_MS:
       case MODEL_AEIF_COND_ALPHA_MS:
       case MODEL_AEIF_PSC_EXP:
-      case MODEL_IZH_COND_BETA: {
+      case MODEL_IZH_COND_BETA:
+      case MODEL_AEIF_COND_ALPHA:
+      case MODEL_AEIF_COND_BETA:
+      case MODEL_AEIF_PSC_ALPHA:
+      case MODEL_AEIF_PSC_ALPHA_MS:
+      case MODEL_AEIF_PSC_DELTA:
+      case MODEL_AEIF_PSC_EXP_MS: {
         RK5Args a{};
         a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
         a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
@@ -1803,7 +1908,7 @@ int Engine::StartSimulation() {
         float v = 0.f;
         if (r.var[k] != "spike") {
           int j = var_idx(*g, r.var[k], r.port[k]);
-          HIPCHK(hipMemcpy(&v, g->var.p + (size_t)j * g->n + r.rel[k], sizeof(float), hipMemcpyDeviceToHost));
+          copy_on(stream_, &v, g->var.p + (size_t)j * g->n + r.rel[k], sizeof(float), hipMemcpyDeviceToHost);
         }
         row.push_back(v);
       }
@@ -1872,7 +1977,7 @@ int Engine::SimulationStep() {
 void Engine::fetch_records(long long n_rows) {
   if (rec_ncol_ == 0 || n_rows == 0) return;
   std::vector<float> buf((size_t)n_rows * (rec_ncol_ + 1));
-  HIPCHK(hipMemcpy(buf.data(), d_rec_buf_.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost));
+  copy_on(stream_, buf.data(), d_rec_buf_.p, buf.size() * sizeof(float), hipMemcpyDeviceToHost);
   int col0 = 1;
   for (auto& r : records_) {
     int nc = (int)r.var.size();
@@ -1923,16 +2028,16 @@ void Engine::Stats(unsigned long long* out) {
   if (!calibrated_) return;
   sync_streams();
   std::vector<unsigned long long> rows(d_ctr_.n);
-  HIPCHK(hipMemcpy(rows.data(), d_ctr_.p, rows.size() * sizeof(unsigned long long),
-                   hipMemcpyDeviceToHost));
+  copy_on(stream_, rows.data(), d_ctr_.p, rows.size() * sizeof(unsigned long long),
+                   hipMemcpyDeviceToHost);
   for (size_t r = 0; r < rows.size(); r += kCtrCols) {
     for (int k = 0; k < 4; ++k) out[k] += rows[r + k];
     out[6] += rows[r + 4];
     out[7] += rows[r + 5];
   }
-  HIPCHK(hipMemcpy(&out[4], d_stats_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  copy_on(stream_, &out[4], d_stats_.p, sizeof(unsigned long long), hipMemcpyDeviceToHost);
   int ov = 0;
-  HIPCHK(hipMemcpy(&ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost));
+  copy_on(stream_, &ov, d_overflow_.p, sizeof(int), hipMemcpyDeviceToHost);
   out[5] = (unsigned long long)ov;
 }
 

@@ -36,6 +36,8 @@ struct ModelDesc {
   int in_var;             // input variable (port-block index if in_port_block)
   bool in_port_block;
   int w_param;            // port-param index of the input weight, -1 => 1
+  int in_scal_stride = 1; // scalar-block input: variable index step between ports (0: one shared variable)
+  bool w_in_scal = false; // w_param is a scalar-param index, one per port in a row
 };
 
 const ModelDesc* find_model(const std::string& name);

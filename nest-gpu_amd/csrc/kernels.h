@@ -20,6 +20,12 @@ enum ModelId {
   MODEL_AEIF_COND_ALPHA_MS,
   MODEL_AEIF_PSC_EXP,
   MODEL_IZH_COND_BETA,
+  MODEL_AEIF_COND_ALPHA,
+  MODEL_AEIF_COND_BETA,
+  MODEL_AEIF_PSC_ALPHA,
+  MODEL_AEIF_PSC_ALPHA_MS,
+  MODEL_AEIF_PSC_DELTA,
+  MODEL_AEIF_PSC_EXP_MS,
   MODEL_IAF_PSC_EXP,
   MODEL_POISSON_GEN,
   MODEL_SPIKE_GEN,
@@ -184,6 +190,23 @@ struct RecordArgs {
   long long row_base;    // step index (ctx->it) stored in buffer row 0
   long long max_rows;    // rows in buf
 };
+
+// Per-family RK5 launchers and input-layout checks (rk5_*.hip), called by
+// launch_rk5 / rk5_fold_layout_matches.
+void launch_rk5_cond_beta_ms(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_cond_alpha_ms(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_izh_cond_beta(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_psc_alpha_ms(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_psc_exp_ms(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_psc_delta(int n_port, const RK5Args& a, hipStream_t s);
+void launch_rk5_two_port(int model, const RK5Args& a, hipStream_t s);
+bool layout_cond_beta_ms(int n_port, int iv, int is, int wp, int ws);
+bool layout_cond_alpha_ms(int n_port, int iv, int is, int wp, int ws);
+bool layout_izh_cond_beta(int n_port, int iv, int is, int wp, int ws);
+bool layout_psc_alpha_ms(int n_port, int iv, int is, int wp, int ws);
+bool layout_psc_exp_ms(int n_port, int iv, int is, int wp, int ws);
+bool layout_psc_delta(int n_port, int iv, int is, int wp, int ws);
+bool layout_two_port(int model, int iv, int is, int wp, int ws);
 
 int rk5_supported(int model, int n_port);
 void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s);

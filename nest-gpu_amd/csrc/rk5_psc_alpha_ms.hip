@@ -1,0 +1,15 @@
+// RK5 kernels of the AeifPscAlphaMS<1..kMaxPort> model family (rk5_models.h),
+// instantiated in a file of their own so the families compile in parallel.
+#include "rk5_device.h"
+
+namespace ngpu {
+
+void launch_rk5_psc_alpha_ms(int n_port, const RK5Args& a, hipStream_t s) {
+  dispatch_ports<AeifPscAlphaMS>(n_port, a, s);
+}
+
+bool layout_psc_alpha_ms(int n_port, int iv, int is, int wp, int ws) {
+  return fold_layout_ports<AeifPscAlphaMS>(n_port, iv, is, wp, ws);
+}
+
+}  // namespace ngpu

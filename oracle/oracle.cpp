@@ -49,6 +49,12 @@ enum Model {
   M_AEIF_COND_ALPHA_MS,
   M_AEIF_PSC_EXP,
   M_IZH_COND_BETA,
+  M_AEIF_COND_ALPHA,
+  M_AEIF_COND_BETA,
+  M_AEIF_PSC_ALPHA,
+  M_AEIF_PSC_ALPHA_MS,
+  M_AEIF_PSC_DELTA,
+  M_AEIF_PSC_EXP_MS,
   M_IAF_PSC_EXP,
   M_POISSON_GEN,
   M_SPIKE_GEN,
@@ -65,7 +71,18 @@ struct ModelInfo {
                          // multisynapse models, scalar var otherwise)
   bool input_in_port_block;
   int weight_param;      // port param index of the input weight, -1 => 1.0
+  int input_scal_step = 1;    // scalar input var: index step between ports (0: shared)
+  bool weight_in_scal = false; // weight_param indexes the scalar block, one per port
 };
+
+static std::vector<std::string> adex_tail() {
+  return {"V_th", "Delta_T", "g_L", "E_L", "C_m", "a", "b", "tau_w", "I_e",
+          "V_peak", "V_reset", "t_ref", "refractory_step", "den_delay"};
+}
+static std::vector<std::string> join(std::vector<std::string> a, const std::vector<std::string>& b) {
+  a.insert(a.end(), b.begin(), b.end());
+  return a;
+}
 
 static const ModelInfo& info(int m) {
   static const std::vector<ModelInfo> tab = {
@@ -98,6 +115,36 @@ static const ModelInfo& info(int m) {
         "den_delay"},
        {"E_rev", "tau_rise", "tau_decay", "g0"},
        -1, true, 1, true, 3},
+      // src/aeif_cond_alpha_kernel.h:42-110: port k -> g1_ex/g1_in, weight g0_ex/g0_in
+      {"aeif_cond_alpha",
+       {"g_ex", "g_in", "g1_ex", "g1_in", "V_m", "w"},
+       {},
+       join({"g0_ex", "g0_in", "E_rev_ex", "E_rev_in", "tau_syn_ex", "tau_syn_in"}, adex_tail()),
+       {},
+       2, true, 2, false, 0, 1, true},
+      // src/aeif_cond_beta_kernel.h:42-110
+      {"aeif_cond_beta",
+       {"g_ex", "g_in", "g1_ex", "g1_in", "V_m", "w"},
+       {},
+       join({"g0_ex", "g0_in", "E_rev_ex", "E_rev_in", "tau_rise_ex", "tau_rise_in",
+             "tau_decay_ex", "tau_decay_in"}, adex_tail()),
+       {},
+       2, true, 2, false, 0, 1, true},
+      // src/aeif_psc_alpha_kernel.h:42-110: port k -> I1_syn_ex/in, weight I0_ex/in
+      {"aeif_psc_alpha",
+       {"I_syn_ex", "I_syn_in", "I1_syn_ex", "I1_syn_in", "V_m", "w"},
+       {},
+       join({"tau_syn_ex", "tau_syn_in", "I0_ex", "I0_in"}, adex_tail()),
+       {},
+       2, true, 2, false, 2, 1, true},
+      // src/aeif_psc_alpha_multisynapse_kernel.h:42-110: input I1_syn, weight I0
+      {"aeif_psc_alpha_multisynapse", {"V_m", "w"}, {"I_syn", "I1_syn"}, adex_tail(),
+       {"tau_syn", "I0"}, -1, true, 1, true, 1},
+      // src/aeif_psc_delta.cu:123-133: every port into V_m, weight 1
+      {"aeif_psc_delta", {"V_m", "w"}, {}, adex_tail(), {}, -1, true, 0, false, -1, 0, false},
+      // src/aeif_psc_exp_multisynapse_kernel.h:42-100: input I_syn, weight 1
+      {"aeif_psc_exp_multisynapse", {"V_m", "w"}, {"I_syn"}, adex_tail(), {"tau_syn"},
+       -1, true, 0, true, -1},
       {"iaf_psc_exp",
        {"I_syn_ex", "I_syn_in", "V_m_rel", "refractory_step"},
        {},
@@ -235,6 +282,119 @@ struct IzhCondBeta {  // src/izhikevich_cond_beta_kernel.h:152-170
       dy[3 + 2 * i] = -y[3 + 2 * i] / p[9 + 4 * i + 1];
       dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / p[9 + 4 * i + 2];
     }
+  }
+};
+
+// aeif_cond_alpha (src/aeif_cond_alpha_kernel.h:158-176) and aeif_cond_beta
+// (src/aeif_cond_beta_kernel.h:160-180): state g_ex, g_in, g1_ex, g1_in,
+// V_m, w; leading parameters g0_ex, g0_in, E_rev_ex, E_rev_in and the
+// synaptic time constants (alpha: tau_syn_ex, tau_syn_in; beta: tau_rise_ex,
+// tau_rise_in, tau_decay_ex, tau_decay_in), then the AdEx block.
+template <class T, bool BETA>
+struct AeifCond2 {
+  static void deriv(int, const T* y, T* dy, const T* p) {
+    enum { g_ex, g_in, g1_ex, g1_in, V_m, w };
+    const T* q = p + (BETA ? 8 : 6);
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    const T tr_ex = p[4], tr_in = p[5];
+    const T td_ex = BETA ? p[6] : p[4], td_in = BETA ? p[7] : p[5];
+    T I_syn_ex = 0, I_syn_in = 0;
+    T V = (q[ref] > 0) ? q[V_reset] : std::min(y[V_m], q[V_peak]);
+    I_syn_ex = std::fma(y[g_ex], p[2] - V, I_syn_ex);
+    I_syn_in = std::fma(y[g_in], p[3] - V, I_syn_in);
+    T V_spike = q[Delta_T] * exp_model((V - q[V_th]) / q[Delta_T]);
+    dy[V_m] = (q[ref] > 0) ? T(0)
+                           : (std::fma(-q[g_L], V - q[E_L] - V_spike, I_syn_ex) + I_syn_in -
+                              y[w] + q[I_e]) / q[C_m];
+    dy[w] = std::fma(q[a], V - q[E_L], -y[w]) / q[tau_w];
+    dy[g1_ex] = -y[g1_ex] / tr_ex;
+    dy[g_ex] = y[g1_ex] - y[g_ex] / td_ex;
+    dy[g1_in] = -y[g1_in] / tr_in;
+    dy[g_in] = y[g1_in] - y[g_in] / td_in;
+  }
+};
+
+// aeif_psc_alpha, src/aeif_psc_alpha_kernel.h:160-180
+template <class T>
+struct AeifPscAlpha {
+  static void deriv(int, const T* y, T* dy, const T* p) {
+    enum { I_ex, I_in, I1_ex, I1_in, V_m, w };
+    const T* q = p + 4;   // after tau_syn_ex, tau_syn_in, I0_ex, I0_in
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T I_syn_tot = 0;
+    T V = (q[ref] > 0) ? q[V_reset] : std::min(y[V_m], q[V_peak]);
+    I_syn_tot += y[I_ex] - y[I_in];
+    T V_spike = q[Delta_T] == 0. ? T(0.)
+                                 : q[Delta_T] * exp_model((V - q[V_th]) / q[Delta_T]);
+    dy[V_m] = (q[ref] > 0) ? T(0)
+                           : (std::fma(-q[g_L], V - q[E_L] - V_spike, I_syn_tot) - y[w] +
+                              q[I_e]) / q[C_m];
+    dy[w] = std::fma(q[a], V - q[E_L], -y[w]) / q[tau_w];
+    dy[I1_ex] = -y[I1_ex] / p[0];
+    dy[I1_in] = -y[I1_in] / p[1];
+    dy[I_ex] = y[I1_ex] - y[I_ex] / p[0];
+    dy[I_in] = y[I1_in] - y[I_in] / p[1];
+  }
+};
+
+// aeif_psc_alpha_multisynapse, src/aeif_psc_alpha_multisynapse_kernel.h:157-181
+// (state V_m, w, then I_syn, I1_syn per port; port params tau_syn, I0)
+template <class T>
+struct AeifPscAlphaMS {
+  static void deriv(int np, const T* y, T* dy, const T* p) {
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T I_syn_tot = 0;
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
+    for (int i = 0; i < np; ++i) I_syn_tot += y[2 + 2 * i];
+    T V_spike = p[Delta_T] == 0. ? T(0.)
+                                 : p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : (std::fma(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) - y[1] +
+                            p[I_e]) / p[C_m];
+    dy[1] = std::fma(p[a], V - p[E_L], -y[1]) / p[tau_w];
+    for (int i = 0; i < np; ++i) {
+      const T tau = p[14 + 2 * i];
+      dy[3 + 2 * i] = -y[3 + 2 * i] / tau;
+      dy[2 + 2 * i] = y[3 + 2 * i] - y[2 + 2 * i] / tau;
+    }
+  }
+};
+
+// aeif_psc_exp_multisynapse, src/aeif_psc_exp_multisynapse_kernel.h:157-177
+template <class T>
+struct AeifPscExpMS {
+  static void deriv(int np, const T* y, T* dy, const T* p) {
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T I_syn_tot = 0;
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
+    for (int i = 0; i < np; ++i) I_syn_tot += y[2 + i];
+    T V_spike = p[Delta_T] == 0. ? T(0.)
+                                 : p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : (std::fma(-p[g_L], V - p[E_L] - V_spike, I_syn_tot) - y[1] +
+                            p[I_e]) / p[C_m];
+    dy[1] = std::fma(p[a], V - p[E_L], -y[1]) / p[tau_w];
+    for (int i = 0; i < np; ++i) dy[2 + i] = -y[2 + i] / p[14 + i];
+  }
+};
+
+// aeif_psc_delta, src/aeif_psc_delta_kernel.h:166-178 (no synaptic state:
+// inputs jump V_m at the fold)
+template <class T>
+struct AeifPscDelta {
+  static void deriv(int, const T* y, T* dy, const T* p) {
+    enum { V_th, Delta_T, g_L, E_L, C_m, a, b, tau_w, I_e, V_peak, V_reset,
+           t_ref, ref, den };
+    T V = (p[ref] > 0) ? p[V_reset] : std::min(y[0], p[V_peak]);
+    T V_spike = p[Delta_T] == 0. ? T(0.)
+                                 : p[Delta_T] * exp_model((V - p[V_th]) / p[Delta_T]);
+    dy[0] = (p[ref] > 0) ? T(0)
+                         : (std::fma(-p[g_L], V - p[E_L] - V_spike, -y[1]) + p[I_e]) / p[C_m];
+    dy[1] = std::fma(p[a], V - p[E_L], -y[1]) / p[tau_w];
   }
 };
 
@@ -584,15 +744,30 @@ class Engine {
         case M_AEIF_COND_BETA_MS:
         case M_AEIF_COND_ALPHA_MS:
         case M_AEIF_PSC_EXP:
+        case M_AEIF_COND_ALPHA:
+        case M_AEIF_COND_BETA:
+        case M_AEIF_PSC_ALPHA:
+        case M_AEIF_PSC_ALPHA_MS:
+        case M_AEIF_PSC_DELTA:
+        case M_AEIF_PSC_EXP_MS:
           P(r, "V_th", 0, -50.4); P(r, "Delta_T", 0, 2.0); P(r, "g_L", 0, 30.0);
           P(r, "E_L", 0, -70.6); P(r, "C_m", 0, 281.0); P(r, "a", 0, 4.0);
           P(r, "b", 0, 80.5); P(r, "tau_w", 0, 144.0); P(r, "I_e", 0, 0.0);
           P(r, "V_peak", 0, 0.0); P(r, "V_reset", 0, -60.0); P(r, "t_ref", 0, 0.0);
           P(r, "den_delay", 0, 0.0); P(r, "refractory_step", 0, 0.0);
           V(r, "V_m", 0, -70.6); V(r, "w", 0, 0.0);  // V_m = E_L
-          if (g.model == M_AEIF_PSC_EXP) {
+          if (g.model == M_AEIF_PSC_EXP || g.model == M_AEIF_PSC_ALPHA) {
             P(r, "tau_syn_ex", 0, 0.2); P(r, "tau_syn_in", 0, 2.0);
-          } else {
+          } else if (g.model == M_AEIF_COND_ALPHA) {  // src/aeif_cond_alpha.cu:55-58
+            P(r, "E_rev_ex", 0, 0.0); P(r, "E_rev_in", 0, -85.0);
+            P(r, "tau_syn_ex", 0, 0.2); P(r, "tau_syn_in", 0, 2.0);
+          } else if (g.model == M_AEIF_COND_BETA) {   // src/aeif_cond_beta.cu:56-61
+            P(r, "E_rev_ex", 0, 0.0); P(r, "E_rev_in", 0, 0.0);
+            P(r, "tau_decay_ex", 0, 20.0); P(r, "tau_decay_in", 0, 20.0);
+            P(r, "tau_rise_ex", 0, 2.0); P(r, "tau_rise_in", 0, 2.0);
+          } else if (g.model == M_AEIF_PSC_ALPHA_MS || g.model == M_AEIF_PSC_EXP_MS) {
+            for (int q = 0; q < g.n_port; ++q) P(r, "tau_syn", q, 0.2);
+          } else if (g.model == M_AEIF_COND_BETA_MS || g.model == M_AEIF_COND_ALPHA_MS) {
             for (int q = 0; q < g.n_port; ++q) {
               P(r, "E_rev", q, 0.0);
               if (g.model == M_AEIF_COND_BETA_MS) {
@@ -720,27 +895,43 @@ class Engine {
     int ref = param_index(g, "refractory_step", 0);
     p[ref] = 0;
     int ns = info(g.model).scal_param.size();
+    // beta peak normalisation of a rise/decay pair into g0
+    auto beta_norm = [](T tr, T td) -> T {
+      T denom1 = td - tr;
+      T denom2 = 0;
+      if (denom1 != 0) {
+        T t_p = td * tr * std::log(td / tr) / denom1;
+        denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
+      }
+      if (denom2 == 0) return (T)(M_E / td);
+      return (T)((1. / tr - 1. / td) / denom2);
+    };
+    auto PI = [&](const char* nm, int port) { return param_index(g, nm, port); };
     if (g.model == M_AEIF_COND_BETA_MS || g.model == M_IZH_COND_BETA) {
       for (int i = 0; i < g.n_port; ++i) {
         T* pp = p + ns + 4 * i;  // E_rev, tau_rise, tau_decay, g0
-        T tr = pp[1], td = pp[2];
-        T denom1 = td - tr;
-        T denom2 = 0;
-        if (denom1 != 0) {
-          T t_p = td * tr * std::log(td / tr) / denom1;
-          denom2 = std::exp(-t_p / td) - std::exp(-t_p / tr);
-        }
-        if (denom2 == 0)
-          pp[3] = (T)(M_E / td);
-        else
-          pp[3] = (T)((1. / tr - 1. / td) / denom2);
+        pp[3] = beta_norm(pp[1], pp[2]);
       }
     } else if (g.model == M_AEIF_COND_ALPHA_MS) {
       for (int i = 0; i < g.n_port; ++i) {
         T* pp = p + ns + 3 * i;  // E_rev, tau_syn, g0
         pp[2] = (T)(M_E / pp[1]);
       }
-    } else if (g.model == M_AEIF_PSC_EXP) {
+    } else if (g.model == M_AEIF_COND_BETA) {   // src/aeif_cond_beta.cu:81-113
+      p[PI("g0_ex", 0)] = beta_norm(p[PI("tau_rise_ex", 0)], p[PI("tau_decay_ex", 0)]);
+      p[PI("g0_in", 0)] = beta_norm(p[PI("tau_rise_in", 0)], p[PI("tau_decay_in", 0)]);
+    } else if (g.model == M_AEIF_COND_ALPHA) {  // src/aeif_cond_alpha.cu:79-80
+      p[PI("g0_ex", 0)] = (T)(M_E / p[PI("tau_syn_ex", 0)]);
+      p[PI("g0_in", 0)] = (T)(M_E / p[PI("tau_syn_in", 0)]);
+    } else if (g.model == M_AEIF_PSC_ALPHA) {   // src/aeif_psc_alpha.cu:78-79
+      p[PI("I0_ex", 0)] = (T)(M_E / p[PI("tau_syn_ex", 0)]);
+      p[PI("I0_in", 0)] = (T)(M_E / p[PI("tau_syn_in", 0)]);
+    } else if (g.model == M_AEIF_PSC_ALPHA_MS) {  // src/aeif_psc_alpha_multisynapse.cu:80-82
+      for (int i = 0; i < g.n_port; ++i) p[PI("I0", i)] = (T)(M_E / p[PI("tau_syn", i)]);
+    }
+    if (g.model == M_AEIF_PSC_EXP || g.model == M_AEIF_PSC_ALPHA ||
+        g.model == M_AEIF_PSC_ALPHA_MS || g.model == M_AEIF_PSC_DELTA ||
+        g.model == M_AEIF_PSC_EXP_MS) {
       int iDT = param_index(g, "Delta_T", 0);
       if (p[iDT] <= 0.0) p[param_index(g, "V_peak", 0)] = p[param_index(g, "V_th", 0)];
     }
@@ -912,25 +1103,35 @@ class Engine {
           rk5_step<T, AeifPscExp<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
         case M_IZH_COND_BETA:
           rk5_step<T, IzhCondBeta<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_COND_ALPHA:
+          rk5_step<T, AeifCond2<T, false>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_COND_BETA:
+          rk5_step<T, AeifCond2<T, true>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_PSC_ALPHA:
+          rk5_step<T, AeifPscAlpha<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_PSC_ALPHA_MS:
+          rk5_step<T, AeifPscAlphaMS<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_PSC_DELTA:
+          rk5_step<T, AeifPscDelta<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
+        case M_AEIF_PSC_EXP_MS:
+          rk5_step<T, AeifPscExpMS<T>>(g.n_port, g.nv, x, y, h, h_min, hmax, p, st); break;
       }
       end = (x >= x1 - (double)h_min);
       external_update(g, r, y, p, end);
     }
   }
 
-  // ExternalUpdate of the four RK5 models
+  // ExternalUpdate of the RK5 models (src/aeif_*_kernel.h, src/izhikevich_cond_beta_kernel.h):
+  // the same rule on each model's own names
   void external_update(Group<T>& g, int r, T* y, T* p, bool end) {
-    int iV = (g.model == M_AEIF_PSC_EXP) ? 2 : 0;
-    int iw = iV + 1;
+    const bool izh = g.model == M_IZH_COND_BETA;
+    int iV = var_index(g, "V_m", 0);
+    int iw = var_index(g, izh ? "u" : "w", 0);
     int ref = param_index(g, "refractory_step", 0);
-    int i_reset, i_thr, i_jump;
-    if (g.model == M_IZH_COND_BETA) {
-      i_reset = 3; i_thr = 0; i_jump = 4;  // c, V_th, d
-    } else {
-      int off = (g.model == M_AEIF_PSC_EXP) ? 2 : 0;
-      i_reset = 10 + off; i_thr = 9 + off; i_jump = 6 + off;  // V_reset, V_peak, b
-    }
-    int i_tref = ref - 1;
+    int i_reset = param_index(g, izh ? "c" : "V_reset", 0);
+    int i_thr = param_index(g, izh ? "V_th" : "V_peak", 0);
+    int i_jump = param_index(g, izh ? "d" : "b", 0);
+    int i_tref = param_index(g, "t_ref", 0);
     if (y[iV] < -1.0e3) { y[iV] = p[i_reset]; y[iw] = 0; return; }
     if (y[iw] < -1.0e6 || y[iw] > 1.0e6) { y[iV] = p[i_reset]; y[iw] = 0; return; }
     if (p[ref] > 0.0) {
@@ -1022,14 +1223,18 @@ class Engine {
     const ModelInfo& mi = info(g.model);
     int nsv = mi.scal_var.size(), npv = mi.port_var.size();
     int nsp = mi.scal_param.size(), npp = mi.port_param.size();
+    // ports in order (a model whose ports share one variable adds them one after another)
     for (int q = 0; q < g.n_port; ++q) {
       for (int r = 0; r < g.n; ++r) {
         double a = g.acc[(size_t)q * g.n + r];
         int iv = mi.input_in_port_block ? nsv + q * npv + mi.input_var
-                                        : mi.input_var + q;
+                                        : mi.input_var + q * mi.input_scal_step;
         T* in = &g.y[(size_t)r * g.nv + iv];
         double wgt = 1.0;
-        if (mi.weight_param >= 0) wgt = (double)g.p[(size_t)r * g.np_ + nsp + q * npp + mi.weight_param];
+        if (mi.weight_param >= 0) {
+          int ip = mi.weight_in_scal ? mi.weight_param + q : nsp + q * npp + mi.weight_param;
+          wgt = (double)g.p[(size_t)r * g.np_ + ip];
+        }
         *in = (T)((double)*in + a * wgt);
       }
     }

@@ -55,6 +55,58 @@ SCENARIOS = {
         syn=[(0, 1.0, 1.0), (1, 2.0, 100.0)],
         var="V_m", v_offset=0.0,
         golden="test_aeif_psc_exp_nest.txt", tol=5e-6),
+    # python/test/test_aeif_cond_alpha.py:4-23
+    "aeif_cond_alpha": dict(
+        model="aeif_cond_alpha", n_port=2,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0,
+              "E_rev_ex": 20.0, "E_rev_in": -85.0, "tau_syn_ex": 40.0, "tau_syn_in": 20.0},
+        port={},
+        syn=[(0, 0.1, 1.0), (1, 0.2, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_cond_alpha_nest.txt", tol=5e-5),
+    # python/test/test_aeif_cond_beta.py:4-25
+    "aeif_cond_beta": dict(
+        model="aeif_cond_beta", n_port=2,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0,
+              "E_rev_ex": 20.0, "E_rev_in": -85.0, "tau_decay_ex": 40.0, "tau_decay_in": 20.0,
+              "tau_rise_ex": 20.0, "tau_rise_in": 5.0},
+        port={},
+        syn=[(0, 0.1, 1.0), (1, 0.2, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_cond_beta_nest.txt", tol=5e-5),
+    # python/test/test_aeif_psc_alpha.py:4-26
+    "aeif_psc_alpha": dict(
+        model="aeif_psc_alpha", n_port=2,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0,
+              "tau_syn_ex": 40.0, "tau_syn_in": 20.0},
+        port={},
+        syn=[(0, 1.0, 1.0), (1, 2.0, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_psc_alpha_nest.txt", tol=3e-6),
+    # python/test/test_aeif_psc_alpha_multisynapse.py:4-25
+    "aeif_psc_alpha_multisynapse": dict(
+        model="aeif_psc_alpha_multisynapse", n_port=2,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0},
+        port={"tau_syn": [40.0, 20.0]},
+        syn=[(0, 1.0, 1.0), (1, -2.0, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_psc_alpha_multisynapse_nest.txt", tol=5e-4),
+    # python/test/test_aeif_psc_delta.py:4-24 (one port: both inputs on port 0)
+    "aeif_psc_delta": dict(
+        model="aeif_psc_delta", n_port=1,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0, "C_m": 20000.0},
+        port={},
+        syn=[(0, 1.0, 1.0), (0, -2.0, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_psc_delta_nest.txt", tol=5e-4),
+    # python/test/test_aeif_psc_exp_multisynapse.py:4-25
+    "aeif_psc_exp_multisynapse": dict(
+        model="aeif_psc_exp_multisynapse", n_port=2,
+        scal={"V_peak": 0.0, "a": 4.0, "b": 80.5, "E_L": -70.6, "g_L": 300.0},
+        port={"tau_syn": [40.0, 20.0]},
+        syn=[(0, 1.0, 1.0), (1, -2.0, 100.0)],
+        var="V_m", v_offset=0.0,
+        golden="test_aeif_psc_exp_multisynapse_nest.txt", tol=5e-4),
     # python/test/test_iaf_psc_exp.py:4-30 (records V_m_rel, adds E_L=-70)
     "iaf_psc_exp": dict(
         model="iaf_psc_exp", n_port=2, scal={}, port={},

@@ -49,10 +49,83 @@ def test_single_neuron_trace(ngpu, name):
     rel = np.max(np.abs(v - vo) / np.maximum(np.abs(vo), 1e-6))
     print(name, "max rel diff vs oracle float", rel,
           "vs FP64", np.max(np.abs(v - v64) / np.abs(v64)))
-    if sc["model"] == "iaf_psc_exp":
-        np.testing.assert_array_equal(v, vo)
-    else:
-        assert rel < 1e-5
+    # within 1e-5 relative of the reference float path, and in fact the same
+    # float operation sequence as the oracle's: identical values
+    assert rel < 1e-5
+    np.testing.assert_array_equal(v, vo)
+
+
+# (model, ports): the reference's maximum of 20 receptor ports
+# (src/aeif_cond_beta_multisynapse.h:165) for the multisynapse models, and
+# aeif_psc_delta, whose ports all jump V_m (applied port after port)
+MANY_PORTS = [("aeif_cond_beta_multisynapse", 20), ("aeif_cond_alpha_multisynapse", 20),
+              ("izhikevich_cond_beta", 20), ("aeif_psc_alpha_multisynapse", 20),
+              ("aeif_psc_exp_multisynapse", 20), ("aeif_psc_delta", 3),
+              ("aeif_cond_beta_multisynapse", 9)]
+
+
+def _many_ports_case(model, n_port):
+    n = 3
+    cond = "cond" in model
+    izh = model.startswith("izhikevich")
+    # one generator per port; several ports' inputs arrive in the same step
+    times = [[5.0 + 3.0 * (k % 7), 40.0 + 2.0 * (k % 4), 60.0 + (k % 2)] for k in range(n_port)]
+    delay = [0.1 * (1 + k % 3) for k in range(n_port)]
+
+    def weight(k, i):
+        if cond:
+            return (0.4 + 0.05 * k) * (1.0 + 0.5 * i) * (4.0 if izh else 1.0)
+        sign = 1.0 if k % 2 == 0 else -1.0
+        scale = 1.0 if model == "aeif_psc_delta" else 25.0
+        return sign * scale * (1.0 + 0.1 * k) * (1.0 + 0.5 * i)
+
+    status = {"I_e": 10.0 if izh else 700.0}
+    if cond:
+        status["E_rev"] = [0.0 if k % 2 == 0 else -85.0 for k in range(n_port)]
+    return n, times, delay, weight, status
+
+
+@pytest.mark.parametrize("model,n_port", MANY_PORTS)
+def test_many_ports_bitexact(ngpu, model, n_port):
+    from oracle.oracle import Oracle
+    n, times, delay, weight, status = _many_ports_case(model, n_port)
+    print("case", model, n_port, flush=True)
+    nrn = ngpu.Create(model, n, n_port)
+    ngpu.SetStatus(nrn, status)
+    sg = ngpu.Create("spike_generator", n_port)
+    for k in range(n_port):
+        ngpu.SetStatus([sg[k]], {"spike_times": times[k]})
+    for k in range(n_port):
+        for i in range(n):
+            ngpu.Connect([sg[k]], [nrn[i]], {"rule": "one_to_one"},
+                         {"receptor": k, "weight": weight(k, i), "delay": delay[k]})
+    wname = "u" if model.startswith("izhikevich") else "w"
+    rec = ngpu.CreateRecord("", ["V_m"] * n + [wname] * n, [nrn[i] for i in range(n)] * 2, [0] * (2 * n))
+    ngpu.Simulate(100.0)
+    g = np.array(ngpu.GetRecordData(rec), np.float32)[:, 1:]
+
+    o = Oracle(prec=0)
+    n0 = o.create(model, n, n_port)
+    for key, v in status.items():
+        if isinstance(v, list):
+            for k, vv in enumerate(v):
+                o.set(n0, n, key, [vv] * n, port=k)
+        else:
+            o.set(n0, n, key, [v] * n)
+    s0 = o.create("spike_generator", n_port, 0)
+    for k in range(n_port):
+        o.set_spike_times(s0 + k, times[k])
+    for k in range(n_port):
+        for i in range(n):
+            o.connect([s0 + k], [n0 + i], [k], [weight(k, i)], [delay[k]])
+    recs = [o.record(n0 + i, "V_m") for i in range(n)] + [o.record(n0 + i, wname) for i in range(n)]
+    o.simulate(100.0)
+    ref = np.stack([o.record_data(r)[1] for r in recs], axis=1)
+    print(model, n_port, "V range", float(ref[:, :n].min()), float(ref[:, :n].max()), flush=True)
+    assert g.shape == ref.shape
+    # inputs changed the trajectory (every port reached the neuron)
+    assert np.ptp(ref[:, 0]) > 1.0
+    np.testing.assert_array_equal(g, ref)
 
 
 def test_izhikevich_cond_beta_single(ngpu):
