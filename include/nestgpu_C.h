@@ -226,6 +226,19 @@ int NESTGPU_AMD_ResetKernel();
 /* Blocks until all queued device work of the simulator is done. */
 int NESTGPU_AMD_Synchronize();
 
+/* Synapse table export (no counterpart in the reference; used by tests to
+ * replay a network's input): out[k] = outgoing synapses of node i_node + k;
+ * the number of synapses into the listed targets; and those synapses in the
+ * order of the source-major synapse table (source, target, port, weight,
+ * delay in ms, and the synapse's index in that table, which is the Poisson
+ * stream id of a poisson_generator's synapse).  Buffers are the caller's,
+ * sized by NESTGPU_AMD_CountIncoming. */
+int NESTGPU_AMD_GetOutDegrees(int i_node, int n_node, long long *out);
+long long NESTGPU_AMD_CountIncoming(const int *targets, int n_target);
+int NESTGPU_AMD_GetIncoming(const int *targets, int n_target, int *source, int *target,
+                            unsigned char *port, float *weight, float *delay,
+                            long long *index);
+
 /* ---- learned step-size predictor (no counterpart in the reference) ----
  * An MLP 8 -> 32 -> 32 -> 1 proposes each RK5 substep's step size for
  * aeif_cond_beta_multisynapse populations with uniform parameters (persistent

@@ -1207,6 +1207,70 @@ void Engine::GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned 
   *w = c_w_[c];
 }
 
+void Engine::OutDegrees(int i_node, int n, long long* out) {
+  sort_connections();
+  if (i_node < 0 || n < 0 || i_node + n > n_nodes_) throw Error("Unrecognized node");
+  for (int k = 0; k < n; ++k) out[k] = row_ptr_[i_node + k + 1] - row_ptr_[i_node + k];
+}
+
+// Per-thread ranges of the source-major table holding synapses into the
+// marked targets, filled in table order.
+static std::vector<char> target_mask(const int* targets, int n, int n_nodes) {
+  std::vector<char> m(n_nodes, 0);
+  for (int k = 0; k < n; ++k) {
+    if (targets[k] < 0 || targets[k] >= n_nodes) throw Error("Unrecognized target node");
+    m[targets[k]] = 1;
+  }
+  return m;
+}
+
+long long Engine::CountIncoming(const int* targets, int n) {
+  sort_connections();
+  const std::vector<char> m = target_mask(targets, n, n_nodes_);
+  long long cnt = 0;
+  const long long nc = (long long)c_tgt_.size();
+#pragma omp parallel for schedule(static) reduction(+ : cnt)
+  for (long long c = 0; c < nc; ++c) cnt += m[c_tgt_[c]];
+  return cnt;
+}
+
+void Engine::GetIncoming(const int* targets, int n, int* src, int* tgt, unsigned char* port,
+                         float* w, float* delay, long long* index) {
+  sort_connections();
+  const std::vector<char> m = target_mask(targets, n, n_nodes_);
+  const long long nc = (long long)c_tgt_.size();
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  std::vector<long long> per(nt + 1, 0);
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    const long long b = nc * t / nt, e = nc * (t + 1) / nt;
+    long long c0 = 0;
+    for (long long c = b; c < e; ++c) c0 += m[c_tgt_[c]];
+    per[t + 1] = c0;
+#pragma omp barrier
+#pragma omp single
+    for (int k = 0; k < nt; ++k) per[k + 1] += per[k];
+    long long o = per[t];
+    for (long long c = b; c < e; ++c) {
+      if (!m[c_tgt_[c]]) continue;
+      src[o] = c_src_[c];
+      tgt[o] = c_tgt_[c];
+      port[o] = c_port_[c];
+      w[o] = c_w_[c];
+      delay[o] = dt_ * (c_d_[c] + 1);   // src/connect.cu:168
+      index[o] = c;
+      ++o;
+    }
+  }
+}
+
 // ---- recording ----------------------------------------------------------------
 int Engine::CreateRecord(const std::string& file, char** var_names, const int* nodes,
                          const int* ports, int n) {

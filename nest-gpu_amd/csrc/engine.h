@@ -201,6 +201,14 @@ class Engine {
   void GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned char* port,
                            unsigned char* syn, float* delay, float* w);
   long long NConnections() const { return (long long)c_src_.size(); }
+  // export of the synapse table (not part of the reference API): outgoing
+  // synapse counts, and the synapses into a set of targets in table order
+  // with their index in the source-major table (the Poisson stream id of a
+  // generator's synapse)
+  void OutDegrees(int i_node, int n, long long* out);
+  long long CountIncoming(const int* targets, int n);
+  void GetIncoming(const int* targets, int n, int* src, int* tgt, unsigned char* port, float* w,
+                   float* delay, long long* index);
 
   // ---- host random numbers (own Philox stream; src/random.cu counterpart)
   unsigned int* RandomInt(size_t n);

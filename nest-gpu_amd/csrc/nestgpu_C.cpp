@@ -632,6 +632,23 @@ int NESTGPU_AMD_Synchronize() {
   });
 }
 
+int NESTGPU_AMD_GetOutDegrees(int i_node, int n_node, long long* out) {
+  return guarded(0, [&] {
+    E().OutDegrees(i_node, n_node, out);
+    return 0;
+  });
+}
+long long NESTGPU_AMD_CountIncoming(const int* targets, int n_target) {
+  return guarded(0LL, [&] { return E().CountIncoming(targets, n_target); });
+}
+int NESTGPU_AMD_GetIncoming(const int* targets, int n_target, int* source, int* target,
+                            unsigned char* port, float* weight, float* delay, long long* index) {
+  return guarded(0, [&] {
+    E().GetIncoming(targets, n_target, source, target, port, weight, delay, index);
+    return 0;
+  });
+}
+
 int NESTGPU_AMD_SetStepPredictorMode(int mode) {
   return guarded(0, [&] {
     ngpu::Predictor::get().SetMode(mode);

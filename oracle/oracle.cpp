@@ -683,8 +683,12 @@ class Engine {
   }
 
   // src/connect.cu:38-54: d_int = round(delay/dt) - 1; delay < dt rejected.
+  // stream: for a poisson_generator's connection, the Poisson stream id to
+  // use instead of the connection's index in the sorted store (null: the
+  // index); lets a test replay part of a larger network with its draws
   void connect(long long n, const int* src, const int* tgt,
-               const unsigned char* port, const float* w, const float* delay) {
+               const unsigned char* port, const float* w, const float* delay,
+               const long long* stream = nullptr) {
     if (calibrated_) throw std::runtime_error("Connections cannot be created after calibration");
     for (long long i = 0; i < n; ++i) {
       if (delay[i] < dt_) throw std::runtime_error("Delay must be >= time resolution");
@@ -694,6 +698,7 @@ class Engine {
       c_port_.push_back(port[i]);
       c_w_.push_back(w[i]);
       c_d_.push_back(d_int);
+      c_stream_.push_back(stream ? stream[i] : -1);
     }
   }
 
@@ -822,12 +827,13 @@ class Engine {
       if (c_src_[a] != c_src_[b]) return c_src_[a] < c_src_[b];
       return c_d_[a] < c_d_[b];
     });
-    s_tgt_.resize(nc); s_port_.resize(nc); s_w_.resize(nc);
+    s_tgt_.resize(nc); s_port_.resize(nc); s_w_.resize(nc); s_stream_.resize(nc);
     grp_off_.assign(n_nodes_ + 1, 0);
     std::vector<int> src_sorted(nc), d_sorted(nc);
     for (size_t k = 0; k < nc; ++k) {
       long long o = order[k];
       s_tgt_[k] = c_tgt_[o]; s_port_[k] = c_port_[o]; s_w_[k] = c_w_[o];
+      s_stream_[k] = c_stream_[o];
       src_sorted[k] = c_src_[o]; d_sorted[k] = c_d_[o];
     }
     // delay groups per source
@@ -857,7 +863,7 @@ class Engine {
             dc.port = s_port_[c];
             dc.weight = s_w_[c];
             dc.delay = dt_ * (grp_delay_[gi] + 1);
-            dc.stream = (uint32_t)c;
+            dc.stream = s_stream_[c] >= 0 ? (uint32_t)s_stream_[c] : (uint32_t)c;
             g.dir.push_back(dc);
           }
         }
@@ -1276,6 +1282,7 @@ class Engine {
   std::vector<int> c_src_, c_tgt_, c_d_;
   std::vector<unsigned char> c_port_;
   std::vector<float> c_w_;
+  std::vector<long long> c_stream_, s_stream_;
   std::vector<int> s_tgt_;
   std::vector<unsigned char> s_port_;
   std::vector<float> s_w_;
@@ -1363,6 +1370,13 @@ int orc_connect(void* hv, long long n, const int* src, const int* tgt,
                 const unsigned char* port, const float* w, const float* delay) {
   OrcHandle* h = (OrcHandle*)hv;
   return guard(h, [&] { DISPATCH(h, connect(n, src, tgt, port, w, delay)); });
+}
+
+int orc_connect_stream(void* hv, long long n, const int* src, const int* tgt,
+                       const unsigned char* port, const float* w, const float* delay,
+                       const long long* stream) {
+  OrcHandle* h = (OrcHandle*)hv;
+  return guard(h, [&] { DISPATCH(h, connect(n, src, tgt, port, w, delay, stream)); });
 }
 
 int orc_record(void* hv, int node, const char* var, int port) {

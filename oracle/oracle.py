@@ -39,6 +39,8 @@ def lib():
         L.orc_set_spike_times.argtypes = [vp, ctypes.c_int, ctypes.c_int, fp, fp]
         L.orc_connect.argtypes = [vp, ctypes.c_longlong, ip, ip,
                                   ctypes.POINTER(ctypes.c_ubyte), fp, fp]
+        L.orc_connect_stream.argtypes = [vp, ctypes.c_longlong, ip, ip,
+                                         ctypes.POINTER(ctypes.c_ubyte), fp, fp, vp]
         L.orc_record.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.orc_simulate.argtypes = [vp, ctypes.c_float]
         L.orc_record_rows.restype = ctypes.c_longlong
@@ -120,19 +122,25 @@ class Oracle:
         self._check(self.L.orc_set_spike_times(self.h, node, len(t),
                                                t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), hp))
 
-    def connect(self, src, tgt, port, weight, delay):
+    def connect(self, src, tgt, port, weight, delay, stream=None):
+        """Connections src[k] -> tgt[k]; stream (optional): explicit Poisson
+        stream ids for the connections of poisson generators."""
         src = np.ascontiguousarray(src, dtype=np.int32)
         n = len(src)
         tgt = np.ascontiguousarray(np.broadcast_to(tgt, (n,)), dtype=np.int32)
         port = np.ascontiguousarray(np.broadcast_to(port, (n,)), dtype=np.uint8)
         w = _f32(np.broadcast_to(np.asarray(weight, dtype=np.float32), (n,)))
         d = _f32(np.broadcast_to(np.asarray(delay, dtype=np.float32), (n,)))
-        self._check(self.L.orc_connect(
-            self.h, n, src.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-            tgt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-            port.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
-            w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-            d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        args = [self.h, n, src.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                tgt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                port.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                d.ctypes.data_as(ctypes.POINTER(ctypes.c_float))]
+        if stream is None:
+            self._check(self.L.orc_connect(*args))
+        else:
+            st = np.ascontiguousarray(np.broadcast_to(stream, (n,)), dtype=np.int64)
+            self._check(self.L.orc_connect_stream(*args, st.ctypes.data_as(ctypes.c_void_p)))
 
     def record(self, node, var, port=0):
         return self._check(self.L.orc_record(self.h, node, var.encode(), port))

@@ -219,3 +219,63 @@ def test_exit_on_error_by_default():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "not reached" not in r.stdout
+
+
+def test_nodeseq_semantics(ngpu):
+    s = ngpu.NodeSeq(10, 5)
+    assert len(s) == 5 and s[0] == 10 and s[4] == 14 and s[-1] == 14 and s[-5] == 10
+    assert s.ToList() == [10, 11, 12, 13, 14] and list(s) == s.ToList()
+    sub = s[1:4]
+    assert (sub.i0, sub.n) == (11, 3)
+    assert (s.Subseq(0, -1).i0, s.Subseq(0, -1).n) == (10, 5)
+    for bad in (lambda: s[5], lambda: s[-6], lambda: s[0:5:2], lambda: s.Subseq(3, 1),
+                lambda: s.Subseq(0, 5)):
+        with pytest.raises(ValueError):
+            bad()
+    allseq = ngpu.NodeSeq(None)
+    assert (allseq.i0, allseq.n) == (0, -1)
+
+
+def test_status_shapes_and_dict_values(ngpu):
+    n = ngpu.Create("aeif_cond_beta_multisynapse", 3, 2, {"I_e": 5.0})
+    assert ngpu.GetStatus(n, "I_e") == [5.0, 5.0, 5.0]
+    # a port parameter: one list per node; a group parameter: one value per node
+    assert ngpu.GetStatus(n, "E_rev") == [[0.0, 0.0]] * 3
+    assert ngpu.GetStatus(n, "h_min_rel") == [pytest.approx(1e-3)] * 3
+    assert ngpu.GetStatus([n[0], n[2]], ["I_e", "V_th"]) == [[5.0, pytest.approx(-50.4)]] * 2
+    # integer variables read as one-element lists
+    ngpu.ActivateSpikeCount(n)
+    assert ngpu.GetStatus(n, "spike_count") == [[0], [0], [0]]
+    # per-node values from an array dictionary
+    ngpu.SetStatus(n, "V_m", {"array": [-60.0, -61.0, -62.0]})
+    assert ngpu.GetStatus(n, "V_m") == [-60.0, -61.0, -62.0]
+    with pytest.raises(ValueError):
+        ngpu.SetStatus(n, "V_m", {"array": [-60.0]})
+    with pytest.raises(ValueError):
+        ngpu.SetStatus(n, "V_m", {"distribution": "cauchy"})
+    # a distribution drawn by the library, clipped to its bounds
+    ngpu.SetStatus(n, "I_e", {"distribution": "normal_clipped", "mu": 0.0, "sigma": 5.0,
+                              "low": -1.0, "high": 1.0})
+    assert all(-1.0 <= v <= 1.0 for v in ngpu.GetStatus(n, "I_e"))
+    assert ngpu.GetKernelStatus("print_time") is False
+    ngpu.SetKernelStatus("print_time", True)
+    assert ngpu.GetKernelStatus("print_time") is True
+
+
+def test_connect_argument_errors(ngpu):
+    a = ngpu.Create("iaf_psc_exp", 4)
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, {"rule": "no_such_rule"}, {"weight": 1.0, "delay": 1.0})
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, {"rule": "all_to_all", "bogus": 1}, {"weight": 1.0, "delay": 1.0})
+    with pytest.raises(ValueError):
+        ngpu.Connect(a, a, {"rule": "all_to_all"}, {"bogus": 1.0})
+    with pytest.raises(ValueError):
+        ngpu.Connect(5, a, {"rule": "all_to_all"}, {"weight": 1.0, "delay": 1.0})
+    # weights from a distribution dictionary, one per connection
+    ngpu.Connect(a, a, {"rule": "all_to_all"},
+                 {"weight": {"distribution": "normal_clipped", "mu": 1.0, "sigma": 1.0,
+                             "low": 0.5, "high": 1.5}, "delay": 1.0})
+    w = [ngpu.GetStatus(c, "weight") for c in ngpu.GetConnections(a, a)]
+    assert len(w) == 16 and all(0.5 <= x <= 1.5 for x in w)
+    assert ngpu.RuleArraySize({"rule": "fixed_indegree", "indegree": 3}, a, a) == 12
