@@ -44,7 +44,8 @@ struct StepCtx {
   double t0;           // neural time at the start of Simulate()
   double t1;           // neural time at the end of this step
   int slot;            // ring slot folded at the end of this step (t_idx mod n_slot)
-  int parity;          // spike list written this step (t_idx & 1)
+  int parity;          // t_idx & 1 (hot-first flags)
+  int lst;             // spike list written this step (t_idx mod 3)
 };
 
 // Per-population parameter table: values of uniform parameters and the
@@ -57,15 +58,18 @@ struct ParamTable {
   }
 };
 
-// Spike list of one step (double buffered by t_idx parity).  A source with
-// bit 31 set (kSpikeLagBit) was emitted in the previous step by a neuron
-// whose step was split over two RK5 launches; it is delivered one slot
-// earlier, i.e. with its own emission time.
+// Spike list of one step; three lists rotate by t_idx mod 3, so the list of
+// step T can be delivered while step T+1 writes its own and the list of step
+// T-1 (delivered) is cleared for step T+2.  A source with bit 31 set
+// (kSpikeLagBit) was emitted in the previous step by a neuron whose step was
+// split over two RK5 launches; it is delivered one slot earlier, i.e. with
+// its own emission time.
 constexpr unsigned int kSpikeLagBit = 0x80000000u;
+constexpr int kSpikeLists = 3;
 struct SpikeList {
-  int* count;          // [2]
-  int* src;            // [2][cap] global node id (| kSpikeLagBit)
-  float* height;       // [2][cap]
+  int* count;          // [3]
+  int* src;            // [3][cap] global node id (| kSpikeLagBit)
+  float* height;       // [3][cap]
   int cap;
   int* overflow;       // number of spikes dropped because the list was full
 };

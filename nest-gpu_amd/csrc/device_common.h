@@ -35,20 +35,20 @@ __device__ inline void spike_book(const SpikeBook& sb, int node, int rel, float 
 }
 
 // Appends one source (node | kSpikeLagBit for a lagged spike) to the spike list.
-__device__ inline void spike_list_add(const SpikeList& sl, int parity, int src, float height) {
-  int pos = atomicAdd(&sl.count[parity], 1);
+__device__ inline void spike_list_add(const SpikeList& sl, int lst, int src, float height) {
+  int pos = atomicAdd(&sl.count[lst], 1);
   if (pos < sl.cap) {
-    sl.src[(size_t)parity * sl.cap + pos] = src;
-    sl.height[(size_t)parity * sl.cap + pos] = height;
+    sl.src[(size_t)lst * sl.cap + pos] = src;
+    sl.height[(size_t)lst * sl.cap + pos] = height;
   } else {
     atomicAdd(sl.overflow, 1);
   }
 }
 
 __device__ inline void emit_spike(const SpikeList& sl, const SpikeBook& sb,
-                                  int parity, int node, int rel, float height,
+                                  int lst, int node, int rel, float height,
                                   long long t_idx, double t1, bool lag = false) {
-  spike_list_add(sl, parity, lag ? (int)((unsigned int)node | kSpikeLagBit) : node, height);
+  spike_list_add(sl, lst, lag ? (int)((unsigned int)node | kSpikeLagBit) : node, height);
   spike_book(sb, node, rel, height, t_idx, t1);
 }
 
@@ -123,6 +123,62 @@ __device__ inline void add_wave_counters(unsigned long long* ctr, int slot, int 
     s[1] += c1;
     s[2] += c2;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Delivery of one spike over its source's synapse segment [b, e) (delay
+// groups ascending, src/connect.cu:56-74): each synapse adds height*weight
+// into ring slot base_slot + d_int (mod n_slot), base_slot being the slot of
+// d_int = 0, i.e. (emission time index + 1) mod n_slot.  The arrival step
+// t_emit + 1 + d_int is the step whose SpikeBufferUpdate sends the spike in
+// the reference (src/spike_buffer.cu:247-317, src/get_spike.cu:60-108).  FP64
+// hardware atomics accumulate (double)(height*weight), the reference's
+// accumulator type, so sums of float products are exact and independent of
+// order.  Threads tid, tid + nthr, ... of the caller take the synapses, four
+// records loaded before their atomics are issued.  mode 0: every synapse,
+// 1: only d_int = 0, 2: only d_int >= 1.  Returns the synapses delivered.
+constexpr int kDeliverUnroll = 4;
+
+__device__ inline unsigned long long deliver_segment(const SynTable& t, int base_slot, float hgt,
+                                                      long long b, long long e, int tid,
+                                                      int nthr, int mode) {
+  unsigned long long n_ev = 0;
+  const int stride = nthr * kDeliverUnroll;
+  for (long long k0 = b + tid; k0 < e; k0 += stride) {
+    unsigned int ch[kDeliverUnroll];
+    float w[kDeliverUnroll];
+    int d[kDeliverUnroll];
+#pragma unroll
+    for (int u = 0; u < kDeliverUnroll; ++u) {
+      const long long k = k0 + (long long)u * nthr;
+      if (k < e) {
+        ch[u] = t.syn_ch[k];
+        w[u] = t.syn_w[k];
+        d[u] = t.syn_d[k];
+      } else {
+        d[u] = -1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kDeliverUnroll; ++u) {
+      if (d[u] < 0 || (mode == 1 && d[u] != 0) || (mode == 2 && d[u] == 0)) continue;
+      int slot = base_slot + d[u];
+      if (slot >= t.n_slot) slot -= t.n_slot;
+      unsafeAtomicAdd(&t.ring[(size_t)slot * t.n_ch + ch[u]], (double)(hgt * w[u]));
+      ++n_ev;
+    }
+    // the d_int = 0 group is the segment's head: past it, nothing more
+    if (mode == 1 && __all(k0 >= e || d[kDeliverUnroll - 1] > 0 || d[kDeliverUnroll - 1] < 0)) break;
+  }
+  return n_ev;
+}
+
+// Slot of d_int = 0 for a spike list entry (source | lag bit) of the list of
+// the step with time index t_idx: (t_emit + 1) mod n_slot, t_emit = t_idx,
+// or t_idx - 1 for a lagged spike.
+__device__ inline int spike_base_slot(unsigned int raw, long long t_idx, int n_slot) {
+  int b = (int)((t_idx + 1) % n_slot) - (int)(raw >> 31);
+  return b < 0 ? b + n_slot : b;
 }
 
 }  // namespace ngpu

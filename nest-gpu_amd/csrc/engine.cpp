@@ -123,20 +123,23 @@ static const std::vector<ModelDesc>& model_table() {
        0, false, -1, false, -1},
       {"spike_detector", MODEL_SPIKE_DETECTOR, {"input_spike_height", "spike_height"}, {},
        {"hold_spike_height"}, {}, {}, {}, 1, false, 0, false, -1},
+      // src/parrot_neuron.cu:32-100: port 0 -> input_spike_height (weight 1),
+      // port 1 -> dummy_input (weight 0)
+      {"parrot_neuron", MODEL_PARROT, {"input_spike_height", "dummy_input", "V"}, {},
+       {"hold_spike_height", "den_delay"}, {}, {}, {}, 2, false, 0, false, -1, 1, false, 1},
   };
   return t;
 }
 
 // Reference model names outside this build's scope (src/neuron_models.h:60-86).
 static const char* kOtherModels[] = {
-    "iaf_psc_exp_g", "iaf_psc_exp_hc", "iaf_psc_alpha", "ext_neuron", "parrot_neuron",
-    "izhikevich", "izhikevich_psc_exp_5s", "izhikevich_psc_exp_2s", "izhikevich_psc_exp",
+    "iaf_psc_exp_g", "iaf_psc_exp_hc", "iaf_psc_alpha", "ext_neuron", "izhikevich", "izhikevich_psc_exp_5s", "izhikevich_psc_exp_2s", "izhikevich_psc_exp",
     "user_m1", "user_m2"};
 
 // Where a model's port input lands: variable index of port 0's input and the
 // stride between ports, parameter index of port 0's weight (-1: weight 1).
 struct FoldLayout {
-  int in_var, in_port_stride, w_param, w_port_stride;
+  int in_var, in_port_stride, w_param, w_port_stride, w_unit_ports;
 };
 static FoldLayout fold_layout(const ModelDesc& m) {
   FoldLayout f;
@@ -147,6 +150,7 @@ static FoldLayout fold_layout(const ModelDesc& m) {
     f.in_var = m.in_var;
     f.in_port_stride = m.in_scal_stride;
   }
+  f.w_unit_ports = m.w_unit_ports;
   if (m.w_param < 0) {
     f.w_param = -1;
     f.w_port_stride = 0;
@@ -196,6 +200,7 @@ StepCtx Engine::step_ctx(long long it) const {
   c.t1 = neur_t0_ + (double)dt_ * (double)(it + 1);
   c.slot = (int)(c.t_idx % n_slot_);
   c.parity = (int)(c.t_idx & 1);
+  c.lst = (int)(c.t_idx % kSpikeLists);
   return c;
 }
 
@@ -380,6 +385,9 @@ int Engine::Create(const std::string& model, int n, int n_port) {
       break;
     case MODEL_SPIKE_DETECTOR:
       P("hold_spike_height", 1.f);
+      break;
+    case MODEL_PARROT:   // src/parrot_neuron.cu:84-88
+      P("hold_spike_height", 0.f); P("den_delay", 0.f);
       break;
     default:
       break;
@@ -1568,11 +1576,11 @@ int Engine::Calibrate() {
   HIPCHK(hipMemsetAsync(d_ring_.p, 0, d_ring_.n * sizeof(double), stream_));
   // spike list: every node may emit in a step; headroom for repeated spikes
   spk_cap_ = std::max(2 * n_nodes_, 1024);
-  d_spk_count_.alloc(2);
+  d_spk_count_.alloc(kSpikeLists);
   d_overflow_.alloc(1);
-  d_spk_src_.alloc((size_t)2 * spk_cap_);
-  d_spk_h_.alloc((size_t)2 * spk_cap_);
-  HIPCHK(hipMemsetAsync(d_spk_count_.p, 0, 2 * sizeof(int), stream_));
+  d_spk_src_.alloc((size_t)kSpikeLists * spk_cap_);
+  d_spk_h_.alloc((size_t)kSpikeLists * spk_cap_);
+  HIPCHK(hipMemsetAsync(d_spk_count_.p, 0, kSpikeLists * sizeof(int), stream_));
   HIPCHK(hipMemsetAsync(d_overflow_.p, 0, sizeof(int), stream_));
   std::vector<long long> guard(n_nodes_, 0x70000000LL);   // src/spike_buffer.cu:37
   upload(d_last_idx_, guard, stream_);
@@ -1590,6 +1598,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_RK5_HOT_ONLY")) rk5_hot_only_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_FOLD_IN_RK5")) fold_in_rk5_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_FUSED_DELIVER")) fused_deliver_env_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
@@ -1728,6 +1737,22 @@ int Engine::Calibrate() {
     stage_var_[k].clear(); stage_var_[k].shrink_to_fit();
     stage_param_[k].clear(); stage_param_[k].shrink_to_fit();
   }
+  // fused delivery: the only spike-emitting population is a persistent RK5
+  // population with split steps, whose launch can deliver the previous
+  // step's list (spike generators, iaf_psc_exp and further RK5 populations
+  // keep the delivery launch)
+  {
+    int emitters = 0;
+    Group* rk = nullptr;
+    for (auto* g : groups_) {
+      const int id = g->md->id;
+      if (id == MODEL_POISSON_GEN || id == MODEL_SPIKE_DETECTOR) continue;
+      ++emitters;
+      if (g->md->rk5 && g->lag.p) rk = g;
+    }
+    fused_deliver_ = fused_deliver_env_ && emitters == 1 && rk != nullptr;
+    prev_list_pending_ = false;
+  }
   calibrated_ = true;
   neural_time_ = t_min_;
   return 0;
@@ -1782,6 +1807,7 @@ FoldArgs Engine::make_fold_args(const Group& g) {
   fa.param = g.param.p; fa.tab = make_table(g);
   fa.w_param = fl.w_param;
   fa.w_port_stride = fl.w_port_stride;
+  fa.w_unit_ports = fl.w_unit_ports;
   fa.ring = d_ring_.p; fa.n_ch = n_ch_;
   if (g.has_pe) {
     fa.pe_off = g.pe_off.p; fa.pe_gen = g.pe_gen.p; fa.pe_w = g.pe_w.p; fa.pe_delay = g.pe_delay.p;
@@ -1856,6 +1882,17 @@ void Engine::launch_step() {
             a.pf.ctx = step_ctx(it_ - 1);
           }
         }
+        if (fused_deliver_) {
+          a.fd.lag_d0_at_exit = 1;
+          a.fd.st = syn_table();
+          a.fd.spikes = d_stats_.p;
+          a.clear_next_list = 1;
+          if (prev_list_pending_) {
+            a.fd.enable = 1;
+            a.fd.prev_t_idx = sc.t_idx - 1;
+            a.fd.prev_lst = (int)((sc.t_idx - 1) % kSpikeLists);
+          }
+        }
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
       }
@@ -1878,6 +1915,13 @@ void Engine::launch_step() {
         timed(KT_OTHER, [&] { launch_detector(a, stream_); });
         break;
       }
+      case MODEL_PARROT: {
+        ParrotArgs a{};
+        a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.ctx = sc; a.sl = sl;
+        a.sb = sb;
+        timed(KT_OTHER, [&] { launch_parrot(a, stream_); });
+        break;
+      }
       default:
         break;
     }
@@ -1887,11 +1931,19 @@ void Engine::launch_step() {
                   rec_row_base_, rec_buf_rows_};
     timed(KT_OTHER, [&] { launch_record(ra, stream_); });
   }
-  DeliverArgs da{};
-  da.ctx = sc; da.sl = sl; da.row_ptr = d_row_ptr_.p; da.syn_ch = d_syn_ch_.p; da.syn_w = d_syn_w_.p;
-  da.syn_d = d_syn_d_.p; da.ring = d_ring_.p; da.n_ch = n_ch_;
-  da.n_slot = n_slot_; da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
-  timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
+  // delivery of this step's list: a launch of its own, or (fused) by the
+  // next step's RK5 launch, except after the last step of a Simulate call
+  // (or a step driven by SimulationStep), which leaves nothing pending
+  if (!fused_deliver_ || !defer_ok_) {
+    DeliverArgs da{};
+    da.ctx = sc; da.sl = sl; da.st = syn_table();
+    da.skip_lagged_d0 = fused_deliver_ ? 1 : 0;
+    da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
+    timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
+    prev_list_pending_ = false;
+  } else {
+    prev_list_pending_ = true;
+  }
   for (auto* g : groups_) {
     if (g->n_port == 0) continue;
     // a split-step RK5 population's fold is done by its next RK5 launch, as
@@ -1907,6 +1959,18 @@ void Engine::launch_step() {
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
   if (timing_ && ev_pending_.size() >= 4096) flush_timing();
+}
+
+SynTable Engine::syn_table() const {
+  SynTable t;
+  t.row_ptr = d_row_ptr_.p;
+  t.syn_ch = d_syn_ch_.p;
+  t.syn_w = d_syn_w_.p;
+  t.syn_d = d_syn_d_.p;
+  t.ring = d_ring_.p;
+  t.n_ch = n_ch_;
+  t.n_slot = n_slot_;
+  return t;
 }
 
 void Engine::timed(int cat, const std::function<void()>& launch, hipStream_t st) {

@@ -38,6 +38,7 @@ struct ModelDesc {
   int w_param;            // port-param index of the input weight, -1 => 1
   int in_scal_stride = 1; // scalar-block input: variable index step between ports (0: one shared variable)
   bool w_in_scal = false; // w_param is a scalar-param index, one per port in a row
+  int w_unit_ports = 1 << 30;  // w_param < 0: ports below this have weight 1, the rest 0
 };
 
 const ModelDesc* find_model(const std::string& name);
@@ -254,6 +255,7 @@ class Engine {
   void upload_poisson_tables();
   void launch_step();
   FoldArgs make_fold_args(const Group& g);
+  SynTable syn_table() const;
   void fetch_records(long long n_rows);
   void buffer_rec_spike_times(Group& g);
   int param_idx(const Group& g, const std::string& name, int port) const;
@@ -311,6 +313,9 @@ class Engine {
   bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
   int rk5_hot_lanes_ = 64;             // hot entries per wave
   bool fold_in_rk5_ = true;            // split-step RK5 populations fold inside the next RK5 launch
+  bool fused_deliver_env_ = true;      // allow the RK5 launch to deliver the previous step's spikes
+  bool fused_deliver_ = false;         // ... and the network qualifies (one emitting RK5 population)
+  bool prev_list_pending_ = false;     // the previous step's list awaits delivery by the next RK5 launch
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]

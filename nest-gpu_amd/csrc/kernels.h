@@ -30,6 +30,7 @@ enum ModelId {
   MODEL_POISSON_GEN,
   MODEL_SPIKE_GEN,
   MODEL_SPIKE_DETECTOR,
+  MODEL_PARROT,
   N_MODEL_ID
 };
 
@@ -44,8 +45,9 @@ struct FoldArgs {
   int in_port_stride;    // var index stride between ports
   const float* param;
   ParamTable tab;
-  int w_param;           // param index of port 0's weight, -1 => 1.0
-  int w_port_stride;
+  int w_param;           // param index of port 0's weight, -1 => constant weights:
+  int w_port_stride;     //   1.0 for ports below w_unit_ports, 0.0 above
+  int w_unit_ports;
   double* ring;
   long long n_ch;
   StepCtx ctx;           // this step's time (by value)
@@ -73,6 +75,31 @@ struct FoldArgs {
   // added; the neuron's next RK5 launch adds it when the step ends
   const unsigned char* lag;
   double* pend;
+};
+
+// Synapse table and input ring, as delivery reads them.
+struct SynTable {
+  const long long* row_ptr;    // [n_nodes+1] synapse segment per source
+  const unsigned int* syn_ch;  // [n_syn] target channel
+  const float* syn_w;          // [n_syn] weight
+  const unsigned short* syn_d; // [n_syn] delay slot d_int (ascending within a segment)
+  double* ring;                // [n_slot][n_ch]
+  long long n_ch;
+  int n_slot;
+};
+
+// Delivery done by the persistent RK5 kernel (one spike-emitting population):
+// its idle waves deliver the previous step's spike list (the step's fold of
+// that list's earliest slot comes one launch later), and each wave delivers
+// the d_int = 0 synapses of its own lagged spikes when it exits (their slot
+// is folded by the next launch).
+struct FusedDeliver {
+  int enable;            // deliver the previous step's list in this launch
+  int lag_d0_at_exit;    // lagged spikes: d_int = 0 synapses at wave exit, the rest later
+  int prev_lst;          // list index of the previous step
+  long long prev_t_idx;  // its time index
+  SynTable st;
+  unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 
 struct RK5Args {
@@ -123,6 +150,8 @@ struct RK5Args {
   // separate fold launch of that step was skipped
   int prefold;
   FoldArgs pf;
+  FusedDeliver fd;
+  int clear_next_list;   // clear the spike count of list (lst + 1) mod 3 (fused delivery)
 };
 
 // Whether the persistent RK5 kernel's compile-time input layout of a model
@@ -159,16 +188,20 @@ struct DetectorArgs {
   float* param;
 };
 
+struct ParrotArgs {
+  int n, i0;
+  float* var;            // [input_spike_height, dummy_input, V] x n
+  const float* param;    // [hold_spike_height, den_delay] x n
+  StepCtx ctx;           // this step's time (by value)
+  SpikeList sl;
+  SpikeBook sb;
+};
+
 struct DeliverArgs {
   StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
-  const long long* row_ptr;    // [n_nodes+1] synapse segment per source
-  const unsigned int* syn_ch;  // [n_syn] target channel
-  const float* syn_w;          // [n_syn] weight
-  const unsigned short* syn_d; // [n_syn] delay slot d_int
-  double* ring;                // [n_slot][n_ch]
-  long long n_ch;
-  int n_slot;
+  SynTable st;
+  int skip_lagged_d0;          // the d_int = 0 synapses of lagged spikes were delivered already
   unsigned long long* ctr;     // per-wave counters [slot][kCtrCols] (column 3) or null
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
@@ -215,6 +248,7 @@ int rk5_predictor_capable(int model, int n_port);                // model suppor
 void launch_iaf(const IafArgs& a, hipStream_t s);
 void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);
+void launch_parrot(const ParrotArgs& a, hipStream_t s);
 void launch_deliver(const DeliverArgs& a, hipStream_t s);   // kDeliverBlocks x 256
 void launch_fold(const FoldArgs& a, hipStream_t s);
 void launch_record(const RecordArgs& a, hipStream_t s);

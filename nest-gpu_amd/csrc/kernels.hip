@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) iaf_kernel(IafArgs a) {
   iex *= P(P11ex);
   iin *= P(P11in);
   if (v >= P(Theta)) {
-    emit_spike(a.sl, a.sb, a.ctx.parity, a.i0 + i, i, 1.0f, a.ctx.t_idx, a.ctx.t1);
+    emit_spike(a.sl, a.sb, a.ctx.lst, a.i0 + i, i, 1.0f, a.ctx.t_idx, a.ctx.t1);
     v = P(V_reset);
     ref = (float)(int)roundf(P(t_ref) / a.dt);
   }
@@ -147,7 +147,7 @@ __global__ void spike_gen_kernel(SpikeGenArgs a) {
   if (b == e) return;
   const int is = b + a.next[i];
   if (is < e && (long long)a.t_idx[is] == a.ctx.it) {
-    emit_spike(a.sl, a.sb, a.ctx.parity, a.i0 + i, i, a.height[is], a.ctx.t_idx, a.ctx.t1);
+    emit_spike(a.sl, a.sb, a.ctx.lst, a.i0 + i, i, a.height[is], a.ctx.t_idx, a.ctx.t1);
     a.next[i]++;
   }
 }
@@ -174,71 +174,54 @@ void launch_detector(const DetectorArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(detector_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
 }
 
-// ---------------------------------------------------------------------------
-// Delivery: every spike of this step is expanded over its source's synapse
-// segment (source-major, delay groups ascending) by one 256-lane block; each
-// synapse adds height*weight into ring slot (t_idx+1+d_int) mod n_slot at its
-// channel.  Replaces the reference's per-node spike ring + send list +
-// per-(spike, group) blocks (src/spike_buffer.cu:247-317,
-// src/get_spike.cu:60-108): the arrival step t_idx+1+d_int is the step whose
-// SpikeBufferUpdate would send the spike in the reference.  FP64 hardware
-// atomics accumulate (double)(height*weight), the reference's accumulator
-// type, so sums of float products are exact and order-independent.  Each
-// lane loads the synapse records of up to 4 synapses before issuing their
-// atomics, so a 1000-synapse segment is one round of loads and adds.
-constexpr int kDeliverUnroll = 4;
+// parrot_neuron, src/parrot_neuron.cu:55-81: V shows the input of the last
+// fold; a non-zero input is re-emitted (height 1 unless hold_spike_height)
+// and cleared.
+__global__ void parrot_kernel(ParrotArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float sh = a.var[i];                       // input_spike_height
+  a.var[(size_t)2 * a.n + i] = sh;           // V
+  if (sh != 0.0f) {
+    if (a.param[i] == 0.0f) sh = 1.0f;       // hold_spike_height
+    a.var[i] = 0.0f;
+    emit_spike(a.sl, a.sb, a.ctx.lst, a.i0 + i, i, sh, a.ctx.t_idx, a.ctx.t1);
+  }
+}
 
+void launch_parrot(const ParrotArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(parrot_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Delivery launch: every spike of this step's list is expanded over its
+// source's synapse segment by one 256-lane block (deliver_segment).  Replaces
+// the reference's per-node spike ring + send list + per-(spike, group)
+// blocks (src/spike_buffer.cu:247-317, src/get_spike.cu:60-108).
 __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
-  const int parity = a.ctx.parity;
-  int n_spk = a.sl.count[parity];
+  const int lst = a.ctx.lst;
+  int n_spk = a.sl.count[lst];
   if (n_spk > a.sl.cap) n_spk = a.sl.cap;
-  const int base = (int)((a.ctx.t_idx + 1) % a.n_slot);
-  const int* srcs = a.sl.src + (size_t)parity * a.sl.cap;
-  const float* hts = a.sl.height + (size_t)parity * a.sl.cap;
-  const int stride = blockDim.x * kDeliverUnroll;
+  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
+  const float* hts = a.sl.height + (size_t)lst * a.sl.cap;
   unsigned long long n_ev = 0;
   for (int s = blockIdx.x; s < n_spk; s += gridDim.x) {
-    // a lagged spike (emitted in the previous step) arrives one slot earlier
     const unsigned int raw = (unsigned int)srcs[s];
     const int src = (int)(raw & ~kSpikeLagBit);
-    int sbase = base - (int)(raw >> 31);
-    if (sbase < 0) sbase += a.n_slot;
-    const float hgt = hts[s];
-    const long long b = a.row_ptr[src], e = a.row_ptr[src + 1];
-    n_ev += (unsigned long long)(e - b);
-    for (long long k0 = b + threadIdx.x; k0 < e; k0 += stride) {
-      unsigned int ch[kDeliverUnroll];
-      float w[kDeliverUnroll];
-      int d[kDeliverUnroll];
-#pragma unroll
-      for (int u = 0; u < kDeliverUnroll; ++u) {
-        const long long k = k0 + (long long)u * blockDim.x;
-        if (k < e) {
-          ch[u] = a.syn_ch[k];
-          w[u] = a.syn_w[k];
-          d[u] = a.syn_d[k];
-        } else {
-          d[u] = -1;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kDeliverUnroll; ++u) {
-        if (d[u] < 0) continue;
-        int slot = sbase + d[u];
-        if (slot >= a.n_slot) slot -= a.n_slot;
-        unsafeAtomicAdd(&a.ring[(size_t)slot * a.n_ch + ch[u]], (double)(hgt * w[u]));
-      }
-    }
+    const int mode = (a.skip_lagged_d0 && (raw >> 31)) ? 2 : 0;
+    n_ev += deliver_segment(a.st, spike_base_slot(raw, a.ctx.t_idx, a.st.n_slot), hts[s],
+                            a.st.row_ptr[src], a.st.row_ptr[src + 1], threadIdx.x, blockDim.x,
+                            mode);
   }
-  if (threadIdx.x == 0) {
-    if (a.ctr && n_ev)
-      a.ctr[(size_t)blockIdx.x * (blockDim.x / kWave) * kCtrCols + 3] += n_ev;
-    if (blockIdx.x == 0) {
-      if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
-      // the next step writes the other list: clear its counter now (it was
-      // consumed by the previous step's delivery)
-      a.sl.count[parity ^ 1] = 0;
-    }
+  n_ev = wave_sum(n_ev);
+  if ((threadIdx.x & (kWave - 1)) == 0 && a.ctr && n_ev)
+    a.ctr[((size_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kCtrCols + 3] += n_ev;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
+    // the next step writes list (lst + 1) mod 3: clear its count now (it held
+    // the list of step t_idx - 2, delivered two steps ago)
+    a.sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
   }
 }
 
@@ -278,7 +261,7 @@ __global__ void __launch_bounds__(256) fold_kernel(FoldArgs a) {
     acc[p] = *rp[p];
     in[p] = a.var + (size_t)(a.in_var + p * a.in_port_stride) * a.n + i;
     in0[p] = *in[p];
-    w[p] = 1.0f;
+    w[p] = p < a.w_unit_ports ? 1.0f : 0.0f;
     if (a.w_param >= 0) {
       const int j = a.w_param + p * a.w_port_stride;
       w[p] = a.tab.is_per_neuron(j) ? a.param[(size_t)j * a.n + i] : a.tab.val[j];

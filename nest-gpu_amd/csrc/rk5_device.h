@@ -104,6 +104,8 @@ __device__ inline void rhs(const float* y, float* dy, const float* p, const floa
   }
 }
 
+__device__ inline bool lagged_spike(int raw) { return ((unsigned int)raw >> 31) != 0; }
+
 // ---------------------------------------------------------------------------
 // One Cash-Karp attempt of step h from (y, k1), src/rk5.h:88-131: the five
 // further stages, the 5th-order solution yn and the error ratio
@@ -182,7 +184,6 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
     float h = a.h[i];
     const double x1 = a.ctx.t1;
     const long long t_idx = a.ctx.t_idx;
-    const int parity = a.ctx.parity;
     const float h_min = a.h_min;
     const float ref_in = p[M::i_ref];
 
@@ -235,7 +236,7 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
         y[M::iV] = p[M::i_reset];
         if (end_step) p[M::i_ref] -= 1.0f;
       } else if (y[M::iV] >= p[M::i_thr]) {
-        emit_spike(a.sl, a.sb, parity, a.i0 + i, i, 1.0f, t_idx, x1);
+        emit_spike(a.sl, a.sb, a.ctx.lst, a.i0 + i, i, 1.0f, t_idx, x1);
         y[M::iV] = p[M::i_reset];
         y[M::iw] += p[M::i_jump];
         float r = (float)(int)roundf(p[M::i_tref] / a.dt);
@@ -282,6 +283,7 @@ rk5_persistent_kernel(RK5Args a) {
   const double x1 = a.ctx.t1;
   const long long t_idx = a.ctx.t_idx;
   const int parity = a.ctx.parity;
+  const int lst = a.ctx.lst;
   // end of the previous step (the end_step_kernel formula one step back): a
   // lagged neuron integrates up to it before starting this step
   const double x1_prev = a.ctx.t0 + (double)a.dt * (double)a.ctx.it;
@@ -307,6 +309,11 @@ rk5_persistent_kernel(RK5Args a) {
     flag_out = a.hot_flag + (size_t)(parity ^ 1) * n;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.hot_count[q_clr] = 0;
   }
+  // with fused delivery no delivery launch clears the spike list the next
+  // step writes: it held the list of step t_idx - 2, delivered by the
+  // previous launch
+  if (a.clear_next_list && blockIdx.x == 0 && threadIdx.x == 0)
+    a.sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
   // wave w takes hot entries [w*hp, w*hp + hp) (one per lane, so the long
   // attempt chains start together in the first iteration, packed into few
   // waves), then its equal share of all neurons
@@ -365,6 +372,7 @@ rk5_persistent_kernel(RK5Args a) {
     __syncthreads();
   }
   unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
+  unsigned long long n_dev = 0;   // synapses delivered by this lane (fused delivery)
   unsigned int n_split = 0;
 #pragma unroll
   for (int j = 0; j < M::NPAR; ++j) {
@@ -603,8 +611,18 @@ rk5_persistent_kernel(RK5Args a) {
       const unsigned long long ms = __ballot(spk);
       if (spk) {
         const int pos = scnt + __popcll(ms & ((1ull << lane) - 1ull));
-        if (pos < kSpkLds) my_spk[pos] = spk_src;
-        else spike_list_add(a.sl, parity, spk_src, 1.0f);
+        if (pos < kSpkLds) {
+          my_spk[pos] = spk_src;
+        } else {
+          spike_list_add(a.sl, lst, spk_src, 1.0f);
+          if (a.fd.lag_d0_at_exit && lagged_spike(spk_src)) {
+            // its d_int = 0 synapses now (the next launch folds their slot)
+            const int src = (int)((unsigned int)spk_src & ~kSpikeLagBit);
+            n_dev += deliver_segment(a.fd.st, spike_base_slot((unsigned int)spk_src, t_idx,
+                                                              a.fd.st.n_slot),
+                                     1.0f, a.fd.st.row_ptr[src], a.fd.st.row_ptr[src + 1], 0, 1, 1);
+          }
+        }
         spk = false;
       }
       scnt += __popcll(ms);
@@ -633,18 +651,55 @@ rk5_persistent_kernel(RK5Args a) {
     const int listed = scnt < kSpkLds ? scnt : kSpkLds;
     if (listed > 0) {
       int base = 0;
-      if (lane == 0) base = atomicAdd(a.sl.count + parity, listed);
+      if (lane == 0) base = atomicAdd(a.sl.count + lst, listed);
       base = __shfl(base, 0, kWave);
       for (int e = lane; e < listed; e += kWave) {
         const int pos = base + e;
         if (pos < a.sl.cap) {
-          a.sl.src[(size_t)parity * a.sl.cap + pos] = my_spk[e];
-          a.sl.height[(size_t)parity * a.sl.cap + pos] = 1.0f;
+          a.sl.src[(size_t)lst * a.sl.cap + pos] = my_spk[e];
+          a.sl.height[(size_t)lst * a.sl.cap + pos] = 1.0f;
         } else {
           atomicAdd(a.sl.overflow, 1);
         }
       }
+      if (a.fd.lag_d0_at_exit) {
+        // the d_int = 0 synapses of the wave's lagged spikes: their slot
+        // (t_idx) is folded by the next launch; the other synapses are
+        // delivered with the list by the next launch's idle waves
+        for (int e = 0; e < listed; ++e) {
+          const int raw = my_spk[e];
+          if (!lagged_spike(raw)) continue;
+          const int src = (int)((unsigned int)raw & ~kSpikeLagBit);
+          n_dev += deliver_segment(a.fd.st, spike_base_slot((unsigned int)raw, t_idx,
+                                                            a.fd.st.n_slot),
+                                   1.0f, a.fd.st.row_ptr[src], a.fd.st.row_ptr[src + 1], lane,
+                                   kWave, 1);
+        }
+      }
     }
+  }
+  if (a.fd.enable) {
+    // the previous step's spike list, one spike per wave at a time, taken
+    // from the last wave down so that the first waves (the hot ones, which
+    // end last) take none while there are fewer spikes than waves
+    const int pl = a.fd.prev_lst;
+    int n_prev = a.sl.count[pl];
+    if (n_prev > a.sl.cap) n_prev = a.sl.cap;
+    const int* srcs = a.sl.src + (size_t)pl * a.sl.cap;
+    const float* hts = a.sl.height + (size_t)pl * a.sl.cap;
+    for (int sp = n_waves - 1 - wave; sp < n_prev; sp += n_waves) {
+      const unsigned int raw = (unsigned int)srcs[sp];
+      const int src = (int)(raw & ~kSpikeLagBit);
+      const int mode = (a.fd.lag_d0_at_exit && (raw >> 31)) ? 2 : 0;
+      n_dev += deliver_segment(a.fd.st, spike_base_slot(raw, a.fd.prev_t_idx, a.fd.st.n_slot),
+                               hts[sp], a.fd.st.row_ptr[src], a.fd.st.row_ptr[src + 1], lane,
+                               kWave, mode);
+    }
+    if (wave == 0 && lane == 0 && a.fd.spikes && n_prev) *a.fd.spikes += (unsigned long long)n_prev;
+  }
+  if (a.ctr) {
+    n_dev = wave_sum(n_dev);
+    if (lane == 0 && n_dev) a.ctr[(size_t)wave * kCtrCols + 3] += n_dev;
   }
   if (a.hot_flag) {
     // append the wave's list to the next step's (one atomic per wave; the

@@ -59,6 +59,7 @@ enum Model {
   M_POISSON_GEN,
   M_SPIKE_GEN,
   M_SPIKE_DETECTOR,
+  M_PARROT,
   N_MODEL
 };
 
@@ -73,6 +74,7 @@ struct ModelInfo {
   int weight_param;      // port param index of the input weight, -1 => 1.0
   int input_scal_step = 1;    // scalar input var: index step between ports (0: shared)
   bool weight_in_scal = false; // weight_param indexes the scalar block, one per port
+  int weight_unit_ports = 1 << 30;  // weight_param < 0: weight 1 below this port, 0 from it
 };
 
 static std::vector<std::string> adex_tail() {
@@ -158,6 +160,10 @@ static const ModelInfo& info(int m) {
       {"spike_generator", {}, {}, {}, {}, 0, false, -1, false, -1},
       {"spike_detector", {"input_spike_height", "spike_height"}, {},
        {"hold_spike_height"}, {}, 1, false, 0, false, -1},
+      // src/parrot_neuron.cu:32-100: port 0 -> input_spike_height (weight 1),
+      // port 1 -> dummy_input (weight 0)
+      {"parrot_neuron", {"input_spike_height", "dummy_input", "V"}, {},
+       {"hold_spike_height", "den_delay"}, {}, 2, false, 0, false, -1, 1, false, 1},
   };
   return tab.at(m);
 }
@@ -1086,6 +1092,18 @@ class Engine {
         }
         y[1] = sh;
       }
+    } else if (g.model == M_PARROT) {   // src/parrot_neuron.cu:55-81
+      for (int r = 0; r < g.n; ++r) {
+        T* y = &g.y[(size_t)r * g.nv];
+        const T* p = &g.p[(size_t)r * g.np_];
+        T sh = y[0];      // input_spike_height
+        y[2] = sh;        // V
+        if (sh != 0.0) {
+          if (p[0] == 0.0) sh = 1.0;   // hold_spike_height
+          y[0] = 0;
+          push_spike(g.i0 + r, (float)sh);
+        }
+      }
     }
   }
 
@@ -1236,7 +1254,7 @@ class Engine {
         int iv = mi.input_in_port_block ? nsv + q * npv + mi.input_var
                                         : mi.input_var + q * mi.input_scal_step;
         T* in = &g.y[(size_t)r * g.nv + iv];
-        double wgt = 1.0;
+        double wgt = q < mi.weight_unit_ports ? 1.0 : 0.0;
         if (mi.weight_param >= 0) {
           int ip = mi.weight_in_scal ? mi.weight_param + q : nsp + q * npp + mi.weight_param;
           wgt = (double)g.p[(size_t)r * g.np_ + ip];

@@ -557,3 +557,64 @@ def test_fold_in_rk5_matches_fold_launch(ngpu):
     np.testing.assert_array_equal(v1, v0)
     np.testing.assert_array_equal(g11, g10)
     assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
+
+
+@pytest.mark.parametrize("cap", ["3", "24"])
+def test_fused_delivery_matches_delivery_launch(ngpu, cap):
+    # delivery of a step's spikes by the next RK5 launch's idle waves (the
+    # d_int = 0 synapses of lagged spikes at the emitting wave's exit) against
+    # a delivery launch after every step: with split steps, delays from 0.1 ms
+    # and three Simulate() calls (each call's last step delivers by launch,
+    # the next call's first launch delivers nothing), rasters, state,
+    # attempts and delivered event counts are equal
+    net = netbuild.c1_network(seed=13)
+    net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2], np.full_like(net["exc"][3], 0.1))
+    res = []
+    for fused in ("1", "0"):
+        os.environ["NGPU_RK5_DEFER"] = cap
+        os.environ["NGPU_FUSED_DELIVER"] = fused
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            nrn = ngpu.Create("aeif_cond_beta_multisynapse", net["n"], 2)
+            ngpu.SetStatus(nrn, netbuild.C1_RECEPTORS)
+            pg = ngpu.Create("poisson_generator")
+            ngpu.SetStatus(pg, "rate", net["rate"])
+            ngpu.Connect(pg, nrn, {"rule": "all_to_all"},
+                         {"weight": netbuild.C1_POISSON_W, "delay": 0.1, "receptor": 0})
+            se, te, we, de = net["exc"]
+            si, ti, wi, di = net["inh"]
+            netbuild.connect_edges_fast(ngpu, se + nrn[0], te + nrn[0], 0, we, de)
+            netbuild.connect_edges_fast(ngpu, si + nrn[0], ti + nrn[0], 1, wi, di)
+            ngpu.ActivateRecSpikeTimes(nrn, 5000)
+            for ms in (20.0, 25.0, 15.0):
+                ngpu.Simulate(ms)
+            st = [list(x) for x in ngpu.GetRecSpikeTimes(nrn)]
+            v = np.array(ngpu.GetStatus(nrn, "V_m"))
+            g1 = np.array(ngpu.GetStatus(nrn, "g1"))
+            res.append((st, v, g1, ngpu.GetStepStats()))
+        finally:
+            os.environ.pop("NGPU_RK5_DEFER", None)
+            os.environ.pop("NGPU_FUSED_DELIVER", None)
+    (st1, v1, g11, s1), (st0, v0, g10, s0) = res
+    n_spk = sum(len(x) for x in st1)
+    print("cap", cap, "split steps", s1["split_steps"], "spikes", n_spk, "events", s1["syn_events"])
+    assert n_spk > 50 and (cap != "3" or s1["split_steps"] > 500)
+    assert st1 == st0
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(g11, g10)
+    for k in ("attempts", "rejected", "syn_events", "spikes", "dropped_spikes"):
+        assert s1[k] == s0[k], k
+
+
+@pytest.mark.parametrize("hold", [0.0, 1.0])
+def test_parrot_neuron_matches_oracle(ngpu, hold):
+    # src/parrot_neuron.cu: the chain generator -> parrot -> detector, records
+    # identical to the oracle's (whose timeline tests/test_oracle_pins.py
+    # pins by hand)
+    from test_oracle_pins import _parrot_chain
+    vg, hg = _parrot_chain(hold, ngpu)
+    vo, ho = _parrot_chain(hold, None)
+    np.testing.assert_array_equal(vg, vo)
+    np.testing.assert_array_equal(hg, ho)
+    assert np.count_nonzero(hg) == 1

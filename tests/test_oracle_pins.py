@@ -176,3 +176,46 @@ def test_poisson_window_defaults_and_empty():
     h = _detector_with_poisson(2.0e5, 3.0, 0.0, 2.0, 0.1, 10.0)
     nz = np.nonzero(h)[0]
     assert nz[0] == 32 and nz[-1] == 51 and len(nz) == 20
+
+
+def _parrot_chain(hold, backend):
+    """spike_generator (t = 1.0 ms, height 2.5) -> parrot (delay 0.5, weight
+    1) -> spike_detector (delay 0.1); records parrot V and detector
+    spike_height."""
+    if backend is None:
+        o = Oracle(prec=0)
+        sg = o.create("spike_generator", 1, 0)
+        o.set_spike_times(sg, [1.0], [2.5])
+        par = o.create("parrot_neuron", 1, 2)
+        o.set(par, 1, "hold_spike_height", [hold])
+        det = o.create("spike_detector", 1, 1)
+        o.connect([sg], [par], [0], [1.0], [0.5])
+        o.connect([par], [det], [0], [1.0], [0.1])
+        rv, rs = o.record(par, "V"), o.record(det, "spike_height")
+        o.simulate(4.0)
+        return o.record_data(rv)[1], o.record_data(rs)[1]
+    ngpu = backend
+    sg = ngpu.Create("spike_generator")
+    ngpu.SetStatus(sg, {"spike_times": [1.0], "spike_heights": [2.5]})
+    par = ngpu.Create("parrot_neuron")
+    ngpu.SetStatus(par, {"hold_spike_height": hold})
+    det = ngpu.Create("spike_detector")
+    ngpu.Connect(sg, par, {"rule": "one_to_one"}, {"weight": 1.0, "delay": 0.5})
+    ngpu.Connect(par, det, {"rule": "one_to_one"}, {"weight": 1.0, "delay": 0.1})
+    rec = ngpu.CreateRecord("", ["V", "spike_height"], [par[0], det[0]], [0, 0])
+    ngpu.Simulate(4.0)
+    d = np.array(ngpu.GetRecordData(rec), np.float32)
+    return d[:, 1], d[:, 2]
+
+
+@pytest.mark.parametrize("hold", [0.0, 1.0])
+def test_parrot_neuron_known_answer(hold):
+    # the generator emits in the step with time index 11 (spike time 1.0 ms,
+    # src/spike_generator.cu:49-67); with d_int = 4 the input arrives in step
+    # 16 and is folded at its end; the parrot reads it in step 17 (V = 2.5)
+    # and re-emits (height 1, or 2.5 with hold_spike_height); d_int = 0: the
+    # detector's input is folded at the end of step 18 and read in step 19
+    v, h = _parrot_chain(hold, None)
+    assert np.nonzero(v)[0].tolist() == [17] and v[17] == np.float32(2.5)
+    assert np.nonzero(h)[0].tolist() == [19]
+    assert h[19] == np.float32(2.5 if hold else 1.0)
