@@ -203,7 +203,10 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
                              "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
                              "bytes_per_launch": b / cnt}
     ms, cnt = kt["deliver"]
-    if cnt:
+    if cnt and cnt < steps // 2:
+        out["deliver"] = {"note": "delivery runs inside the update launches (fused); the %d "
+                                  "delivery launch(es) only close Simulate calls" % cnt}
+    elif cnt:
         b = st["syn_events"] * DELIVER_BYTES_PER_EVENT
         a = b / (ms * 1e-3) / 1e9
         out["deliver"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
@@ -249,6 +252,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=20000)
     ap.add_argument("--cpu-ms", type=float, default=1000.0)
+    ap.add_argument("--tune-steps", type=int, default=20,
+                    help="batching workload: steps per autotuner candidate (0: no autotuning)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB fill and a 1 GiB elementwise multiply (torch) whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figures")
@@ -258,7 +263,9 @@ def main():
               "scope); run with --gpus 1", file=sys.stderr)
         sys.exit(0)
     mc = args.workload == "microcircuit"
-    seed = args.seed if args.seed is not None else {"scale_adex": 12345, "microcircuit": 7}.get(args.workload, 200)
+    bt = args.workload == "batching"
+    seed = args.seed if args.seed is not None else {"scale_adex": 12345, "microcircuit": 7,
+                                                    "batching": 300}.get(args.workload, 200)
     pred_path = args.predictor
     if pred_path is None:
         p = os.path.join(ROOT, "nest-gpu_amd", "predictors", args.workload + ".json")
@@ -280,13 +287,15 @@ def main():
     import nestgpu as ngpu
     ngpu.SetVerbosityLevel(0)
     ngpu.SetStepPredictorMode("plain")
-    n = sum(workloads.MC_SIZES) if mc else args.neurons
+    n = (sum(workloads.MC_SIZES) if mc else
+         sum(p[1] for p in workloads.BATCH_POPS) if bt else args.neurons)
     n_port = 4 if args.workload == "predictor_adex" else 2
     n_poisson_edges = n          # Poisson edges built (one per neuron)
     n_poisson_csr_edges = 0      # of which kept as CSR (each workload's are one run per population)
     t0 = time.perf_counter()
     neurons = workloads.WORKLOADS[args.workload](ngpu, n, seed)
-    n_syn = workloads.microcircuit.n_syn if mc else n * 1000
+    n_syn = (workloads.microcircuit.n_syn if mc else workloads.batching.n_syn if bt else
+             n * 1000)
     log("built %s: %d neurons, %d synapses + %d Poisson edges in %.1f s"
         % (args.workload, n, n_syn, n, time.perf_counter() - t0))
     # no RK5 in the microcircuit: its timed windows bracket every launch
@@ -303,6 +312,20 @@ def main():
     ngpu.Synchronize()
     log("settled: %d untimed steps (%.0f ms bio)" % (settle, settle * DT))
     t_plain0 = settle * DT
+    batch = None
+    if bt:
+        batch = {"config_default": ngpu.GetBatchConfig()}
+        if args.tune_steps > 0:
+            t0 = time.perf_counter()
+            best_ms, ncand = ngpu.AutotuneBatch(args.tune_steps)
+            batch["autotune"] = {"candidates": ncand, "steps_per_candidate": args.tune_steps,
+                                 "best_ms_per_step": best_ms,
+                                 "wall_s": time.perf_counter() - t0}
+            t_plain0 += ncand * args.tune_steps * DT
+            log("autotuned the batch kernel: %d candidates x %d steps, best %.4f ms per step"
+                % (ncand, args.tune_steps, best_ms))
+        batch["config"] = ngpu.GetBatchConfig()
+        log("batch configuration: %s" % batch["config"])
     wp = timed_window(ngpu, args.steps, timing=win_timing)
     sp = window_summary(wp, args.steps, n)
     log("plain: %d steps, wall %.3f s, RTF %.4f, events %d, kernel ms %s"
@@ -375,7 +398,9 @@ def main():
     # the committed FETCH_SIZE/WRITE_SIZE summary of this workload, if any
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
     roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges).get(dom, roof_all.get(dom, {})))
-    kname = {"rk5_update": "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port,
+    rk5_kname = ("rk5_batch_kernel<%d>" % (3 + batch["config"]["variant"]) if bt else
+                 "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port)
+    kname = {"rk5_update": rk5_kname,
              "deliver": "deliver_kernel", "fold_poisson": "fold_kernel<%d>" % n_port,
              "iaf_update": "iaf_kernel"}.get(dom, dom)
     roof["kernel"] = kname
@@ -409,7 +434,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": ("%s: iaf_psc_exp %d neurons in 8 populations, fixed_total_number per "
+        "config": {"workload": ("%s: %d neurons in six populations of five models (300 + 3000 "
+                                "izhikevich_cond_beta, 30000 aeif_psc_exp, 120000 "
+                                "aeif_cond_alpha_multisynapse with 4 ports, 300000 "
+                                "aeif_cond_beta_multisynapse with 2 ports, 50000 iaf_psc_exp), fixed "
+                                "indegree 500 across populations (%.3g synapses), normal weights and "
+                                "delays, Poisson input; one batch launch per step; timed %d steps = "
+                                "%.1f ms bio after %.0f ms bio settling and autotuning"
+                                % (args.workload, n, n_syn, args.steps, args.steps * DT,
+                                   t_plain0)) if bt else
+                               ("%s: iaf_psc_exp %d neurons in 8 populations, fixed_total_number per "
                                 "pair (p ~ U(0.01,0.15)), %.3g synapses, normal weights and delays, "
                                 "Poisson 8 Hz x K_ext, exact propagator; timed %d steps = %.1f ms bio after "
                                 "%.0f ms bio settling" % (args.workload, n, n_syn, args.steps,
@@ -421,6 +455,7 @@ def main():
                                    settle * DT)),
                    "neurons": n, "synapses": n_syn, "dt_ms": DT, "seed": seed,
                    "method": "exact iaf_psc_exp update" if mc else
+                             "plain RK5 + exact iaf_psc_exp in one batch launch per step" if bt else
                              "plain RK5 (value); learned step-size proposal in 'learned'"},
         "synaptic_events_per_s": sp["synaptic_events_per_s"],
         "neuron_updates_per_s": sp["neuron_updates_per_s"],
@@ -435,6 +470,11 @@ def main():
         "rooflines": roof_all,
         "cpu_baseline": cpu,
     }
+    if bt:
+        out["batch"] = batch
+        out["roofline"]["note"] = ("flops counted at aeif_cond_beta_multisynapse (2 ports) per RK5 "
+                                   "attempt for every model of the batch; iaf_psc_exp updates not "
+                                   "counted")
     print(json.dumps(out), flush=True)
 
 

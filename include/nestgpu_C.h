@@ -226,6 +226,26 @@ int NESTGPU_AMD_ResetKernel();
 /* Blocks until all queued device work of the simulator is done. */
 int NESTGPU_AMD_Synchronize();
 
+/* Batch kernel (no counterpart in the reference, which launches one kernel
+ * per node group, src/nestgpu.cu:501-503): when every population other than
+ * Poisson generators is aeif_cond_beta_multisynapse (2 ports),
+ * aeif_cond_alpha_multisynapse (4 ports), aeif_psc_exp,
+ * izhikevich_cond_beta (2 ports) with uniform parameters, or iaf_psc_exp
+ * (2 to 8 populations), each step is one launch, replayed as a one-node HIP
+ * graph.  cfg[9] = {on, variant (0: 3 waves per SIMD, 1: 4), iaf_psc_exp
+ * neurons per lane per pass (1, 2, 4), RK5 block weight x 100, graph,
+ * grid blocks, populations, dense (tiled) delivery available, used}.
+ * tiled: 1 deliver each step's list by the dense tiled launch, 0 inside the
+ * next step's launch, -1 unchanged.  A configuration changes scheduling
+ * only, never results.  AutotuneBatch times every candidate configuration
+ * over `steps` simulated steps (the simulation advances by candidates x
+ * steps steps) and keeps the fastest; ms_per_step (or NULL) receives its
+ * time. */
+int NESTGPU_AMD_GetBatchConfig(int *cfg);
+int NESTGPU_AMD_SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph,
+                               int tiled);
+int NESTGPU_AMD_AutotuneBatch(int steps, double *ms_per_step, int *n_candidates);
+
 /* Synapse table export (no counterpart in the reference; used by tests to
  * replay a network's input): out[k] = outgoing synapses of node i_node + k;
  * the number of synapses into the listed targets; and those synapses in the

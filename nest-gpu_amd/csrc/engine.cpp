@@ -177,6 +177,7 @@ const ModelDesc* find_model(const std::string& name) {
 Engine::Engine() { start_real_time_ = wall_time(); }
 
 Engine::~Engine() {
+  batch_graph_destroy(batch_graph_);
   for (auto* g : groups_) delete g;
   for (auto& r : records_)
     if (r.fp) fclose(r.fp);
@@ -1571,6 +1572,8 @@ int Engine::Calibrate() {
   upload(d_syn_w_, c_w_, stream_);
   upload(d_syn_d_, dd, stream_);
   sync_streams();
+  if (const char* env = getenv("NGPU_TILED")) tiled_env_ = atoi(env);
+  build_tiled(ch, dd);
   ch.clear(); ch.shrink_to_fit(); dd.clear(); dd.shrink_to_fit();
   d_ring_.alloc((size_t)n_slot_ * std::max<long long>(n_ch_, 1));
   HIPCHK(hipMemsetAsync(d_ring_.p, 0, d_ring_.n * sizeof(double), stream_));
@@ -1599,6 +1602,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_FOLD_IN_RK5")) fold_in_rk5_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_FUSED_DELIVER")) fused_deliver_env_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_LAG_D0_AT_EXIT")) lag_d0_at_exit_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
@@ -1753,6 +1757,8 @@ int Engine::Calibrate() {
     fused_deliver_ = fused_deliver_env_ && emitters == 1 && rk != nullptr;
     prev_list_pending_ = false;
   }
+  if (const char* env = getenv("NGPU_BATCH")) batch_env_ = atoi(env) != 0;
+  plan_batch();
   calibrated_ = true;
   neural_time_ = t_min_;
   return 0;
@@ -1824,9 +1830,70 @@ FoldArgs Engine::make_fold_args(const Group& g) {
   return fa;
 }
 
+RK5Args Engine::rk5_args(Group& gr, const StepCtx& sc, const SpikeList& sl, const SpikeBook& sb) {
+  Group* g = &gr;
+  RK5Args a{};
+  a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
+  a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
+  a.ctr = d_ctr_.p;
+  a.persistent = rk5_persistent_ ? 1 : 0;
+  a.max_blocks = rk5_max_blocks_;
+  if (g->err_last.p && rk5_persistent_) {
+    Predictor& pr = Predictor::get();
+    a.pred_mode = pr.mode();
+    a.err_last = g->err_last.p;
+    a.pred_params = pr.d_params();
+    a.log_buf = pr.d_log();
+    a.log_count = pr.d_log_count();
+    a.log_cap = pr.log_cap();
+    a.log_stride = pr.log_stride();
+  }
+  if (g->hot_flag.p) {
+    a.hot_flag = g->hot_flag.p;
+    a.hot_list = g->hot_list.p;
+    a.hot_count = g->hot_count.p;
+    a.hot_margin = rk5_hot_margin_;
+    a.hot_only = rk5_hot_only_ ? 1 : 0;
+    a.hot_lanes = rk5_hot_lanes_;
+  }
+  if (g->lag.p) {
+    a.lag = g->lag.p;
+    a.pend = g->pend.p;
+    // no split in a population a multimeter reads (it would see
+    // mid-step states), nor in the last step of a Simulate() call
+    // (state is complete between calls)
+    // learned proposals shorten the spike upswing's attempt chain, so
+    // the split point that balances two launches comes earlier
+    const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 14 : rk5_defer_cap_);
+    a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
+    if (g->fold_pending) {
+      a.prefold = 1;
+      a.pf = make_fold_args(*g);
+      a.pf.ctx = step_ctx(it_ - 1);
+    }
+  }
+  if (fused()) {
+    a.fd.lag_d0_at_exit = lag_d0_at_exit_ ? 1 : 0;
+    a.fd.skip_lagged_d0 = 1;
+    a.fd.st = syn_table();
+    a.fd.spikes = d_stats_.p;
+    a.clear_next_list = 1;
+    if (prev_list_pending_) {
+      a.fd.enable = 1;
+      a.fd.prev_t_idx = sc.t_idx - 1;
+      a.fd.prev_lst = (int)((sc.t_idx - 1) % kSpikeLists);
+    }
+  }
+  return a;
+}
+
 void Engine::launch_step() {
   const StepCtx sc = step_ctx(it_);
   SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
+  if (batch_active_) {
+    batch_step(sc, sl);
+    return;
+  }
   for (size_t gi = 0; gi < groups_.size(); ++gi) {
     Group* g = groups_[gi];
     SpikeBook sb{d_last_idx_.p, d_last_h_.p, g->spike_count.p, g->rec_times.p, g->n_rec.p,
@@ -1842,57 +1909,7 @@ void Engine::launch_step() {
       case MODEL_AEIF_PSC_ALPHA_MS:
       case MODEL_AEIF_PSC_DELTA:
       case MODEL_AEIF_PSC_EXP_MS: {
-        RK5Args a{};
-        a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.x = g->x.p; a.h = g->h.p;
-        a.tab = make_table(*g); a.h_min = g->h_min; a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
-        a.ctr = d_ctr_.p;
-        a.persistent = rk5_persistent_ ? 1 : 0;
-        a.max_blocks = rk5_max_blocks_;
-        if (g->err_last.p && rk5_persistent_) {
-          Predictor& pr = Predictor::get();
-          a.pred_mode = pr.mode();
-          a.err_last = g->err_last.p;
-          a.pred_params = pr.d_params();
-          a.log_buf = pr.d_log();
-          a.log_count = pr.d_log_count();
-          a.log_cap = pr.log_cap();
-          a.log_stride = pr.log_stride();
-        }
-        if (g->hot_flag.p) {
-          a.hot_flag = g->hot_flag.p;
-          a.hot_list = g->hot_list.p;
-          a.hot_count = g->hot_count.p;
-          a.hot_margin = rk5_hot_margin_;
-          a.hot_only = rk5_hot_only_ ? 1 : 0;
-          a.hot_lanes = rk5_hot_lanes_;
-        }
-        if (g->lag.p) {
-          a.lag = g->lag.p;
-          a.pend = g->pend.p;
-          // no split in a population a multimeter reads (it would see
-          // mid-step states), nor in the last step of a Simulate() call
-          // (state is complete between calls)
-          // learned proposals shorten the spike upswing's attempt chain, so
-          // the split point that balances two launches comes earlier
-          const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 14 : rk5_defer_cap_);
-          a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
-          if (g->fold_pending) {
-            a.prefold = 1;
-            a.pf = make_fold_args(*g);
-            a.pf.ctx = step_ctx(it_ - 1);
-          }
-        }
-        if (fused_deliver_) {
-          a.fd.lag_d0_at_exit = 1;
-          a.fd.st = syn_table();
-          a.fd.spikes = d_stats_.p;
-          a.clear_next_list = 1;
-          if (prev_list_pending_) {
-            a.fd.enable = 1;
-            a.fd.prev_t_idx = sc.t_idx - 1;
-            a.fd.prev_lst = (int)((sc.t_idx - 1) % kSpikeLists);
-          }
-        }
+        RK5Args a = rk5_args(*g, sc, sl, sb);
         timed(KT_RK5, [&] { launch_rk5(g->md->id, g->n_port, a, stream_); });
         break;
       }
@@ -1934,14 +1951,18 @@ void Engine::launch_step() {
   // delivery of this step's list: a launch of its own, or (fused) by the
   // next step's RK5 launch, except after the last step of a Simulate call
   // (or a step driven by SimulationStep), which leaves nothing pending
-  if (!fused_deliver_ || !defer_ok_) {
-    DeliverArgs da{};
-    da.ctx = sc; da.sl = sl; da.st = syn_table();
-    da.skip_lagged_d0 = fused_deliver_ ? 1 : 0;
-    da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
-    timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
+  if (!fused() || !defer_ok_) {
+    deliver_list(sc, sl, fused() && lag_d0_at_exit_ ? 1 : 0);
     prev_list_pending_ = false;
   } else {
+    if (!lag_d0_at_exit_) {
+      // the d_int = 0 synapses of this step's lagged spikes, before the next
+      // launch folds their slot
+      DeliverArgs da{};
+      da.ctx = sc; da.sl = sl; da.st = syn_table();
+      da.ctr = d_ctr_.p;
+      timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
+    }
     prev_list_pending_ = true;
   }
   for (auto* g : groups_) {
@@ -1959,6 +1980,370 @@ void Engine::launch_step() {
     timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
   }
   if (timing_ && ev_pending_.size() >= 4096) flush_timing();
+}
+
+// ---- batch kernel: every population in one launch per step ------------------
+// The network qualifies when every population other than the Poisson
+// generators is of a batch kind (kernels.h BatchKind): an RK5 model of the
+// catalog with uniform parameters, hot-first lists and split steps, or
+// iaf_psc_exp; between 2 and kMaxBatchPops of them.  Spikes are then
+// delivered by the next step's launch (fused delivery), and every
+// population's fold by its next launch, except after the last step of a
+// Simulate call, where the delivery and fold launches leave the state
+// complete, as without the batch.
+void Engine::plan_batch() {
+  batch_on_ = false;
+  batch_active_ = false;
+  batch_groups_.clear();
+  batch_fold_pending_ = false;
+  batch_graph_destroy(batch_graph_);
+  batch_graph_ = nullptr;
+  if (!batch_env_ || !rk5_persistent_ || rk5_defer_cap_ <= 0 || !fold_in_rk5_ || !fused_deliver_env_)
+    return;
+  std::vector<int> gs;
+  for (size_t gi = 0; gi < groups_.size(); ++gi) {
+    Group* g = groups_[gi];
+    const int id = g->md->id;
+    if (id == MODEL_POISSON_GEN) continue;
+    const int kind = batch_kind_of(id, g->n_port, 1);
+    if (kind < 0 || g->n <= 0) return;
+    if (g->md->rk5) {
+      if (!g->hot_flag.p || !g->lag.p || !batch_uniform_ok(kind, make_table(*g))) return;
+    } else {
+      const FoldLayout fl = fold_layout(*g->md);
+      if (g->n_port != 2 || fl.in_var != 0 || fl.in_port_stride != 1) return;
+    }
+    gs.push_back((int)gi);
+  }
+  if (gs.size() < 2 || gs.size() > (size_t)kMaxBatchPops) return;
+  // counter rows for every wave of the widest batch launch (allocated once:
+  // the uploaded per-population arguments hold the address)
+  size_t rows = 0;
+  for (int v = 0; v < kBatchVariants; ++v)
+    rows = std::max(rows, (size_t)std::max(batch_resident_blocks(v), (int)gs.size()) *
+                              (kRK5Block / kWave));
+  if (d_ctr_.n < rows * kCtrCols) {
+    d_ctr_.alloc(rows * kCtrCols);
+    HIPCHK(hipMemsetAsync(d_ctr_.p, 0, d_ctr_.n * sizeof(unsigned long long), stream_));
+  }
+  batch_groups_ = gs;
+  batch_on_ = true;
+  batch_partition();
+}
+
+// Blocks of the launch per population, proportional to n x cost with cost =
+// batch_rk5_weight_ x state variables for an RK5 population and 1 for
+// iaf_psc_exp (at least one block each), over every block the launch keeps
+// resident.
+void Engine::batch_partition() {
+  const int np = (int)batch_groups_.size();
+  int grid = batch_resident_blocks(batch_variant_);
+  grid = std::max(grid, np);
+  std::vector<double> w(np);
+  double tot = 0.0;
+  for (int k = 0; k < np; ++k) {
+    const Group* g = groups_[batch_groups_[k]];
+    w[k] = (double)g->n * (g->md->rk5 ? batch_rk5_weight_ * (double)g->nv : 1.0);
+    tot += w[k];
+  }
+  // blocks beyond one lane per neuron are useless: cap each population
+  std::vector<int> nb(np), cap(np);
+  int used = 0;
+  for (int k = 0; k < np; ++k) {
+    const Group* g = groups_[batch_groups_[k]];
+    cap[k] = std::max(1, (g->n + kRK5Block - 1) / kRK5Block);
+    nb[k] = std::min(cap[k], std::max(1, (int)std::floor((double)grid * w[k] / tot)));
+    used += nb[k];
+  }
+  // hand the remaining blocks out by largest deficit, within the caps
+  while (used < grid) {
+    int best = -1;
+    double bd = -1e300;
+    for (int k = 0; k < np; ++k) {
+      if (nb[k] >= cap[k]) continue;
+      const double d = (double)grid * w[k] / tot - nb[k];
+      if (d > bd) { bd = d; best = k; }
+    }
+    if (best < 0) break;
+    ++nb[best];
+    ++used;
+  }
+  while (used > grid) {   // more populations than blocks would fit: never below one each
+    int best = -1;
+    for (int k = 0; k < np; ++k)
+      if (nb[k] > 1 && (best < 0 || nb[k] > nb[best])) best = k;
+    if (best < 0) break;
+    --nb[best];
+    --used;
+  }
+  BatchArgs b{};
+  b.n_pop = np;
+  b.blk0[0] = 0;
+  for (int k = 0; k < np; ++k) {
+    const Group* g = groups_[batch_groups_[k]];
+    b.kind[k] = batch_kind_of(g->md->id, g->n_port, batch_iaf_unroll_);
+    b.blk0[k + 1] = b.blk0[k] + nb[k];
+  }
+  batch_grid_ = b.blk0[np];
+  batch_plan_ = b;
+  if ((size_t)batch_grid_ * (kRK5Block / kWave) * kCtrCols > d_ctr_.n)
+    throw Error("Batch kernel: more waves than counter rows");
+  batch_graph_destroy(batch_graph_);
+  batch_graph_ = nullptr;
+}
+
+void Engine::batch_upload_args() {
+  const int np = (int)batch_groups_.size();
+  std::vector<RK5Args> rk(np);
+  std::vector<IafArgs> ia(np);
+  std::vector<FoldArgs> fo(np);
+  SpikeList sl{d_spk_count_.p, d_spk_src_.p, d_spk_h_.p, spk_cap_, d_overflow_.p};
+  const StepCtx sc = step_ctx(0);
+  const bool saved_defer = defer_ok_, saved_fused = fused_deliver_;
+  defer_ok_ = true;        // the cap each population takes when splits are on
+  fused_deliver_ = true;   // lagged spikes: d_int = 0 synapses at wave exit
+  for (int k = 0; k < np; ++k) {
+    Group* g = groups_[batch_groups_[k]];
+    SpikeBook sb{d_last_idx_.p, d_last_h_.p, g->spike_count.p, g->rec_times.p, g->n_rec.p,
+                 g->max_rec};
+    std::memset(&rk[k], 0, sizeof(RK5Args));
+    std::memset(&ia[k], 0, sizeof(IafArgs));
+    std::memset(&fo[k], 0, sizeof(FoldArgs));
+    if (g->md->rk5) {
+      const bool fp = g->fold_pending;
+      g->fold_pending = false;
+      rk[k] = rk5_args(*g, sc, sl, sb);
+      g->fold_pending = fp;
+      rk[k].pred_mode = 0;
+      rk[k].pf = make_fold_args(*g);   // the previous step's fold (step values per launch)
+      rk[k].fd.enable = 0;
+      rk[k].clear_next_list = 0;
+    } else {
+      IafArgs& a = ia[k];
+      a.n = g->n; a.i0 = g->i0; a.var = g->var.p; a.param = g->param.p; a.tab = make_table(*g);
+      a.dt = dt_; a.ctx = sc; a.sl = sl; a.sb = sb;
+      fo[k] = make_fold_args(*g);
+    }
+  }
+  defer_ok_ = saved_defer;
+  fused_deliver_ = saved_fused;
+  batch_upload(rk.data(), ia.data(), fo.data(), np, stream_);
+}
+
+void Engine::batch_step(const StepCtx& sc, const SpikeList& sl) {
+  BatchArgs b = batch_plan_;
+  b.ctx = sc;
+  b.pctx = step_ctx(it_ - 1);
+  b.prefold = batch_fold_pending_ ? 1 : 0;
+  b.defer_on = defer_ok_ ? 1 : 0;
+  b.clear_next_list = 1;
+  b.sl = sl;
+  const bool fu = !tiled_use_;   // the next launch delivers this step's list
+  b.clear_next_list = fu ? 1 : 0;
+  b.fd.lag_d0_at_exit = fu && lag_d0_at_exit_ ? 1 : 0;
+  b.fd.skip_lagged_d0 = 1;
+  b.fd.st = syn_table();
+  b.fd.spikes = d_stats_.p;
+  if (fu && prev_list_pending_) {
+    b.fd.enable = 1;
+    b.fd.prev_t_idx = sc.t_idx - 1;
+    b.fd.prev_lst = (int)((sc.t_idx - 1) % kSpikeLists);
+  }
+  b.ctr = d_ctr_.p;
+  timed(KT_RK5, [&] {
+    if (batch_graph_on_) {
+      if (!batch_graph_) batch_graph_ = batch_graph_create(batch_variant_, batch_grid_, b);
+      batch_graph_launch(batch_graph_, b, stream_);
+    } else {
+      launch_batch(b, batch_grid_, batch_variant_, stream_);
+    }
+  });
+  if (rec_ncol_ > 0) {
+    RecordArgs ra{d_rec_cols_.p, rec_ncol_, d_rec_buf_.p, sc, d_last_idx_.p, d_last_h_.p,
+                  rec_row_base_, rec_buf_rows_};
+    timed(KT_OTHER, [&] { launch_record(ra, stream_); });
+  }
+  if (defer_ok_) {
+    // this step's folds are done by the next launch; with fused delivery its
+    // spikes too, except the d_int = 0 synapses of lagged spikes, whose slot
+    // that launch folds
+    if (!fu) {
+      deliver_list(sc, sl, 0);
+    } else if (!lag_d0_at_exit_) {
+      DeliverArgs da{};
+      da.ctx = sc; da.sl = sl; da.st = syn_table();
+      da.ctr = d_ctr_.p;
+      timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
+    }
+    prev_list_pending_ = fu;
+    batch_fold_pending_ = true;
+    for (int gi : batch_groups_) groups_[gi]->fold_pending = true;
+  } else {
+    // last step of a Simulate call: deliver this step's list and fold every
+    // population, so the state is complete between calls
+    deliver_list(sc, sl, fu && lag_d0_at_exit_ ? 1 : 0);
+    prev_list_pending_ = false;
+    batch_fold_pending_ = false;
+    for (int gi : batch_groups_) {
+      Group* g = groups_[gi];
+      g->fold_pending = false;
+      FoldArgs fa = make_fold_args(*g);
+      fa.ctx = sc;
+      fa.lag = g->lag.p;
+      fa.pend = g->pend.p;
+      timed(KT_FOLD, [&] { launch_fold(fa, stream_); });
+    }
+  }
+  if (timing_ && ev_pending_.size() >= 4096) flush_timing();
+}
+
+void Engine::GetBatchConfig(int* cfg) {
+  cfg[0] = batch_on_ && Predictor::get().mode() == 0 ? 1 : 0;
+  cfg[1] = batch_variant_;
+  cfg[2] = batch_iaf_unroll_;
+  cfg[3] = (int)std::lround(batch_rk5_weight_ * 100.0f);
+  cfg[4] = batch_graph_on_ ? 1 : 0;
+  cfg[5] = batch_on_ ? batch_grid_ : 0;
+  cfg[6] = (int)batch_groups_.size();
+  cfg[7] = tiled_on_ ? 1 : 0;
+  cfg[8] = tiled_use_ ? 1 : 0;
+}
+
+void Engine::SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled) {
+  if (variant < 0 || variant >= kBatchVariants) throw Error("Batch kernel variant must be 0 or 1");
+  if (iaf_unroll != 1 && iaf_unroll != 2 && iaf_unroll != 4)
+    throw Error("Batch iaf_psc_exp unroll must be 1, 2 or 4");
+  if (!(rk5_weight > 0.0f && rk5_weight < 1.0e6f)) throw Error("Batch RK5 weight must be positive");
+  batch_variant_ = variant;
+  batch_iaf_unroll_ = iaf_unroll;
+  batch_rk5_weight_ = rk5_weight;
+  batch_graph_on_ = graph != 0;
+  if (tiled >= 0) {
+    if (tiled && !tiled_on_) throw Error("Tiled delivery is not available for this network");
+    tiled_use_ = tiled != 0;
+  }
+  if (batch_on_) batch_partition();
+}
+
+// Autotuner of the batch kernel: register budget (3 or 4 waves per SIMD),
+// iaf_psc_exp unroll (1, 2, 4 neurons per lane per pass), the RK5 block
+// weight (2, 4, 8 per state variable) and, where the network has the tiled
+// synapse table, fused delivery or a dense delivery launch, each candidate timed with HIP events
+// around `steps` consecutive steps of the running simulation (a Simulate
+// call of that length); the fastest is kept.  Every candidate computes the
+// same results, so tuning on live steps changes nothing but time.
+double Engine::AutotuneBatch(int steps, int* n_candidates) {
+  if (!calibrated_) Calibrate();
+  if (!batch_on_) throw Error("Autotune: the network does not run as one batch launch");
+  if (Predictor::get().mode() != 0)
+    throw Error("Autotune: the batch launch runs with the step-size predictor off only");
+  if (steps < 2) throw Error("Autotune: at least 2 steps per candidate");
+  struct Cand { int v, u; float w; int t; };
+  std::vector<Cand> cands;
+  for (int t = 0; t <= (tiled_on_ ? 1 : 0); ++t)
+    for (int v = 0; v < kBatchVariants; ++v)
+      for (int u : {1, 2, 4})
+        for (float w : {2.0f, 4.0f, 8.0f}) cands.push_back({v, u, w, t});
+  const float saved_sim = sim_time_;
+  const int saved_verb = verbosity_;
+  verbosity_ = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best_ms = 1e300;
+  Cand best = cands[0];
+  for (const Cand& c : cands) {
+    SetBatchConfig(c.v, c.u, c.w, batch_graph_on_ ? 1 : 0, tiled_on_ ? c.t : -1);
+    sim_time_ = dt_ * steps;
+    HIPCHK(hipEventRecord(e0, stream_));
+    Simulate();
+    HIPCHK(hipEventRecord(e1, stream_));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  sim_time_ = saved_sim;
+  verbosity_ = saved_verb;
+  SetBatchConfig(best.v, best.u, best.w, batch_graph_on_ ? 1 : 0, tiled_on_ ? best.t : -1);
+  if (n_candidates) *n_candidates = (int)cands.size();
+  return best_ms / steps;
+}
+
+// Tiled copy of the synapse table for dense delivery (TiledSyn): tiles of
+// tile_ch channels, tile_ch the largest multiple of 32 whose all-slot FP64
+// tile fits kTileLdsBytes.  Built when a step's events may outnumber the
+// ring's entries (at least 4 synapses per ring entry; NGPU_TILED=1 builds it
+// whenever it fits, 0 never) and the (source, tile) table stays below 2^28
+// entries.  Within a source the synapses keep their order inside a tile.
+void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd) {
+  tiled_on_ = tiled_use_ = false;
+  d_toff_.free(); d_tch_.free(); d_tw_.free(); d_td_.free();
+  const size_t nc = ch.size();
+  if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0) return;
+  const int tc = (kTileLdsBytes / (n_slot_ * (int)sizeof(double))) / 32 * 32;
+  if (tc < 32) return;
+  const long long nt = (n_ch_ + tc - 1) / tc;
+  const long long entries = (long long)n_nodes_ * (nt + 1);
+  if (entries >= (1ll << 28)) return;
+  const bool dense = (double)nc >= 4.0 * (double)n_ch_ * (double)n_slot_;
+  if (tiled_env_ < 0 && !dense) return;
+  std::vector<unsigned int> off((size_t)entries), tch(nc);
+  std::vector<float> tw(nc);
+  std::vector<unsigned short> td(nc);
+#pragma omp parallel
+  {
+    std::vector<unsigned int> pos(nt + 1);
+#pragma omp for schedule(dynamic, 64)
+    for (int s = 0; s < n_nodes_; ++s) {
+      const long long b = row_ptr_[s], e = row_ptr_[s + 1];
+      std::fill(pos.begin(), pos.end(), 0u);
+      for (long long k = b; k < e; ++k) pos[ch[k] / tc + 1]++;
+      unsigned int* o = off.data() + (size_t)s * (nt + 1);
+      unsigned int run = (unsigned int)b;
+      for (long long t = 0; t <= nt; ++t) {
+        run += (t > 0) ? pos[t] : 0u;
+        o[t] = run;
+        pos[t] = run;   // next free position of tile t
+      }
+      for (long long k = b; k < e; ++k) {
+        const unsigned int j = pos[ch[k] / tc]++;
+        tch[j] = ch[k];
+        tw[j] = c_w_[k];
+        td[j] = dd[k];
+      }
+    }
+  }
+  upload(d_toff_, off, stream_);
+  upload(d_tch_, tch, stream_);
+  upload(d_tw_, tw, stream_);
+  upload(d_td_, td, stream_);
+  sync_streams();
+  tile_ch_ = tc;
+  n_tiles_ = (int)nt;
+  tiled_on_ = true;
+  tiled_use_ = true;
+}
+
+// Delivery launch of this step's list: dense (tiled) or global atomics.
+void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagged_d0) {
+  if (tiled_use_) {
+    TiledDeliverArgs ta{};
+    ta.ctx = sc; ta.sl = sl;
+    ta.ts = TiledSyn{d_toff_.p, d_tch_.p, d_tw_.p, d_td_.p, n_tiles_, tile_ch_};
+    ta.ring = d_ring_.p; ta.n_ch = n_ch_; ta.n_slot = n_slot_;
+    ta.skip_lagged_d0 = skip_lagged_d0;
+    ta.ctr = d_ctr_.p; ta.spikes = d_stats_.p;
+    timed(KT_DELIVER, [&] { launch_deliver_tiled(ta, stream_); });
+    return;
+  }
+  DeliverArgs da{};
+  da.ctx = sc; da.sl = sl; da.st = syn_table();
+  da.skip_lagged_d0 = skip_lagged_d0;
+  da.ctr = d_ctr_.p; da.spikes = d_stats_.p;
+  timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
 }
 
 SynTable Engine::syn_table() const {
@@ -2079,6 +2464,12 @@ int Engine::StartSimulation() {
   }
   rec_rows_pending_ = 0;
   rec_row_base_ = 0;
+  // the batch kernel's per-population arguments (parameters may have been
+  // set since the last call, and which populations are recorded decides
+  // their split caps)
+  // (learned step-size proposals run in the single-population RK5 launches)
+  batch_active_ = batch_on_ && Predictor::get().mode() == 0;
+  if (batch_active_) batch_upload_args();
   return 0;
 }
 

@@ -240,6 +240,16 @@ class Engine {
   // 0 RK5 update, 1 iaf_psc_exp update, 2 delivery, 3 fold+Poisson, 4 other.
   enum { KT_RK5 = 0, KT_IAF, KT_DELIVER, KT_FOLD, KT_OTHER, KT_N };
   void SetKernelTiming(int mode);   // 0 off, 1 every launch, 2 RK5 launches only
+  // Batch kernel (all populations in one launch per step, replayed as a
+  // one-node HIP graph): configuration in effect, a new one (takes effect
+  // with the next step; results do not depend on it), and the autotuner,
+  // which times every candidate over `steps` simulated steps each (so the
+  // simulation advances by candidates x steps) and keeps the fastest.
+  // cfg[9] = {on, variant, iaf_unroll, rk5_weight_x100, graph, grid, populations,
+  //          tiled delivery available, tiled delivery used}
+  void GetBatchConfig(int* cfg);
+  void SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled = -1);
+  double AutotuneBatch(int steps, int* n_candidates);
   void KernelTimes(double* ms, long long* launches);
   void Synchronize();
   hipStream_t stream() const { return stream_; }
@@ -254,6 +264,14 @@ class Engine {
   void calibrate_group(Group& g);
   void upload_poisson_tables();
   void launch_step();
+  RK5Args rk5_args(Group& g, const StepCtx& sc, const SpikeList& sl, const SpikeBook& sb);
+  void plan_batch();
+  void batch_partition();
+  void batch_upload_args();
+  void batch_step(const StepCtx& sc, const SpikeList& sl);
+  void build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd);
+  void deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagged_d0);
+  bool fused() const { return fused_deliver_ && !tiled_use_; }
   FoldArgs make_fold_args(const Group& g);
   SynTable syn_table() const;
   void fetch_records(long long n_rows);
@@ -316,8 +334,30 @@ class Engine {
   bool fused_deliver_env_ = true;      // allow the RK5 launch to deliver the previous step's spikes
   bool fused_deliver_ = false;         // ... and the network qualifies (one emitting RK5 population)
   bool prev_list_pending_ = false;     // the previous step's list awaits delivery by the next RK5 launch
+  bool lag_d0_at_exit_ = false;        // lagged spikes' d_int = 0 synapses by the emitting wave (else a launch)
+  // dense (tiled) delivery
+  int tiled_env_ = -1;                 // NGPU_TILED: -1 when dense, 0 never, 1 whenever the table fits
+  bool tiled_on_ = false;              // the tiled copy of the synapse table exists
+  bool tiled_use_ = false;             // deliver with it (a batch autotuner knob)
+  int tile_ch_ = 0, n_tiles_ = 0;
+  DBuf<unsigned int> d_toff_, d_tch_;
+  DBuf<float> d_tw_;
+  DBuf<unsigned short> d_td_;
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
+  // batch kernel
+  bool batch_env_ = true;              // NGPU_BATCH=0: one launch per population
+  bool batch_on_ = false;              // the network qualifies (every emitter in the batch catalog)
+  bool batch_active_ = false;          // ... and this Simulate call uses it (predictor off)
+  bool batch_fold_pending_ = false;    // the previous step's fold is left to the next batch launch
+  int batch_variant_ = 0;              // register budget: 0 = 3 waves per SIMD, 1 = 4
+  int batch_iaf_unroll_ = 2;           // iaf_psc_exp neurons per lane per pass
+  float batch_rk5_weight_ = 4.0f;      // blocks per RK5 neuron and state variable, relative to iaf
+  bool batch_graph_on_ = true;         // replay the launch as a HIP graph node
+  int batch_grid_ = 0;
+  std::vector<int> batch_groups_;      // group index of each batch population
+  BatchArgs batch_plan_{};             // kinds and block partition
+  BatchGraph* batch_graph_ = nullptr;
   DBuf<unsigned long long> d_ctr_;     // per-wave counter rows [rows][kCtrCols]
   DBuf<unsigned long long> d_stats_;   // delivered spikes
   // poisson generator tables

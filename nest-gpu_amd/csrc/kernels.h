@@ -88,14 +88,18 @@ struct SynTable {
   int n_slot;
 };
 
-// Delivery done by the persistent RK5 kernel (one spike-emitting population):
-// its idle waves deliver the previous step's spike list (the step's fold of
-// that list's earliest slot comes one launch later), and each wave delivers
-// the d_int = 0 synapses of its own lagged spikes when it exits (their slot
-// is folded by the next launch).
+// Delivery done by the persistent RK5 kernel (one spike-emitting population)
+// or the batch kernel: its idle waves deliver the previous step's spike list
+// (the step's fold of that list's earliest slot comes one launch later).  The
+// d_int = 0 synapses of a lagged spike (emitted in the previous step's time)
+// land in the slot the next launch folds, so they are delivered before it:
+// by deliver_lagged_d0_kernel right after this launch (lag_d0_at_exit = 0),
+// or by the emitting wave when it exits (lag_d0_at_exit = 1; that puts their
+// loads and atomics at the end of the spike chains, which end the launch).
 struct FusedDeliver {
   int enable;            // deliver the previous step's list in this launch
   int lag_d0_at_exit;    // lagged spikes: d_int = 0 synapses at wave exit, the rest later
+  int skip_lagged_d0;    // the previous list's lagged spikes had their d_int = 0 synapses delivered
   int prev_lst;          // list index of the previous step
   long long prev_t_idx;  // its time index
   SynTable st;
@@ -197,6 +201,38 @@ struct ParrotArgs {
   SpikeBook sb;
 };
 
+// Copy of the synapse table for dense delivery (deliver_tiled_kernel): input
+// channels are cut into tiles of tile_ch consecutive channels; within each
+// source's segment the synapses are grouped by tile, and off holds the first
+// synapse of every (source, tile) pair.
+struct TiledSyn {
+  const unsigned int* off;     // [n_nodes][n_tiles + 1]
+  const unsigned int* syn_ch;  // [n_syn] target channel
+  const float* syn_w;          // [n_syn] weight
+  const unsigned short* syn_d; // [n_syn] delay slot d_int
+  int n_tiles, tile_ch;
+};
+
+// Dense delivery of one step's spike list: one block per channel tile sums
+// every spike's synapses into that tile in LDS (FP64, all n_slot slots), then
+// adds the tile into the ring once.  Replaces global atomics by LDS atomics
+// and one coalesced read-modify-write per ring entry that received input,
+// for networks where a step's events outnumber the ring's entries.
+struct TiledDeliverArgs {
+  StepCtx ctx;           // this step's time (by value)
+  SpikeList sl;
+  TiledSyn ts;
+  double* ring;          // [n_slot][n_ch]
+  long long n_ch;
+  int n_slot;
+  int skip_lagged_d0;    // the d_int = 0 synapses of lagged spikes were delivered already
+  unsigned long long* ctr;     // per-wave counter rows (column 3) or null
+  unsigned long long* spikes;  // delivered spikes (one add per launch) or null
+};
+void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
+// LDS bytes of one tile block; tile_ch = this / (n_slot * 8), a multiple of 32
+constexpr int kTileLdsBytes = 64 * 1024;
+
 struct DeliverArgs {
   StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
@@ -224,6 +260,62 @@ struct RecordArgs {
   long long max_rows;    // rows in buf
 };
 
+// ---------------------------------------------------------------------------
+// Batch kernel (rk5_batch.hip): every population of a mixed network advances
+// in one launch per step.  The grid is split by blocks among the populations
+// (blk0); each block runs the persistent RK5 loop of its population's model,
+// or the iaf_psc_exp update, over the population's share of waves, folding
+// the previous step's ring slot and Poisson input as lanes take their
+// neurons; afterwards every wave of the launch helps deliver the previous
+// step's spike list.  Per-step values travel by value in BatchArgs; the
+// per-population arguments sit in the batch module's constant memory
+// (batch_upload), unchanged from step to step, so the launch can also be
+// replayed as a HIP graph node whose arguments are updated per step.
+constexpr int kMaxBatchPops = 8;
+enum BatchKind {
+  BK_COND_BETA2 = 0,   // aeif_cond_beta_multisynapse, 2 ports
+  BK_COND_ALPHA4,      // aeif_cond_alpha_multisynapse, 4 ports
+  BK_PSC_EXP,          // aeif_psc_exp
+  BK_IZH2,             // izhikevich_cond_beta, 2 ports
+  BK_IAF1,             // iaf_psc_exp, 1 / 2 / 4 neurons per lane per pass
+  BK_IAF2,
+  BK_IAF4,
+  BK_N
+};
+// register budget of the batch kernel: 3 (variant 0) or 4 (variant 1) waves per SIMD
+constexpr int kBatchVariants = 2;
+
+struct BatchArgs {
+  StepCtx ctx;           // this step
+  StepCtx pctx;          // the previous step (its fold, prefold = 1)
+  int prefold;           // fold the previous step's slot as lanes take their neurons
+  int defer_on;          // split steps allowed in this launch (cap per population in RK5Args)
+  int clear_next_list;   // clear the count of the list the next step writes
+  int n_pop;
+  int kind[kMaxBatchPops];
+  int blk0[kMaxBatchPops + 1];   // first block of each population; blk0[n_pop] = grid
+  SpikeList sl;
+  FusedDeliver fd;       // the previous step's list (fd.enable) and lagged spikes at exit
+  unsigned long long* ctr;     // per-wave counter rows (global wave index) or null
+};
+
+// batch kind of a (model, ports) pair with uniform parameters, or -1
+int batch_kind_of(int model, int n_port, int iaf_unroll);
+// an RK5 population's parameters qualify for the batch kernel's
+// uniform-parameter loop (iaf kinds: always)
+bool batch_uniform_ok(int kind, const ParamTable& t);
+// per-population arguments: rk5[k] for RK5 kinds, iaf[k] and fold[k] for iaf
+void batch_upload(const RK5Args* rk5, const IafArgs* iaf, const FoldArgs* fold, int n_pop,
+                  hipStream_t s);
+int batch_resident_blocks(int variant);      // blocks of one launch resident at once
+void launch_batch(const BatchArgs& b, int grid, int variant, hipStream_t s);
+// The batch launch as a one-node HIP graph, instantiated once; each launch
+// sets the node's arguments to this step's and replays it.
+struct BatchGraph;
+BatchGraph* batch_graph_create(int variant, int grid, const BatchArgs& b);
+void batch_graph_launch(BatchGraph* g, const BatchArgs& b, hipStream_t s);
+void batch_graph_destroy(BatchGraph* g);
+
 // Per-family RK5 launchers and input-layout checks (rk5_*.hip), called by
 // launch_rk5 / rk5_fold_layout_matches.
 void launch_rk5_cond_beta_ms(int n_port, const RK5Args& a, hipStream_t s);
@@ -250,6 +342,8 @@ void launch_spike_gen(const SpikeGenArgs& a, hipStream_t s);
 void launch_detector(const DetectorArgs& a, hipStream_t s);
 void launch_parrot(const ParrotArgs& a, hipStream_t s);
 void launch_deliver(const DeliverArgs& a, hipStream_t s);   // kDeliverBlocks x 256
+// the d_int = 0 synapses of the lagged spikes of this step's list (fused delivery)
+void launch_deliver_lagged_d0(const DeliverArgs& a, hipStream_t s);
 void launch_fold(const FoldArgs& a, hipStream_t s);
 void launch_record(const RecordArgs& a, hipStream_t s);
 

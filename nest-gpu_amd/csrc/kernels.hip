@@ -97,8 +97,9 @@ void launch_rk5(int model, int n_port, const RK5Args& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // iaf_psc_exp exact update, src/iaf_psc_exp.cu:85-110.  HBM-bound: 16 B of
 // state read and written per neuron-step, coefficients from the uniform
-// table.  Contraction is off so the float arithmetic matches the reference
-// operation by operation.
+// table.  Contraction is off here (an fma buys nothing in an HBM-bound
+// kernel), operation by operation as the oracle's iaf_psc_exp; the RK5
+// kernels fix a contracted pattern instead.
 __global__ void __launch_bounds__(256) iaf_kernel(IafArgs a) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,6 +228,109 @@ __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
 
 void launch_deliver(const DeliverArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(deliver_kernel, dim3(kDeliverBlocks), dim3(256), 0, s, a);
+}
+
+// Dense delivery (TiledDeliverArgs): block b owns channel tile b (and b +
+// gridDim.x, ...).  Its threads take the list's spikes one each, add every
+// synapse of (spike source, tile) into the LDS tile acc[slot][channel] with
+// FP64 LDS atomics, and then the tile is added into the ring (plain
+// read-modify-write: no other block touches these ring entries, and no other
+// kernel runs meanwhile).  The sums are the exact FP64 sums of the float
+// products (height * weight), as the global-atomic delivery's, so the ring
+// receives the same values bit for bit.
+__global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) {
+  extern __shared__ double acc[];
+  const int tc = a.ts.tile_ch, ns = a.n_slot, nt = a.ts.n_tiles;
+  const int lst = a.ctx.lst;
+  int n_spk = a.sl.count[lst];
+  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
+  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
+  const float* hts = a.sl.height + (size_t)lst * a.sl.cap;
+  unsigned long long n_ev = 0;
+  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const long long c0 = (long long)tile * tc;
+    for (int k = threadIdx.x; k < ns * tc; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < n_spk; s += blockDim.x) {
+      const unsigned int raw = (unsigned int)srcs[s];
+      const int src = (int)(raw & ~kSpikeLagBit);
+      const float hgt = hts[s];
+      const int base = spike_base_slot(raw, a.ctx.t_idx, ns);
+      const bool skip_d0 = a.skip_lagged_d0 && (raw >> 31);
+      const unsigned int* o = a.ts.off + (size_t)src * (nt + 1) + tile;
+      const unsigned int b = o[0], e = o[1];
+      for (unsigned int k = b; k < e; ++k) {
+        const int d = a.ts.syn_d[k];
+        if (skip_d0 && d == 0) continue;
+        int slot = base + d;
+        if (slot >= ns) slot -= ns;
+        atomicAdd(&acc[slot * tc + (int)((long long)a.ts.syn_ch[k] - c0)],
+                  (double)(hgt * a.ts.syn_w[k]));
+        ++n_ev;
+      }
+    }
+    __syncthreads();
+    const long long left = a.n_ch - c0;
+    const int w = left < tc ? (int)left : tc;
+    for (int k = threadIdx.x; k < ns * tc; k += blockDim.x) {
+      const int slot = k / tc, c = k - slot * tc;
+      if (c < w) {
+        const double v = acc[k];
+        if (v != 0.0) a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + c)] += v;
+      }
+    }
+    __syncthreads();
+  }
+  n_ev = wave_sum(n_ev);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane == 0 && a.ctr && n_ev)
+    a.ctr[((size_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kCtrCols + 3] += n_ev;
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
+    // the next step writes list (lst + 1) mod 3: clear its count now (it held
+    // the list of step t_idx - 2, delivered two steps ago)
+    a.sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
+  }
+}
+
+void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
+  const size_t lds = (size_t)a.n_slot * a.ts.tile_ch * sizeof(double);
+  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0)
+    throw std::runtime_error("tiled delivery: tile does not fit its LDS budget");
+  const int grid = a.ts.n_tiles < kDeliverBlocks ? a.ts.n_tiles : kDeliverBlocks;
+  hipLaunchKernelGGL(deliver_tiled_kernel, dim3(grid), dim3(256), lds, s, a);
+}
+
+// Fused delivery, right after the step's update launch: the d_int = 0
+// synapses of the list's lagged spikes (their slot, t_emit + 1, is this
+// step's, which the next launch folds as its lanes take their neurons); the
+// rest of each lagged spike, and every other spike, is delivered with the
+// list by the next launch.  One wave per list entry at a time.
+constexpr int kLagD0Blocks = 64;
+
+__global__ void __launch_bounds__(256) deliver_lagged_d0_kernel(DeliverArgs a) {
+  const int lst = a.ctx.lst;
+  int n_spk = a.sl.count[lst];
+  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
+  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
+  const float* hts = a.sl.height + (size_t)lst * a.sl.cap;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * (blockDim.x / kWave);
+  unsigned long long n_ev = 0;
+  for (int s = wave; s < n_spk; s += n_waves) {
+    const unsigned int raw = (unsigned int)srcs[s];
+    if (!(raw >> 31)) continue;
+    const int src = (int)(raw & ~kSpikeLagBit);
+    n_ev += deliver_segment(a.st, spike_base_slot(raw, a.ctx.t_idx, a.st.n_slot), hts[s],
+                            a.st.row_ptr[src], a.st.row_ptr[src + 1], lane, kWave, 1);
+  }
+  n_ev = wave_sum(n_ev);
+  if (lane == 0 && a.ctr && n_ev) a.ctr[(size_t)wave * kCtrCols + 3] += n_ev;
+}
+
+void launch_deliver_lagged_d0(const DeliverArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(deliver_lagged_d0_kernel, dim3(kLagD0Blocks), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

@@ -713,5 +713,25 @@ int NESTGPU_AMD_PredictorLossGrad(const float* raw, const float* target, const u
     return 0;
   });
 }
+int NESTGPU_AMD_GetBatchConfig(int* cfg) {
+  return guarded(0, [&] {
+    E().GetBatchConfig(cfg);
+    return 0;
+  });
+}
+int NESTGPU_AMD_SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph,
+                               int tiled) {
+  return guarded(0, [&] {
+    E().SetBatchConfig(variant, iaf_unroll, rk5_weight, graph, tiled);
+    return 0;
+  });
+}
+int NESTGPU_AMD_AutotuneBatch(int steps, double* ms_per_step, int* n_candidates) {
+  return guarded(0, [&] {
+    const double ms = E().AutotuneBatch(steps, n_candidates);
+    if (ms_per_step) *ms_per_step = ms;
+    return 0;
+  });
+}
 
 }  // extern "C"

@@ -52,9 +52,10 @@ using ck::kMaxAttempts;
 // ---------------------------------------------------------------------------
 // Neuron models (right-hand sides and post-step updates).
 // Parameter and variable indices follow the reference's name tables.  Each
-// product-plus-sum of the reference's expressions (a*b + c) is one fma, as
-// the reference's compiler contracts them; the oracle evaluates the same
-// sequence.
+// product-plus-sum of the reference's expressions (a*b + c) is one fma: the
+// reference's CUDA build contracts by default, though which products its
+// compiler fuses is its own choice; this build fixes this pattern (fewer
+// instructions on the attempt chain) and the oracle evaluates the same one.
 //
 // Every division of a right-hand side is by a parameter (M::divisor(j)).
 // dv.div(x, p, j) returns x / p[j].  DivIEEE divides.  DivFast, for
@@ -111,7 +112,7 @@ __device__ inline bool lagged_spike(int raw) { return ((unsigned int)raw >> 31) 
 // further stages, the 5th-order solution yn and the error ratio
 // max_j |h sum_i e_i k_i[j]| / ys[j] (before the division by eps).  Each
 // stage's sum_i a_i k_i is accumulated with fma and added to y with one more
-// (y + h*(...) as the reference's compiler contracts it).
+// (y + h*(...)), the contracted pattern this build fixes for kernel and oracle.
 template <class M, class F>
 __device__ inline float rk5_try(const float* y, const float* k1, const float* ys, float h,
                                 float* yn, F&& deriv) {
@@ -266,27 +267,39 @@ __global__ void __launch_bounds__(256) rk5_kernel(RK5Args a) {
 // that of rk5_kernel.  UNI: every parameter except refractory_step is
 // uniform over the population, so parameters come from the kernel-argument
 // table and divisions by them use reciprocals (DivRecip, bit-identical).
-// Register budget of a small model's kernel (at most 6 state variables): the
-// plain kernel is held to 128 VGPRs (4 waves per SIMD) and the
-// learned-proposal kernel to 168 (3 waves); the compiler would otherwise take
-// a few more and drop a wave per SIMD.
-template <class M, bool UNI, int MODE>
-__global__ void __launch_bounds__(kRK5Block)
-__attribute__((amdgpu_waves_per_eu(M::NV > 6 ? 1 : MODE == 2 ? 3 : MODE == 0 ? 4 : 1)))
-rk5_persistent_kernel(RK5Args a) {
+//
+// Step values of one persistent RK5 launch, read where they are used from
+// the launch's own kernel arguments (so the compiler reloads them instead of
+// holding them in registers): a single-population launch takes them from its
+// RK5Args; the multi-population batch kernel from its batch arguments, its
+// per-population RK5Args staying in device memory unchanged from step to step.
+struct StepFromArgs {
+  const RK5Args& a;
+  __device__ const StepCtx& ctx() const { return a.ctx; }
+  __device__ const StepCtx& pctx() const { return a.pf.ctx; }   // the previous step
+  __device__ int prefold() const { return a.prefold; }
+  __device__ int defer_cap() const { return a.defer_cap; }
+};
+
+// The persistent loop of one wave over its population's share: wave / n_waves
+// index the population's waves, ctr_row is the wave's counter row (unique in
+// the launch), my_hot / my_spk its LDS buffers of kHotLds / kSpkLds entries,
+// s_pred the staged predictor (MODE 2).  Synapses delivered by the wave are
+// added to n_dev.
+template <class M, bool UNI, int MODE, class SV>
+__device__ __attribute__((always_inline)) inline void rk5_persistent_body(
+    const RK5Args& a, const SV& sv, const int wave, const int n_waves,
+    const int ctr_row, int* my_hot, int* my_spk, PredLDS* s_pred_p, unsigned long long& n_dev) {
   static_assert(MODE == 0 || PredCapable<M>::value, "predictor modes need the AdEx beta model");
   const int lane = threadIdx.x & (kWave - 1);
-  const int wave = blockIdx.x * (blockDim.x / kWave) +
-                   __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int n_waves = gridDim.x * (blockDim.x / kWave);
   const int n = a.n;
-  const double x1 = a.ctx.t1;
-  const long long t_idx = a.ctx.t_idx;
-  const int parity = a.ctx.parity;
-  const int lst = a.ctx.lst;
+  const double x1 = sv.ctx().t1;
+  const long long t_idx = sv.ctx().t_idx;
+  const int parity = sv.ctx().parity;
+  const int lst = sv.ctx().lst;
   // end of the previous step (the end_step_kernel formula one step back): a
   // lagged neuron integrates up to it before starting this step
-  const double x1_prev = a.ctx.t0 + (double)a.dt * (double)a.ctx.it;
+  const double x1_prev = sv.ctx().t0 + (double)a.dt * (double)sv.ctx().it;
   // the wave's own hot list first (neurons of its chunk that ended the last
   // step about to spike, or split; ~1 per wave, so each takes a lane of its
   // own and the long attempt chains start together in the first iteration),
@@ -296,8 +309,6 @@ rk5_persistent_kernel(RK5Args a) {
   unsigned char* flag_out = nullptr;
   const int* dense = nullptr;
   int n_hot = 0, q_out = 0;
-  __shared__ int s_hot[kRK5Block / kWave][kHotLds];
-  int* my_hot = s_hot[threadIdx.x / kWave];
   if (a.hot_flag) {
     const int q_in = (int)(t_idx % 3);
     q_out = q_in == 2 ? 0 : q_in + 1;
@@ -307,13 +318,8 @@ rk5_persistent_kernel(RK5Args a) {
     dense = a.hot_list + (size_t)q_in * n;
     flag_in = a.hot_flag + (size_t)parity * n;
     flag_out = a.hot_flag + (size_t)(parity ^ 1) * n;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.hot_count[q_clr] = 0;
+    if (wave == 0 && lane == 0) a.hot_count[q_clr] = 0;
   }
-  // with fused delivery no delivery launch clears the spike list the next
-  // step writes: it held the list of step t_idx - 2, delivered by the
-  // previous launch
-  if (a.clear_next_list && blockIdx.x == 0 && threadIdx.x == 0)
-    a.sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
   // wave w takes hot entries [w*hp, w*hp + hp) (one per lane, so the long
   // attempt chains start together in the first iteration, packed into few
   // waves), then its equal share of all neurons
@@ -340,8 +346,6 @@ rk5_persistent_kernel(RK5Args a) {
   if (n_hw > 0) __builtin_amdgcn_s_setprio(3);
   // spikes of the wave, appended to the step's list with one atomic when the
   // wave exits (a returning atomic per spike would stall the long chains)
-  __shared__ int s_spk[kRK5Block / kWave][kSpkLds];
-  int* my_spk = s_spk[threadIdx.x / kWave];
   int scnt = 0;         // spikes of this wave (wave-uniform)
   bool spk = false;     // the lane's neuron spiked in this iteration
   int spk_src = 0;
@@ -366,13 +370,7 @@ rk5_persistent_kernel(RK5Args a) {
   float h_sub = 0.0f;   // controller proposal at the start of the substep
   bool clip = false;    // accepted attempt was cut by the end of the step
   bool rej_sub = false; // the current substep had a rejected attempt
-  __shared__ PredLDS s_pred;
-  if constexpr (MODE == 2) {
-    pred_stage(s_pred, a.pred_params);
-    __syncthreads();
-  }
   unsigned long long n_att = 0, n_rej = 0, n_sub = 0;
-  unsigned long long n_dev = 0;   // synapses delivered by this lane (fused delivery)
   unsigned int n_split = 0;
 #pragma unroll
   for (int j = 0; j < M::NPAR; ++j) {
@@ -414,7 +412,7 @@ rk5_persistent_kernel(RK5Args a) {
           h = a.h[i];
           if constexpr (MODE != 0) el = a.err_last[i];
           lagged = a.lag ? a.lag[i] != 0 : false;
-          if (a.prefold) {
+          if (sv.prefold()) {
             // the previous step's fold of this neuron's channels (the fold
             // kernel's arithmetic): added to the input variables, or kept
             // as the stashed input of a neuron still inside that step
@@ -422,8 +420,8 @@ rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
             for (int k = 0; k < M::NPORT; ++k) {
               const long long lo = (long long)k * n;
-              double* rp = f.ring + (size_t)f.ctx.slot * f.n_ch + (size_t)(f.ch_base + lo + i);
-              double acc = poisson_input(f, lo + i, *rp, f.ctx.t1, f.ctx.t_idx,
+              double* rp = f.ring + (size_t)sv.pctx().slot * f.n_ch + (size_t)(f.ch_base + lo + i);
+              double acc = poisson_input(f, lo + i, *rp, sv.pctx().t1, sv.pctx().t_idx,
                                          f.pu_b < lo + n && f.pu_e > lo);
               *rp = 0.0;
               if (lagged) {
@@ -483,7 +481,7 @@ rk5_persistent_kernel(RK5Args a) {
 #pragma unroll
           for (int k = 0; k < kPI; ++k) raw[k] = 0.0f;
         }
-        const float o = pred_infer_wave(s_pred, raw);
+        const float o = pred_infer_wave(*s_pred_p, raw);
         // the learned proposal only shortens the controller's (the error
         // scale |y| + |k1 h| + 0.1 grows with h, so a longer step than the
         // controller's could pass the test less accurately); floored at
@@ -586,7 +584,7 @@ rk5_persistent_kernel(RK5Args a) {
           lagged = false;
           cx1 = x1;
           att_i = 0;
-        } else if (end_step || (a.defer_cap > 0 && !lagged && att_i >= a.defer_cap)) {
+        } else if (end_step || (sv.defer_cap() > 0 && !lagged && att_i >= sv.defer_cap())) {
           // end of the step, or split: the neuron is stored mid-step with
           // lag = 1 (and listed hot, so the next launch resumes it first)
           if (!end_step) {
@@ -678,29 +676,6 @@ rk5_persistent_kernel(RK5Args a) {
       }
     }
   }
-  if (a.fd.enable) {
-    // the previous step's spike list, one spike per wave at a time, taken
-    // from the last wave down so that the first waves (the hot ones, which
-    // end last) take none while there are fewer spikes than waves
-    const int pl = a.fd.prev_lst;
-    int n_prev = a.sl.count[pl];
-    if (n_prev > a.sl.cap) n_prev = a.sl.cap;
-    const int* srcs = a.sl.src + (size_t)pl * a.sl.cap;
-    const float* hts = a.sl.height + (size_t)pl * a.sl.cap;
-    for (int sp = n_waves - 1 - wave; sp < n_prev; sp += n_waves) {
-      const unsigned int raw = (unsigned int)srcs[sp];
-      const int src = (int)(raw & ~kSpikeLagBit);
-      const int mode = (a.fd.lag_d0_at_exit && (raw >> 31)) ? 2 : 0;
-      n_dev += deliver_segment(a.fd.st, spike_base_slot(raw, a.fd.prev_t_idx, a.fd.st.n_slot),
-                               hts[sp], a.fd.st.row_ptr[src], a.fd.st.row_ptr[src + 1], lane,
-                               kWave, mode);
-    }
-    if (wave == 0 && lane == 0 && a.fd.spikes && n_prev) *a.fd.spikes += (unsigned long long)n_prev;
-  }
-  if (a.ctr) {
-    n_dev = wave_sum(n_dev);
-    if (lane == 0 && n_dev) a.ctr[(size_t)wave * kCtrCols + 3] += n_dev;
-  }
   if (a.hot_flag) {
     // append the wave's list to the next step's (one atomic per wave; the
     // order of the entries only affects scheduling)
@@ -711,14 +686,75 @@ rk5_persistent_kernel(RK5Args a) {
       base = __shfl(base, 0, kWave);
       int* out = a.hot_list + (size_t)q_out * n + base;
       for (int e = lane; e < listed; e += kWave) out[e] = my_hot[e];
-      if (lane == 0 && a.ctr) a.ctr[(size_t)wave * kCtrCols + 4] += (unsigned long long)listed;
+      if (lane == 0 && a.ctr) a.ctr[(size_t)ctr_row * kCtrCols + 4] += (unsigned long long)listed;
     }
   }
   if (a.lag && a.ctr) {
     const unsigned long long ns = wave_sum(n_split);
-    if (lane == 0 && ns) a.ctr[(size_t)wave * kCtrCols + 5] += ns;
+    if (lane == 0 && ns) a.ctr[(size_t)ctr_row * kCtrCols + 5] += ns;
   }
-  add_wave_counters(a.ctr, wave, lane, n_att, n_rej, n_sub);
+  add_wave_counters(a.ctr, ctr_row, lane, n_att, n_rej, n_sub);
+}
+
+// Fused delivery: the previous step's spike list, one spike per wave at a
+// time, taken from the last wave of the launch down so that the first waves
+// (the hot ones, which end last) take none while there are fewer spikes than
+// waves.  Returns the synapses delivered by the lane.
+__device__ inline unsigned long long deliver_prev_list(const FusedDeliver& fd, const SpikeList& sl,
+                                                       int wave, int n_waves, int lane) {
+  unsigned long long n_dev = 0;
+  const int pl = fd.prev_lst;
+  int n_prev = sl.count[pl];
+  if (n_prev > sl.cap) n_prev = sl.cap;
+  const int* srcs = sl.src + (size_t)pl * sl.cap;
+  const float* hts = sl.height + (size_t)pl * sl.cap;
+  for (int sp = n_waves - 1 - wave; sp < n_prev; sp += n_waves) {
+    const unsigned int raw = (unsigned int)srcs[sp];
+    const int src = (int)(raw & ~kSpikeLagBit);
+    const int mode = (fd.skip_lagged_d0 && (raw >> 31)) ? 2 : 0;
+    n_dev += deliver_segment(fd.st, spike_base_slot(raw, fd.prev_t_idx, fd.st.n_slot), hts[sp],
+                             fd.st.row_ptr[src], fd.st.row_ptr[src + 1], lane, kWave, mode);
+  }
+  if (wave == 0 && lane == 0 && fd.spikes && n_prev) *fd.spikes += (unsigned long long)n_prev;
+  return n_dev;
+}
+
+// Clears the count of the spike list the next step writes (with fused
+// delivery no delivery launch does: it held the list of step t_idx - 2,
+// delivered by the previous launch).
+__device__ inline void clear_next_spike_list(const SpikeList& sl, int lst) {
+  sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
+}
+
+// Register budget of a small model's kernel (at most 6 state variables): the
+// plain kernel is held to 128 VGPRs (4 waves per SIMD) and the
+// learned-proposal kernel to 168 (3 waves); the compiler would otherwise take
+// a few more and drop a wave per SIMD.
+template <class M, bool UNI, int MODE>
+__global__ void __launch_bounds__(kRK5Block)
+__attribute__((amdgpu_waves_per_eu(M::NV > 6 ? 1 : MODE == 2 ? 3 : MODE == 0 ? 4 : 1)))
+rk5_persistent_kernel(RK5Args a) {
+  __shared__ int s_hot[kRK5Block / kWave][kHotLds];
+  __shared__ int s_spk[kRK5Block / kWave][kSpkLds];
+  __shared__ PredLDS s_pred;
+  if constexpr (MODE == 2) {
+    pred_stage(s_pred, a.pred_params);
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = blockIdx.x * (blockDim.x / kWave) +
+                   __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int n_waves = gridDim.x * (blockDim.x / kWave);
+  if (a.clear_next_list && blockIdx.x == 0 && threadIdx.x == 0) clear_next_spike_list(a.sl, a.ctx.lst);
+  const StepFromArgs sv{a};
+  unsigned long long n_dev = 0;   // synapses delivered by this lane (fused delivery)
+  rk5_persistent_body<M, UNI, MODE>(a, sv, wave, n_waves, wave, s_hot[threadIdx.x / kWave],
+                                    s_spk[threadIdx.x / kWave], &s_pred, n_dev);
+  if (a.fd.enable) n_dev += deliver_prev_list(a.fd, a.sl, wave, n_waves, lane);
+  if (a.ctr) {
+    n_dev = wave_sum(n_dev);
+    if (lane == 0 && n_dev) a.ctr[(size_t)wave * kCtrCols + 3] += n_dev;
+  }
 }
 
 inline int persistent_grid(int n, int max_blocks) {

@@ -14,6 +14,19 @@ predictor_adex (the learned step-size benchmark): the same neuron with 4
   Poisson generator per neuron with rate U(5,40) kHz at 0.2 nS on port 0
   (mean ~9 Hz, a mix of silent and fast-firing neurons).
 
+batching (six populations of five models in one launch per step): 300 and
+  3,000 izhikevich_cond_beta (2 ports), 30,000 aeif_psc_exp, 120,000
+  aeif_cond_alpha_multisynapse (4 ports), 300,000 aeif_cond_beta_multisynapse
+  (2 ports) and 50,000 iaf_psc_exp (503,300 neurons); the first 80% of the
+  ids are the excitatory sources; every neuron has fixed indegree 500 (400
+  excitatory sources -> port 0, 100 inhibitory -> port 1) over all
+  populations, 2.5e8 synapses; weights normal with sd 10% of the mean, the
+  inhibitory mean 4x the excitatory one (as the microcircuit), the means per
+  target model (nS for conductance ports, pA for current ports; inhibitory
+  current weights negative for iaf_psc_exp, positive on aeif_psc_exp's
+  inhibitory port, which subtracts); delays N(1.5, 0.75) ms exc, N(0.75,
+  0.375) ms inh, at least 0.1 ms; one Poisson generator per population.
+
 microcircuit (the delivery-bound iaf_psc_exp benchmark): 8 populations of
   the cortical microcircuit's sizes (77,169 neurons; microcircuit neuron
   parameters of python/Potjans_2014_s/network_params.py: E_L -65 mV, V_th
@@ -103,6 +116,69 @@ def predictor_adex(ngpu, n=1_000_000, seed=200):
     return neurons
 
 
+# model, size, ports, status, excitatory mean weight, inhibitory sign,
+# Poisson rate (Hz), Poisson weight
+BATCH_POPS = [
+    ("izhikevich_cond_beta", 300, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [2.0, 2.0]}, 0.0004, 1.0,
+     8000.0, 0.006),
+    ("izhikevich_cond_beta", 3000, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [2.0, 2.0]}, 0.0004, 1.0,
+     8000.0, 0.006),
+    ("aeif_psc_exp", 30000, 2, {}, 4.0, 1.0, 20000.0, 170.0),
+    ("aeif_cond_alpha_multisynapse", 120000, 4,
+     {"E_rev": [0.0, -85.0, 0.0, -85.0], "tau_syn": [1.0, 1.0, 5.0, 5.0]}, 0.065, 1.0,
+     20000.0, 0.25),
+    ("aeif_cond_beta_multisynapse", 300000, 2,
+     {"E_rev": [0.0, -85.0], "tau_rise": [1.0, 1.0], "tau_decay": [1.0, 1.0]}, 0.1, 1.0,
+     20000.0, 0.3),
+    ("iaf_psc_exp", 50000, 2, {}, 9.0, -1.0, 8000.0, 22.0),
+]
+BATCH_K = (400, 100)
+
+
+def batching(ngpu, n=None, seed=300):
+    """Build the six-population batching network (n is ignored: the
+    population sizes are fixed); returns the NodeSeq of all its neurons."""
+    rng = np.random.default_rng(seed)
+    ngpu.SetRandomSeed(seed)
+    pops = []
+    for (model, m, npt, st, _, _, _, _) in BATCH_POPS:
+        pop = ngpu.Create(model, m, npt)
+        if st:
+            ngpu.SetStatus(pop, st)
+        pops.append(pop)
+    i0 = pops[0].i0
+    n_tot = sum(p[1] for p in BATCH_POPS)
+    n_exc = int(0.8 * n_tot)
+    exc = ngpu.NodeSeq(i0, n_exc)
+    inh = ngpu.NodeSeq(i0 + n_exc, n_tot - n_exc)
+    lib = ngpu._nestgpu
+    for pop, (model, m, npt, st, we, sgn, rate, pw) in zip(pops, BATCH_POPS):
+        for src, k, port, mean, d_mu, d_sd in ((exc, BATCH_K[0], 0, we, 1.5, 0.75),
+                                               (inh, BATCH_K[1], 1, 4.0 * we * sgn, 0.75, 0.375)):
+            cnt = m * k
+            w = rng.normal(mean, abs(mean) * 0.1, cnt).astype(np.float32)
+            w = np.maximum(w, 0.0) if mean > 0 else np.minimum(w, 0.0)
+            d = np.maximum(rng.normal(d_mu, d_sd, cnt), DT).astype(np.float32)
+            ngpu.ConnSpecInit()
+            ngpu.SynSpecInit()
+            ngpu.SetConnSpecParam("rule", 3)           # fixed_indegree
+            ngpu.SetConnSpecParam("indegree", k)
+            ngpu.SetSynSpecIntParam("receptor", port)
+            lib.NESTGPU_SetSynSpecFloatPtParam(b"weight_array", w.ctypes.data_as(ctypes.c_void_p))
+            lib.NESTGPU_SetSynSpecFloatPtParam(b"delay_array", d.ctypes.data_as(ctypes.c_void_p))
+            lib.NESTGPU_ConnectSeqSeq(src.i0, src.n, pop.i0, pop.n)
+            ngpu._check()
+    for pop, (model, m, npt, st, we, sgn, rate, pw) in zip(pops, BATCH_POPS):
+        pg = ngpu.Create("poisson_generator")
+        ngpu.SetStatus(pg, "rate", rate)
+        ngpu.Connect(pg, pop, {"rule": "all_to_all"}, {"weight": pw, "delay": DT, "receptor": 0})
+    batching.pops = pops
+    batching.n_syn = n_tot * sum(BATCH_K)
+    return ngpu.NodeSeq(i0, n_tot)
+
+
 MC_SIZES = [20683, 5834, 21915, 5479, 4850, 1065, 14395, 2948]
 MC_K_EXT = [1600, 1500, 2100, 1900, 2000, 1900, 2900, 2100]
 MC_PARAMS = {"E_L": -65.0, "Theta_rel": 15.0, "V_reset_rel": 0.0, "C_m": 250.0, "tau_m": 10.0,
@@ -152,4 +228,5 @@ def microcircuit(ngpu, n=None, seed=7):
     return ngpu.NodeSeq(pops[0].i0, sum(MC_SIZES))
 
 
-WORKLOADS = {"scale_adex": scale_adex, "predictor_adex": predictor_adex, "microcircuit": microcircuit}
+WORKLOADS = {"scale_adex": scale_adex, "predictor_adex": predictor_adex, "microcircuit": microcircuit,
+             "batching": batching}

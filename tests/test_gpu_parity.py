@@ -274,6 +274,58 @@ def test_iaf_psc_exp_network_bitexact(ngpu, runs, monkeypatch):
     np.testing.assert_array_equal(ng, no)
 
 
+def test_iaf_psc_exp_network_tiled_bitexact(ngpu, monkeypatch):
+    # the same network with the dense tiled delivery (LDS tile sums added to
+    # the ring once per step) instead of global atomics: identical raster
+    monkeypatch.setenv("NGPU_TILED", "1")
+    net = netbuild.c1_network(seed=3, n=600, k=60, t_rate=8000.0)
+    net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2] * 60.0, net["exc"][3])
+    net["inh"] = (net["inh"][0], net["inh"][1], net["inh"][2] * -60.0, net["inh"][3])
+    tg, ng = _gpu_network(ngpu, net, "iaf_psc_exp", 300.0, poiss_w=30.0)
+    cfg = ngpu.GetBatchConfig()
+    to, no = _oracle_network(net, "iaf_psc_exp", 300.0, poiss_w=30.0)
+    print("iaf tiled spikes gpu", len(tg), "oracle", len(to), cfg)
+    assert cfg["tiled"] == 1
+    assert len(to) > 500
+    np.testing.assert_array_equal(tg, to)
+    np.testing.assert_array_equal(ng, no)
+
+
+def test_tiled_delivery_rk5_split_steps(ngpu):
+    # one RK5 population with split steps (lagged spikes, their d_int = 0
+    # synapses one slot earlier): the tiled delivery launch after every step
+    # against fused delivery inside the next launch: identical rasters, state
+    # and attempt counts, over two Simulate calls
+    net = netbuild.c1_network(seed=7)
+    res = []
+    for tiled in ("1", "0"):
+        os.environ["NGPU_TILED"] = tiled
+        os.environ["NGPU_RK5_DEFER"] = "3"
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            t, n = _gpu_network(ngpu, net, "aeif_cond_beta_multisynapse", 60.0,
+                                poiss_w=netbuild.C1_POISSON_W, status=netbuild.C1_RECEPTORS)
+            cfg = ngpu.GetBatchConfig()
+            nodes = ngpu.NodeSeq(0, net["n"])
+            ngpu.Simulate(20.0)
+            v = np.array(ngpu.GetStatus(nodes, "V_m"))
+            st = ngpu.GetRecSpikeTimes(nodes)
+            res.append((t, n, v, [list(x) for x in st], ngpu.GetStepStats(), cfg))
+        finally:
+            os.environ.pop("NGPU_TILED", None)
+            os.environ.pop("NGPU_RK5_DEFER", None)
+    (t1, n1, v1, st1, s1, c1), (t0, n0, v0, st0, s0, c0) = res
+    print("tiled", c1, "split steps", s1["split_steps"], "spikes", len(t1))
+    assert c1["tiled"] == 1 and c0["tiled"] == 0
+    assert s1["split_steps"] > 300 and sum(len(x) for x in st1) > 50
+    np.testing.assert_array_equal(t1, t0)
+    np.testing.assert_array_equal(n1, n0)
+    np.testing.assert_array_equal(v1, v0)
+    assert st1 == st0
+    assert s1["attempts"] == s0["attempts"] and s1["syn_events"] == s0["syn_events"]
+
+
 def _iaf_one_to_one_poisson(ngpu, net, rates):
     nrn = ngpu.Create("iaf_psc_exp", net["n"], 2)
     pg = ngpu.Create("poisson_generator", net["n"])
@@ -482,6 +534,94 @@ def test_mixed_model_network(ngpu, split):
         assert st["split_steps"] > 1000
     np.testing.assert_array_equal(tg, to)
     np.testing.assert_array_equal(ng, no)
+
+
+def _mixed_run_calls(ngpu, net, calls, batch_cfg=None, autotune_after=None):
+    """The mixed network over consecutive Simulate calls of the given lengths
+    (ms); optional batch configuration, and an autotuner run (steps per
+    candidate) after the first call.  Returns raster, final V of every
+    population and the step counters."""
+    pops = []
+    for p in net["pops"]:
+        nrn = ngpu.Create(p["model"], p["n"], p["n_port"])
+        if p["status"]:
+            ngpu.SetStatus(nrn, p["status"])
+        pops.append(nrn)
+    for p, nrn in zip(net["pops"], pops):
+        pg = ngpu.Create("poisson_generator")
+        ngpu.SetStatus(pg, "rate", p["rate"])
+        ngpu.Connect(pg, nrn, {"rule": "all_to_all"},
+                     {"weight": p["pw"], "delay": 0.1, "receptor": 0})
+    for (s, t, port, w, d) in net["edges"]:
+        for q in (0, 1):
+            m = port == q
+            netbuild.connect_edges_fast(ngpu, s[m], t[m], q, w[m], d[m])
+    for nrn in pops:
+        ngpu.ActivateRecSpikeTimes(nrn, 5000)
+    ngpu.Calibrate()
+    if batch_cfg is not None:
+        ngpu.SetBatchConfig(**batch_cfg)
+    cfg = ngpu.GetBatchConfig()
+    tuned = None
+    for k, ms in enumerate(calls):
+        ngpu.Simulate(ms)
+        if k == 0 and autotune_after:
+            tuned = ngpu.AutotuneBatch(autotune_after)
+    t_idx, node = [], []
+    for nrn in pops:
+        for i, ts in enumerate(ngpu.GetRecSpikeTimes(nrn)):
+            for tt in ts:
+                t_idx.append(int(round(tt / 0.1)))
+                node.append(i + nrn[0])
+    t_idx = np.array(t_idx, np.int64)
+    node = np.array(node, np.int32)
+    o = np.lexsort((node, t_idx))
+    v = [np.array(ngpu.GetStatus(nrn, "V_m_rel" if p["model"] == "iaf_psc_exp" else "V_m"),
+                  np.float32).reshape(-1) for p, nrn in zip(net["pops"], pops)]
+    return t_idx[o], node[o], np.concatenate(v), ngpu.GetStepStats(), cfg, tuned
+
+
+def test_batch_kernel_matches_population_launches(ngpu):
+    # the six-population mixed network as one batch launch per step (graph
+    # replay, default configuration), as direct batch launches with other
+    # register / unroll / block-weight settings, and after the autotuner has
+    # run on live steps, against one launch per population: rasters, final
+    # membrane potentials and RK5 attempt counts are identical (scheduling
+    # never changes arithmetic); two or three Simulate calls each
+    net = netbuild.mixed_network(seed=11)
+    runs = {}
+    cases = [("populations", "0", "0", None, None, (50.0, 50.0)),
+             ("batch", "1", "0", None, None, (50.0, 50.0)),
+             ("batch_direct", "1", "0",
+              dict(variant=1, iaf_unroll=4, rk5_weight=2.0, graph=False), None, (50.0, 50.0)),
+             ("batch_tiled", "1", "1", None, None, (50.0, 50.0)),
+             ("autotuned", "1", "1", None, 4, (30.0, 70.0 - 36 * 4 * 0.1))]
+    for name, env, tiled, cfg, tune, calls in cases:
+        print("batch case", name, flush=True)
+        os.environ["NGPU_BATCH"] = env
+        os.environ["NGPU_TILED"] = tiled
+        try:
+            ngpu.ResetKernel()
+            ngpu.SetVerbosityLevel(0)
+            runs[name] = _mixed_run_calls(ngpu, net, calls, cfg, tune)
+        finally:
+            os.environ.pop("NGPU_BATCH", None)
+            os.environ.pop("NGPU_TILED", None)
+    t0, n0, v0, s0, c0, _ = runs["populations"]
+    assert c0["on"] == 0
+    assert len(t0) > 500
+    for name in ("batch", "batch_direct", "batch_tiled", "autotuned"):
+        t, n, v, st, cfg, tuned = runs[name]
+        print(name, cfg, "tuned", tuned, "spikes", len(t), "split", st["split_steps"])
+        assert cfg["on"] == 1 and cfg["populations"] == 6
+        if name == "batch_tiled":
+            assert cfg["tiled"] == 1
+        np.testing.assert_array_equal(t, t0)
+        np.testing.assert_array_equal(n, n0)
+        np.testing.assert_array_equal(v, v0)
+        assert st["attempts"] == s0["attempts"] and st["rejected"] == s0["rejected"]
+        assert st["syn_events"] == s0["syn_events"] and st["spikes"] == s0["spikes"]
+    assert runs["autotuned"][5][1] == 36     # with the tiled delivery as a knob
 
 
 def test_split_steps_match_unsplit(ngpu):
