@@ -295,6 +295,20 @@ int NESTGPU_AMD_PredictorInfer(const float *raw, long long n, float *out);
 int NESTGPU_AMD_PredictorLossGrad(const float *raw, const float *target,
                                   const unsigned char *censored, int n, float *loss,
                                   float *grad);
+/* NESTGPU_AMD_PredictorLossGrad without any allocation (no counterpart in
+ * the reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Every pointer is DEVICE memory:
+ * raw[n][8], target[n], censored[n] in; loss[1] and grad[1377] out.  The
+ * launches are queued on `stream` (a hipStream_t; NULL = the null stream)
+ * and the call returns without waiting.  The partial sums live in a fixed
+ * array of the library (1.4 MB per GPU, held until the process ends), so
+ * n must be in [1, 16384]: any other n launches nothing and sets error code
+ * 2 with a message (NESTGPU_GetErrorCode/GetErrorMessage).  The
+ * array is shared by every call on one GPU: calls that overlap on one GPU
+ * race.  Results equal NESTGPU_AMD_PredictorLossGrad's bit for bit. */
+int NESTGPU_AMD_PredictorLossGradNoAlloc(const float *raw, const float *target,
+                                         const unsigned char *censored, int n, float *loss,
+                                         float *grad, void *stream);
 
 #ifdef __cplusplus
 }

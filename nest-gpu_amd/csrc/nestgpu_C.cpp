@@ -733,5 +733,14 @@ int NESTGPU_AMD_AutotuneBatch(int steps, double* ms_per_step, int* n_candidates)
     return 0;
   });
 }
+int NESTGPU_AMD_PredictorLossGradNoAlloc(const float* raw, const float* target,
+                                         const unsigned char* censored, int n, float* loss,
+                                         float* grad, void* stream) {
+  return guarded(0, [&] {
+    ngpu::Predictor::get().LossGradNoAlloc(raw, target, censored, n, loss, grad,
+                                           static_cast<hipStream_t>(stream));
+    return 0;
+  });
+}
 
 }  // extern "C"

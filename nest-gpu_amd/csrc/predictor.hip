@@ -11,6 +11,10 @@
 // the update kernel adds the rows in block order.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "philox.h"
 #include "predictor_device.h"
 #include "predictor_kernels.h"
@@ -86,7 +90,9 @@ __global__ void __launch_bounds__(64) pred_grad_kernel(PredGradArgs a) {
   for (int rd = 0; rd < a.rounds; ++rd) {
     const long long slot = ((long long)blockIdx.x * a.rounds + rd) * 64 + lane;
     long long src = -1;
-    if (a.idx) {
+    if (a.raw) {
+      src = slot < a.n_log ? slot : -1;
+    } else if (a.idx) {
       src = slot < a.n_idx ? a.idx[slot] : -1;
     } else {
       const U32x4 ctr = {(uint32_t)slot, (uint32_t)(slot >> 32), (uint32_t)a.step,
@@ -98,7 +104,13 @@ __global__ void __launch_bounds__(64) pred_grad_kernel(PredGradArgs a) {
     float x[kPI], z1[kPH], h1[kPH], z2[kPH], dz2[kPH], dout = 0.0f;
     float t = 0.0f;
     bool cens = false;
-    if (src >= 0) {
+    if (src >= 0 && a.raw) {
+      const float* rec = a.raw + src * kPI;
+#pragma unroll
+      for (int i = 0; i < kPI; ++i) x[i] = bf16_round((rec[i] - L.mu[i]) * L.is[i]);
+      t = a.target[src];
+      cens = a.cens[src] != 0;
+    } else if (src >= 0) {
       const float* rec = a.log + src * kLogCols;
 #pragma unroll
       for (int i = 0; i < kPI; ++i) x[i] = bf16_round((rec[i] - L.mu[i]) * L.is[i]);
@@ -244,6 +256,73 @@ void launch_pred_infer(const float* P, const float* raw, long long n, float* out
   if (grid > 1024) grid = 1024;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(pred_infer_kernel, dim3(grid), dim3(256), 0, s, P, raw, n, out);
+}
+
+// ---- loss and gradient without allocation ------------------------------------
+// The partial gradient rows and loss sums of pred_grad_kernel live in this
+// fixed array (1.4 MB per GPU, held until the process ends): rows
+// [kPredScratchMaxBlocks][kPNTrain], then kPredScratchMaxBlocks loss sums.
+// Every block writes its whole row and loss sum and the update kernel reads
+// only the rows of the current call, so nothing needs clearing between calls.
+constexpr size_t kPredScratchFloats =
+    (size_t)kPredScratchMaxBlocks * kPNTrain + kPredScratchMaxBlocks;
+static __device__ __align__(256) float scratch[kPredScratchFloats];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address; the
+// symbol's name in host code is not that address).
+static float* scratch_ptr() {
+  constexpr int kMaxDev = 64;
+  static float* ptr[kMaxDev] = {};
+  static std::mutex mu;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDev) throw std::runtime_error("device index out of range for the predictor scratch");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ptr[dev] = static_cast<float*>(p);
+  }
+  return ptr[dev];
+}
+
+void pred_loss_grad_noalloc(const float* params, const float* raw, const float* target,
+                            const unsigned char* cens, int n, float* loss, float* grad,
+                            hipStream_t s) {
+  if (n < 1 || n > kPredScratchMaxN)
+    throw std::runtime_error("predictor loss/gradient without allocation: batch of " +
+                             std::to_string(n) + " records outside [1, " +
+                             std::to_string(kPredScratchMaxN) + "]");
+  float* base = scratch_ptr();
+  const int blocks = (n + 63) / 64;
+  PredGradArgs ga{};
+  ga.params = params;
+  ga.n_log = n;
+  ga.raw = raw;
+  ga.target = target;
+  ga.cens = cens;
+  ga.rounds = 1;
+  ga.batch = n;
+  ga.gpart = base;
+  ga.lpart = base + (size_t)kPredScratchMaxBlocks * kPNTrain;
+  launch_pred_grad(ga, blocks, s);
+  PredUpdateArgs ua{};
+  ua.params = const_cast<float*>(params);   // lr = 0: read only
+  ua.gpart = ga.gpart;
+  ua.lpart = ga.lpart;
+  ua.blocks = blocks;
+  ua.batch = n;
+  ua.lr = 0.0f;
+  ua.grad_out = grad;
+  ua.loss_out = loss;
+  launch_pred_update(ua, s);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("predictor loss/gradient launch: ") + hipGetErrorString(e));
 }
 
 }  // namespace ngpu

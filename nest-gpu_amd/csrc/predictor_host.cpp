@@ -263,4 +263,19 @@ void Predictor::LossGrad(const float* raw, const float* target, const unsigned c
   PCHK(hipStreamSynchronize(stream_));
 }
 
+void Predictor::LossGradNoAlloc(const float* raw, const float* target, const unsigned char* cens,
+                                int n, float* loss, float* grad, hipStream_t stream) {
+  if (!have_params_) throw Error("Step predictor has no parameters");
+  if (!raw || !target || !cens || !loss || !grad) throw Error("Step predictor: null buffer");
+  if (n < 1 || n > kPredScratchMaxN)
+    throw Error("Step predictor: batch of " + std::to_string(n) +
+                " records outside [1, " + std::to_string(kPredScratchMaxN) +
+                "] for the loss/gradient without allocation");
+  try {
+    pred_loss_grad_noalloc(params_.p, raw, target, cens, n, loss, grad, stream);
+  } catch (const std::exception& e) {
+    throw Error(e.what());
+  }
+}
+
 }  // namespace ngpu

@@ -28,6 +28,9 @@ class Predictor {
   void Infer(const float* raw, long long n, float* out);
   void LossGrad(const float* raw, const float* target, const unsigned char* cens, int n,
                 float* loss, float* grad);
+  // LossGrad on device buffers, asynchronous on `stream`, allocating nothing
+  void LossGradNoAlloc(const float* raw, const float* target, const unsigned char* cens, int n,
+                       float* loss, float* grad, hipStream_t stream);
 
   // buffers read by the RK5 kernel
   const float* d_params() const { return params_.p; }
