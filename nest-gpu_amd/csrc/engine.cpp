@@ -1602,7 +1602,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_POISSON_RUNS")) poisson_runs_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_FOLD_IN_RK5")) fold_in_rk5_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_FUSED_DELIVER")) fused_deliver_env_ = atoi(env) != 0;
-  if (const char* env = getenv("NGPU_LAG_D0_AT_EXIT")) lag_d0_at_exit_ = atoi(env) != 0;
+  if (const char* env = getenv("NGPU_LAG_D0")) lag_d0_mode_ = atoi(env) == 1 ? 1 : 0;
   if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
@@ -1873,7 +1873,7 @@ RK5Args Engine::rk5_args(Group& gr, const StepCtx& sc, const SpikeList& sl, cons
     }
   }
   if (fused()) {
-    a.fd.lag_d0_at_exit = lag_d0_at_exit_ ? 1 : 0;
+    a.fd.lag_d0_at_exit = lag_d0_mode_;
     a.fd.skip_lagged_d0 = 1;
     a.fd.st = syn_table();
     a.fd.spikes = d_stats_.p;
@@ -1952,10 +1952,10 @@ void Engine::launch_step() {
   // next step's RK5 launch, except after the last step of a Simulate call
   // (or a step driven by SimulationStep), which leaves nothing pending
   if (!fused() || !defer_ok_) {
-    deliver_list(sc, sl, fused() && lag_d0_at_exit_ ? 1 : 0);
+    deliver_list(sc, sl, fused() && lag_d0_mode_ != 0 ? 1 : 0);
     prev_list_pending_ = false;
   } else {
-    if (!lag_d0_at_exit_) {
+    if (lag_d0_mode_ == 0) {
       // the d_int = 0 synapses of this step's lagged spikes, before the next
       // launch folds their slot
       DeliverArgs da{};
@@ -2140,7 +2140,7 @@ void Engine::batch_step(const StepCtx& sc, const SpikeList& sl) {
   b.sl = sl;
   const bool fu = !tiled_use_;   // the next launch delivers this step's list
   b.clear_next_list = fu ? 1 : 0;
-  b.fd.lag_d0_at_exit = fu && lag_d0_at_exit_ ? 1 : 0;
+  b.fd.lag_d0_at_exit = fu ? lag_d0_mode_ : 0;
   b.fd.skip_lagged_d0 = 1;
   b.fd.st = syn_table();
   b.fd.spikes = d_stats_.p;
@@ -2169,7 +2169,7 @@ void Engine::batch_step(const StepCtx& sc, const SpikeList& sl) {
     // that launch folds
     if (!fu) {
       deliver_list(sc, sl, 0);
-    } else if (!lag_d0_at_exit_) {
+    } else if (lag_d0_mode_ == 0) {
       DeliverArgs da{};
       da.ctx = sc; da.sl = sl; da.st = syn_table();
       da.ctr = d_ctr_.p;
@@ -2181,7 +2181,7 @@ void Engine::batch_step(const StepCtx& sc, const SpikeList& sl) {
   } else {
     // last step of a Simulate call: deliver this step's list and fold every
     // population, so the state is complete between calls
-    deliver_list(sc, sl, fu && lag_d0_at_exit_ ? 1 : 0);
+    deliver_list(sc, sl, fu && lag_d0_mode_ != 0 ? 1 : 0);
     prev_list_pending_ = false;
     batch_fold_pending_ = false;
     for (int gi : batch_groups_) {
@@ -2273,26 +2273,26 @@ double Engine::AutotuneBatch(int steps, int* n_candidates) {
 }
 
 // Tiled copy of the synapse table for dense delivery (TiledSyn): tiles of
-// tile_ch channels, tile_ch the largest multiple of 32 whose all-slot FP64
-// tile fits kTileLdsBytes.  Built when a step's events may outnumber the
-// ring's entries (at least 4 synapses per ring entry; NGPU_TILED=1 builds it
-// whenever it fits, 0 never) and the (source, tile) table stays below 2^28
-// entries.  Within a source the synapses keep their order inside a tile.
+// tile_ch channels, tile_ch the largest multiple of 32 (at most 256) whose
+// all-slot FP64 tile fits kTileLdsBytes.  Built when a step's events may
+// outnumber the ring's entries (at least 4 synapses per ring entry;
+// NGPU_TILED=1 builds it whenever it fits, 0 never), delay slots stay below
+// 256 and the (source, tile) table stays below 2^28 entries.  Within a
+// source the records keep their order inside a tile.
 void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd) {
   tiled_on_ = tiled_use_ = false;
-  d_toff_.free(); d_tch_.free(); d_tw_.free(); d_td_.free();
+  d_toff_.free(); d_trec_.free(); d_twork_.free();
   const size_t nc = ch.size();
-  if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0) return;
-  const int tc = (kTileLdsBytes / (n_slot_ * (int)sizeof(double))) / 32 * 32;
+  if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
+  const int tc = std::min(256, (kTileLdsBytes / (n_slot_ * (int)sizeof(double))) / 32 * 32);
   if (tc < 32) return;
   const long long nt = (n_ch_ + tc - 1) / tc;
   const long long entries = (long long)n_nodes_ * (nt + 1);
   if (entries >= (1ll << 28)) return;
   const bool dense = (double)nc >= 4.0 * (double)n_ch_ * (double)n_slot_;
   if (tiled_env_ < 0 && !dense) return;
-  std::vector<unsigned int> off((size_t)entries), tch(nc);
-  std::vector<float> tw(nc);
-  std::vector<unsigned short> td(nc);
+  std::vector<unsigned int> off((size_t)entries);
+  std::vector<TileRec> rec(nc);
 #pragma omp parallel
   {
     std::vector<unsigned int> pos(nt + 1);
@@ -2309,17 +2309,17 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
         pos[t] = run;   // next free position of tile t
       }
       for (long long k = b; k < e; ++k) {
-        const unsigned int j = pos[ch[k] / tc]++;
-        tch[j] = ch[k];
-        tw[j] = c_w_[k];
-        td[j] = dd[k];
+        const unsigned int t = ch[k] / tc;
+        TileRec& r = rec[pos[t]++];
+        r.ch = (unsigned char)(ch[k] - t * (unsigned int)tc);
+        r.d = (unsigned char)dd[k];
+        r.pad = 0;
+        r.w = c_w_[k];
       }
     }
   }
   upload(d_toff_, off, stream_);
-  upload(d_tch_, tch, stream_);
-  upload(d_tw_, tw, stream_);
-  upload(d_td_, td, stream_);
+  upload(d_trec_, rec, stream_);
   sync_streams();
   tile_ch_ = tc;
   n_tiles_ = (int)nt;
@@ -2330,9 +2330,10 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
 // Delivery launch of this step's list: dense (tiled) or global atomics.
 void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagged_d0) {
   if (tiled_use_) {
+    if (!d_twork_.p) d_twork_.alloc((size_t)n_tiles_ * std::max(spk_cap_, 1));
     TiledDeliverArgs ta{};
     ta.ctx = sc; ta.sl = sl;
-    ta.ts = TiledSyn{d_toff_.p, d_tch_.p, d_tw_.p, d_td_.p, n_tiles_, tile_ch_};
+    ta.ts = TiledSyn{d_toff_.p, d_trec_.p, d_twork_.p, n_tiles_, tile_ch_};
     ta.ring = d_ring_.p; ta.n_ch = n_ch_; ta.n_slot = n_slot_;
     ta.skip_lagged_d0 = skip_lagged_d0;
     ta.ctr = d_ctr_.p; ta.spikes = d_stats_.p;
@@ -2574,5 +2575,7 @@ template struct DBuf<unsigned char>;
 template struct DBuf<unsigned long long>;
 template struct DBuf<StepCtx>;
 template struct DBuf<RecCol>;
+template struct DBuf<TileRec>;
+template struct DBuf<uint2>;
 
 }  // namespace ngpu

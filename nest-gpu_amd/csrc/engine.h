@@ -334,15 +334,17 @@ class Engine {
   bool fused_deliver_env_ = true;      // allow the RK5 launch to deliver the previous step's spikes
   bool fused_deliver_ = false;         // ... and the network qualifies (one emitting RK5 population)
   bool prev_list_pending_ = false;     // the previous step's list awaits delivery by the next RK5 launch
-  bool lag_d0_at_exit_ = false;        // lagged spikes' d_int = 0 synapses by the emitting wave (else a launch)
+  // lagged spikes' d_int = 0 synapses (fused delivery): 0 a launch after the
+  // update launch, 1 by the emitting wave at its exit
+  int lag_d0_mode_ = 0;
   // dense (tiled) delivery
   int tiled_env_ = -1;                 // NGPU_TILED: -1 when dense, 0 never, 1 whenever the table fits
   bool tiled_on_ = false;              // the tiled copy of the synapse table exists
   bool tiled_use_ = false;             // deliver with it (a batch autotuner knob)
   int tile_ch_ = 0, n_tiles_ = 0;
-  DBuf<unsigned int> d_toff_, d_tch_;
-  DBuf<float> d_tw_;
-  DBuf<unsigned short> d_td_;
+  DBuf<unsigned int> d_toff_;
+  DBuf<TileRec> d_trec_;
+  DBuf<uint2> d_twork_;               // per-step (first, end) runs [n_tiles][n_spk]
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   // batch kernel

@@ -98,7 +98,7 @@ struct SynTable {
 // loads and atomics at the end of the spike chains, which end the launch).
 struct FusedDeliver {
   int enable;            // deliver the previous step's list in this launch
-  int lag_d0_at_exit;    // lagged spikes: d_int = 0 synapses at wave exit, the rest later
+  int lag_d0_at_exit;    // lagged spikes' d_int = 0 synapses: 0 by a launch, 1 at wave exit
   int skip_lagged_d0;    // the previous list's lagged spikes had their d_int = 0 synapses delivered
   int prev_lst;          // list index of the previous step
   long long prev_t_idx;  // its time index
@@ -203,18 +203,28 @@ struct ParrotArgs {
 
 // Copy of the synapse table for dense delivery (deliver_tiled_kernel): input
 // channels are cut into tiles of tile_ch consecutive channels; within each
-// source's segment the synapses are grouped by tile, and off holds the first
-// synapse of every (source, tile) pair.
+// source's segment the synapses are grouped by tile, each as one 8-byte
+// record (channel within the tile, delay slot, weight), and off holds the
+// first record of every (source, tile) pair.
+struct TileRec {
+  unsigned char ch;      // channel - tile * tile_ch
+  unsigned char d;       // delay slot d_int
+  unsigned short pad;
+  float w;
+};
 struct TiledSyn {
   const unsigned int* off;     // [n_nodes][n_tiles + 1]
-  const unsigned int* syn_ch;  // [n_syn] target channel
-  const float* syn_w;          // [n_syn] weight
-  const unsigned short* syn_d; // [n_syn] delay slot d_int
+  const TileRec* rec;          // [n_syn]
+  uint2* work;                 // [n_tiles][n_spk]: (first, end) record of each (tile, spike)
   int n_tiles, tile_ch;
 };
 
-// Dense delivery of one step's spike list: one block per channel tile sums
-// every spike's synapses into that tile in LDS (FP64, all n_slot slots), then
+// Dense delivery of one step's spike list, two launches: tile_work_kernel
+// turns the offset rows of the list's sources (source-major) into per-tile
+// runs of (first, end) pairs (tile-major, through an LDS transpose, so both
+// sides move whole lines), then deliver_tiled_kernel gives each block one
+// tile of channels: its threads take the spikes, add every record of
+// (spike, tile) into the LDS tile acc[slot][channel] (FP64), and the block
 // adds the tile into the ring once.  Replaces global atomics by LDS atomics
 // and one coalesced read-modify-write per ring entry that received input,
 // for networks where a step's events outnumber the ring's entries.
@@ -230,7 +240,8 @@ struct TiledDeliverArgs {
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
-// LDS bytes of one tile block; tile_ch = this / (n_slot * 8), a multiple of 32
+// LDS bytes of one tile block; tile_ch = this / (n_slot * 8), a multiple of
+// 32 and at most 256 (the record's channel byte); delay slots below 256
 constexpr int kTileLdsBytes = 64 * 1024;
 
 struct DeliverArgs {

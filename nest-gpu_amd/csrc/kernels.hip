@@ -230,14 +230,49 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(deliver_kernel, dim3(kDeliverBlocks), dim3(256), 0, s, a);
 }
 
-// Dense delivery (TiledDeliverArgs): block b owns channel tile b (and b +
-// gridDim.x, ...).  Its threads take the list's spikes one each, add every
-// synapse of (spike source, tile) into the LDS tile acc[slot][channel] with
-// FP64 LDS atomics, and then the tile is added into the ring (plain
+// Dense delivery (TiledDeliverArgs), first launch: block (x, y) takes tiles
+// [64x, 64x + 64) and the list's spikes in chunks of 64 from chunk y on; it
+// reads the 65 offsets of each spike's source row for those tiles (one run of
+// a row each), transposes them in LDS, and writes for each tile the chunk's
+// (first, end) pairs contiguously at work[tile][spike].
+constexpr int kWorkChunk = 64;
+
+__global__ void __launch_bounds__(256) tile_work_kernel(TiledDeliverArgs a) {
+  __shared__ unsigned int sm[kWorkChunk][kWorkChunk + 2];
+  const int nt = a.ts.n_tiles;
+  const int lst = a.ctx.lst;
+  int n_spk = a.sl.count[lst];
+  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
+  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
+  const int t0 = blockIdx.x * kWorkChunk;
+  const int nt_here = nt - t0 < kWorkChunk ? nt - t0 : kWorkChunk;
+  for (int k0 = blockIdx.y * kWorkChunk; k0 < n_spk; k0 += gridDim.y * kWorkChunk) {
+    const int nk = n_spk - k0 < kWorkChunk ? n_spk - k0 : kWorkChunk;
+    for (int idx = threadIdx.x; idx < nk * (kWorkChunk + 1); idx += blockDim.x) {
+      const int k = idx / (kWorkChunk + 1), t = idx - k * (kWorkChunk + 1);
+      if (t <= nt_here) {
+        const int src = (int)((unsigned int)srcs[k0 + k] & ~kSpikeLagBit);
+        sm[k][t] = a.ts.off[(size_t)src * (nt + 1) + t0 + t];
+      }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nt_here * kWorkChunk; idx += blockDim.x) {
+      const int t = idx / kWorkChunk, k = idx - t * kWorkChunk;
+      if (k < nk) a.ts.work[(size_t)(t0 + t) * n_spk + k0 + k] = make_uint2(sm[k][t], sm[k][t + 1]);
+    }
+    __syncthreads();
+  }
+}
+
+// Second launch: block b owns channel tile b (and b + gridDim.x, ...).  Its
+// threads take the list's spikes one each, read the (first, end) run of
+// (spike, tile) (consecutive threads, consecutive pairs), and add every
+// record's (double)(height * weight) into the LDS tile acc[slot][channel]
+// with FP64 LDS atomics; then the block adds the tile into the ring (plain
 // read-modify-write: no other block touches these ring entries, and no other
 // kernel runs meanwhile).  The sums are the exact FP64 sums of the float
-// products (height * weight), as the global-atomic delivery's, so the ring
-// receives the same values bit for bit.
+// products, as the global-atomic delivery's, so the ring receives the same
+// values bit for bit.
 __global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) {
   extern __shared__ double acc[];
   const int tc = a.ts.tile_ch, ns = a.n_slot, nt = a.ts.n_tiles;
@@ -251,21 +286,19 @@ __global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) 
     const long long c0 = (long long)tile * tc;
     for (int k = threadIdx.x; k < ns * tc; k += blockDim.x) acc[k] = 0.0;
     __syncthreads();
+    const uint2* wk = a.ts.work + (size_t)tile * n_spk;
     for (int s = threadIdx.x; s < n_spk; s += blockDim.x) {
       const unsigned int raw = (unsigned int)srcs[s];
-      const int src = (int)(raw & ~kSpikeLagBit);
       const float hgt = hts[s];
       const int base = spike_base_slot(raw, a.ctx.t_idx, ns);
       const bool skip_d0 = a.skip_lagged_d0 && (raw >> 31);
-      const unsigned int* o = a.ts.off + (size_t)src * (nt + 1) + tile;
-      const unsigned int b = o[0], e = o[1];
-      for (unsigned int k = b; k < e; ++k) {
-        const int d = a.ts.syn_d[k];
-        if (skip_d0 && d == 0) continue;
-        int slot = base + d;
+      const uint2 r = wk[s];
+      for (unsigned int k = r.x; k < r.y; ++k) {
+        const TileRec rec = a.ts.rec[k];
+        if (skip_d0 && rec.d == 0) continue;
+        int slot = base + rec.d;
         if (slot >= ns) slot -= ns;
-        atomicAdd(&acc[slot * tc + (int)((long long)a.ts.syn_ch[k] - c0)],
-                  (double)(hgt * a.ts.syn_w[k]));
+        atomicAdd(&acc[slot * tc + rec.ch], (double)(hgt * rec.w));
         ++n_ev;
       }
     }
@@ -295,8 +328,10 @@ __global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) 
 
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
   const size_t lds = (size_t)a.n_slot * a.ts.tile_ch * sizeof(double);
-  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0)
+  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > 256)
     throw std::runtime_error("tiled delivery: tile does not fit its LDS budget");
+  const dim3 wgrid((unsigned)((a.ts.n_tiles + kWorkChunk - 1) / kWorkChunk), 64);
+  hipLaunchKernelGGL(tile_work_kernel, wgrid, dim3(256), 0, s, a);
   const int grid = a.ts.n_tiles < kDeliverBlocks ? a.ts.n_tiles : kDeliverBlocks;
   hipLaunchKernelGGL(deliver_tiled_kernel, dim3(grid), dim3(256), lds, s, a);
 }

@@ -699,20 +699,22 @@ def test_fold_in_rk5_matches_fold_launch(ngpu):
     assert s1["attempts"] == s0["attempts"] and s1["rejected"] == s0["rejected"]
 
 
-@pytest.mark.parametrize("cap", ["3", "24"])
-def test_fused_delivery_matches_delivery_launch(ngpu, cap):
-    # delivery of a step's spikes by the next RK5 launch's idle waves (the
-    # d_int = 0 synapses of lagged spikes at the emitting wave's exit) against
-    # a delivery launch after every step: with split steps, delays from 0.1 ms
-    # and three Simulate() calls (each call's last step delivers by launch,
-    # the next call's first launch delivers nothing), rasters, state,
-    # attempts and delivered event counts are equal
+@pytest.mark.parametrize("cap,d0", [("3", "0"), ("24", "0"), ("3", "1")])
+def test_fused_delivery_matches_delivery_launch(ngpu, cap, d0):
+    # delivery of a step's spikes by the next RK5 launch's idle waves against
+    # a delivery launch after every step, with the d_int = 0 synapses of
+    # lagged spikes by a launch after the update launch (d0 = 0, the
+    # default) or at the emitting wave's exit (1): with
+    # split steps, delays from 0.1 ms and three Simulate() calls (each call's
+    # last step delivers by launch, the next call's first launch delivers
+    # nothing), rasters, state, attempts and delivered event counts are equal
     net = netbuild.c1_network(seed=13)
     net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2], np.full_like(net["exc"][3], 0.1))
     res = []
     for fused in ("1", "0"):
         os.environ["NGPU_RK5_DEFER"] = cap
         os.environ["NGPU_FUSED_DELIVER"] = fused
+        os.environ["NGPU_LAG_D0"] = d0
         try:
             ngpu.ResetKernel()
             ngpu.SetVerbosityLevel(0)
@@ -736,6 +738,7 @@ def test_fused_delivery_matches_delivery_launch(ngpu, cap):
         finally:
             os.environ.pop("NGPU_RK5_DEFER", None)
             os.environ.pop("NGPU_FUSED_DELIVER", None)
+            os.environ.pop("NGPU_LAG_D0", None)
     (st1, v1, g11, s1), (st0, v0, g10, s0) = res
     n_spk = sum(len(x) for x in st1)
     print("cap", cap, "split steps", s1["split_steps"], "spikes", n_spk, "events", s1["syn_events"])
