@@ -340,8 +340,13 @@ void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
 // synapses of the list's lagged spikes (their slot, t_emit + 1, is this
 // step's, which the next launch folds as its lanes take their neurons); the
 // rest of each lagged spike, and every other spike, is delivered with the
-// list by the next launch.  One wave per list entry at a time.
+// list by the next launch.  A wave takes kD0Batch list entries: lanes
+// 0..kD0Batch-1 load them and the segment bounds of the lagged ones, then
+// every lane loads one record of each (the d_int = 0 group heads the segment
+// and has far fewer than 64 records) before any atomic is issued, so a wave
+// waits for memory three times in all.
 constexpr int kLagD0Blocks = 64;
+constexpr int kD0Batch = 8;
 
 __global__ void __launch_bounds__(256) deliver_lagged_d0_kernel(DeliverArgs a) {
   const int lst = a.ctx.lst;
@@ -352,13 +357,50 @@ __global__ void __launch_bounds__(256) deliver_lagged_d0_kernel(DeliverArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_waves = gridDim.x * (blockDim.x / kWave);
+  const SynTable& t = a.st;
   unsigned long long n_ev = 0;
-  for (int s = wave; s < n_spk; s += n_waves) {
-    const unsigned int raw = (unsigned int)srcs[s];
-    if (!(raw >> 31)) continue;
-    const int src = (int)(raw & ~kSpikeLagBit);
-    n_ev += deliver_segment(a.st, spike_base_slot(raw, a.ctx.t_idx, a.st.n_slot), hts[s],
-                            a.st.row_ptr[src], a.st.row_ptr[src + 1], lane, kWave, 1);
+  for (int e0 = wave * kD0Batch; e0 < n_spk; e0 += n_waves * kD0Batch) {
+    const int e = e0 + lane;
+    const bool mine = lane < kD0Batch && e < n_spk;
+    const unsigned int raw = mine ? (unsigned int)srcs[e] : 0u;
+    const float hgt = mine ? hts[e] : 0.0f;
+    const bool lg = mine && (raw >> 31);
+    long long sb = 0, se = 0;
+    if (lg) {
+      const int src = (int)(raw & ~kSpikeLagBit);
+      sb = t.row_ptr[src];
+      se = t.row_ptr[src + 1];
+    }
+    if (!__any(lg)) continue;
+    unsigned int ch[kD0Batch];
+    float w[kD0Batch];
+    int d[kD0Batch];
+    long long kk[kD0Batch];
+#pragma unroll
+    for (int q = 0; q < kD0Batch; ++q) {
+      const long long b = __shfl(sb, q, kWave), en = __shfl(se, q, kWave);
+      kk[q] = b + lane;
+      d[q] = -1;
+      if (kk[q] < en) {   // en = 0 for an unlagged entry
+        ch[q] = t.syn_ch[kk[q]];
+        w[q] = t.syn_w[kk[q]];
+        d[q] = t.syn_d[kk[q]];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kD0Batch; ++q) {
+      const int base = spike_base_slot(__shfl(raw, q, kWave), a.ctx.t_idx, t.n_slot);
+      const float h = __shfl(hgt, q, kWave);
+      if (d[q] == 0) {
+        unsafeAtomicAdd(&t.ring[(size_t)base * t.n_ch + ch[q]], (double)(h * w[q]));
+        ++n_ev;
+      }
+      // a d_int = 0 group longer than one record per lane: the rest of it by
+      // the whole wave
+      if (__shfl(d[q], kWave - 1, kWave) == 0)
+        n_ev += deliver_segment(t, base, h, __shfl(kk[q], kWave - 1, kWave) + 1,
+                                __shfl(se, q, kWave), lane, kWave, 1);
+    }
   }
   n_ev = wave_sum(n_ev);
   if (lane == 0 && a.ctr && n_ev) a.ctr[(size_t)wave * kCtrCols + 3] += n_ev;
