@@ -1609,7 +1609,7 @@ int Engine::Calibrate() {
     rk5_defer_env_ = true;
   }
   // per-wave counter rows: as many as the widest launch that adds to them
-  long long rows = 4LL * kDeliverBlocks;
+  long long rows = kCtrMinRows;
   for (auto* g : groups_)
     if (g->md->rk5)
       rows = std::max(rows, rk5_waves(g->n, rk5_persistent_ ? 1 : 0, rk5_max_blocks_));
@@ -2272,20 +2272,47 @@ double Engine::AutotuneBatch(int steps, int* n_candidates) {
   return best_ms / steps;
 }
 
-// Tiled copy of the synapse table for dense delivery (TiledSyn): tiles of
-// tile_ch channels, tile_ch the largest multiple of 32 (at most 256) whose
-// all-slot FP64 tile fits kTileLdsBytes.  Built when a step's events may
-// outnumber the ring's entries (at least 4 synapses per ring entry;
-// NGPU_TILED=1 builds it whenever it fits, 0 never), delay slots stay below
-// 256 and the (source, tile) table stays below 2^28 entries.  Within a
-// source the records keep their order inside a tile.
+// Tiled copy of the synapse table for dense delivery (TiledSyn).  Built when
+// a step's events may outnumber the ring's entries (at least 4 synapses per
+// ring entry; NGPU_TILED=1 builds it whenever it fits, 0 never), delay slots
+// stay below 256 and the (source, tile) table stays below 2^28 entries.
+// Within a source the records keep their order inside a tile.  tile_ch (a
+// multiple of 8, at most 256, the all-slot FP64 tile within kTileLdsBytes)
+// minimises rounds x (tile_ch + 32): a round is one tile block on every
+// block slot of the chip (LDS per CU / tile bytes, at most 4 blocks of 8
+// waves per CU), and 32 channels stand for a tile's fixed cost (reading and
+// scanning the step's (first, end) runs).  NGPU_TILE_CH=c sets it.
+int Engine::choose_tile_ch() const {
+  const int row = n_slot_ * (int)sizeof(double);
+  const int tc_max = std::min(256, kTileLdsBytes / row / 8 * 8);
+  if (const char* env = getenv("NGPU_TILE_CH")) {
+    const int c = atoi(env) / 8 * 8;
+    if (c >= 8 && c <= tc_max) return c;
+  }
+  int dev = 0, cus = 256, lds_cu = 160 * 1024;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  lds_cu = std::max(lds_cu, kTileLdsBytes);
+  int best = 0;
+  long long best_cost = 0;
+  for (int tc = 8; tc <= tc_max; tc += 8) {
+    const long long nt = (n_ch_ + tc - 1) / tc;
+    const long long per_cu = std::min(4, lds_cu / (tc * row));
+    const long long rounds = (nt + cus * per_cu - 1) / (cus * per_cu);
+    const long long cost = rounds * (tc + 32);
+    if (!best || cost < best_cost) { best = tc; best_cost = cost; }
+  }
+  return best;
+}
+
 void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd) {
   tiled_on_ = tiled_use_ = false;
-  d_toff_.free(); d_trec_.free(); d_twork_.free();
+  d_toff_.free(); d_trec_.free();
   const size_t nc = ch.size();
   if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
-  const int tc = std::min(256, (kTileLdsBytes / (n_slot_ * (int)sizeof(double))) / 32 * 32);
-  if (tc < 32) return;
+  if (n_slot_ * (int)sizeof(double) * 8 > kTileLdsBytes) return;
+  const int tc = choose_tile_ch();
   const long long nt = (n_ch_ + tc - 1) / tc;
   const long long entries = (long long)n_nodes_ * (nt + 1);
   if (entries >= (1ll << 28)) return;
@@ -2330,10 +2357,9 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
 // Delivery launch of this step's list: dense (tiled) or global atomics.
 void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagged_d0) {
   if (tiled_use_) {
-    if (!d_twork_.p) d_twork_.alloc((size_t)n_tiles_ * std::max(spk_cap_, 1));
     TiledDeliverArgs ta{};
     ta.ctx = sc; ta.sl = sl;
-    ta.ts = TiledSyn{d_toff_.p, d_trec_.p, d_twork_.p, n_tiles_, tile_ch_};
+    ta.ts = TiledSyn{d_toff_.p, d_trec_.p, n_tiles_, tile_ch_};
     ta.ring = d_ring_.p; ta.n_ch = n_ch_; ta.n_slot = n_slot_;
     ta.skip_lagged_d0 = skip_lagged_d0;
     ta.ctr = d_ctr_.p; ta.spikes = d_stats_.p;

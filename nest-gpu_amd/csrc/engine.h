@@ -342,9 +342,9 @@ class Engine {
   bool tiled_on_ = false;              // the tiled copy of the synapse table exists
   bool tiled_use_ = false;             // deliver with it (a batch autotuner knob)
   int tile_ch_ = 0, n_tiles_ = 0;
+  int choose_tile_ch() const;
   DBuf<unsigned int> d_toff_;
   DBuf<TileRec> d_trec_;
-  DBuf<uint2> d_twork_;               // per-step (first, end) runs [n_tiles][n_spk]
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   // batch kernel

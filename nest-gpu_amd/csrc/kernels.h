@@ -11,9 +11,12 @@ namespace ngpu {
 // (global wave index) with plain loads and stores, and the host sums the
 // rows when asked.  Columns: RK5 attempts, rejected attempts, accepted
 // substeps, delivered (spike, synapse) pairs, neurons put on a hot list,
-// neuron-steps split over two RK5 launches.
+// neuron-steps split over two RK5 launches.  The array holds at least
+// kCtrMinRows rows; a launch with more waves (the dense delivery's) adds
+// with atomics into row (wave mod kCtrMinRows).
 constexpr int kCtrCols = 6;
 constexpr int kDeliverBlocks = 1024;
+constexpr int kCtrMinRows = 4 * kDeliverBlocks;
 
 enum ModelId {
   MODEL_AEIF_COND_BETA_MS = 0,
@@ -215,19 +218,17 @@ struct TileRec {
 struct TiledSyn {
   const unsigned int* off;     // [n_nodes][n_tiles + 1]
   const TileRec* rec;          // [n_syn]
-  uint2* work;                 // [n_tiles][n_spk]: (first, end) record of each (tile, spike)
   int n_tiles, tile_ch;
 };
 
-// Dense delivery of one step's spike list, two launches: tile_work_kernel
-// turns the offset rows of the list's sources (source-major) into per-tile
-// runs of (first, end) pairs (tile-major, through an LDS transpose, so both
-// sides move whole lines), then deliver_tiled_kernel gives each block one
-// tile of channels: its threads take the spikes, add every record of
-// (spike, tile) into the LDS tile acc[slot][channel] (FP64), and the block
-// adds the tile into the ring once.  Replaces global atomics by LDS atomics
-// and one coalesced read-modify-write per ring entry that received input,
-// for networks where a step's events outnumber the ring's entries.
+// Dense delivery of one step's spike list (deliver_tiled_kernel): each block
+// owns one tile of channels; its waves take the list's spikes 64 at a time,
+// read their (first, end) record runs for the tile from the source-major
+// offset rows, walk the 64 runs as one flat range of records, add every
+// record into the LDS tile acc[slot][channel] (FP64), and the block adds the
+// tile into the ring once.  Replaces global atomics by LDS atomics and one
+// coalesced read-modify-write per ring entry that received input, for
+// networks where a step's events outnumber the ring's entries.
 struct TiledDeliverArgs {
   StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
@@ -240,8 +241,9 @@ struct TiledDeliverArgs {
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
-// LDS bytes of one tile block; tile_ch = this / (n_slot * 8), a multiple of
-// 32 and at most 256 (the record's channel byte); delay slots below 256
+// LDS bytes of one tile block at most (tile_ch * n_slot * 8); tile_ch at
+// most 256 (the record's channel byte), a multiple of 8; delay slots below
+// 256
 constexpr int kTileLdsBytes = 64 * 1024;
 
 struct DeliverArgs {

@@ -230,94 +230,126 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(deliver_kernel, dim3(kDeliverBlocks), dim3(256), 0, s, a);
 }
 
-// Dense delivery (TiledDeliverArgs), first launch: block (x, y) takes tiles
-// [64x, 64x + 64) and the list's spikes in chunks of 64 from chunk y on; it
-// reads the 65 offsets of each spike's source row for those tiles (one run of
-// a row each), transposes them in LDS, and writes for each tile the chunk's
-// (first, end) pairs contiguously at work[tile][spike].
-constexpr int kWorkChunk = 64;
-
-__global__ void __launch_bounds__(256) tile_work_kernel(TiledDeliverArgs a) {
-  __shared__ unsigned int sm[kWorkChunk][kWorkChunk + 2];
-  const int nt = a.ts.n_tiles;
-  const int lst = a.ctx.lst;
-  int n_spk = a.sl.count[lst];
-  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
-  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
-  const int t0 = blockIdx.x * kWorkChunk;
-  const int nt_here = nt - t0 < kWorkChunk ? nt - t0 : kWorkChunk;
-  for (int k0 = blockIdx.y * kWorkChunk; k0 < n_spk; k0 += gridDim.y * kWorkChunk) {
-    const int nk = n_spk - k0 < kWorkChunk ? n_spk - k0 : kWorkChunk;
-    for (int idx = threadIdx.x; idx < nk * (kWorkChunk + 1); idx += blockDim.x) {
-      const int k = idx / (kWorkChunk + 1), t = idx - k * (kWorkChunk + 1);
-      if (t <= nt_here) {
-        const int src = (int)((unsigned int)srcs[k0 + k] & ~kSpikeLagBit);
-        sm[k][t] = a.ts.off[(size_t)src * (nt + 1) + t0 + t];
-      }
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nt_here * kWorkChunk; idx += blockDim.x) {
-      const int t = idx / kWorkChunk, k = idx - t * kWorkChunk;
-      if (k < nk) a.ts.work[(size_t)(t0 + t) * n_spk + k0 + k] = make_uint2(sm[k][t], sm[k][t + 1]);
-    }
-    __syncthreads();
-  }
-}
-
-// Second launch: block b owns channel tile b (and b + gridDim.x, ...).  Its
-// threads take the list's spikes one each, read the (first, end) run of
-// (spike, tile) (consecutive threads, consecutive pairs), and add every
-// record's (double)(height * weight) into the LDS tile acc[slot][channel]
-// with FP64 LDS atomics; then the block adds the tile into the ring (plain
+// Dense delivery (TiledDeliverArgs): block b owns one channel tile.  Blocks
+// are handed out round-robin over the 8 XCDs, so block b takes tile
+// xcd_tile(b): XCD x gets a contiguous run of tiles, and what neighbouring
+// tiles read of a source (its offsets off[src][tile], and its record runs,
+// adjacent in memory) meets in the same L2.  Each wave takes the list's
+// spikes 64 at a time: lane l reads the (first, end) run of (spike l, tile)
+// from its source's offset row, a wave prefix sum turns the 64 runs into one
+// flat range of records, and the wave walks that range 64
+// records per pass, each lane finding its spike by a 6-step binary search
+// over the prefix sums (lane shuffles) -- every lane busy, and one pass reads
+// a few whole runs instead of one record of 64 different runs.  Every record
+// adds (double)(height * weight) into the LDS tile acc[slot][channel] with
+// FP64 LDS atomics; then the block adds the tile into the ring (plain
 // read-modify-write: no other block touches these ring entries, and no other
 // kernel runs meanwhile).  The sums are the exact FP64 sums of the float
 // products, as the global-atomic delivery's, so the ring receives the same
 // values bit for bit.
-__global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) {
+constexpr int kTiledThreads = 512;
+constexpr int kTiledUnroll = 4;
+
+__device__ inline int xcd_tile(int b, int n) {
+  const int xcd = b & 7, local = b >> 3, base = n >> 3, rem = n & 7;
+  return xcd < rem ? xcd * (base + 1) + local : rem * (base + 1) + (xcd - rem) * base + local;
+}
+
+__global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliverArgs a) {
   extern __shared__ double acc[];
-  const int tc = a.ts.tile_ch, ns = a.n_slot, nt = a.ts.n_tiles;
+  const int tc = a.ts.tile_ch, ns = a.n_slot;
+  const int n_acc = ns * tc;
+  const int tile = xcd_tile(blockIdx.x, a.ts.n_tiles);
   const int lst = a.ctx.lst;
   int n_spk = a.sl.count[lst];
   if (n_spk > a.sl.cap) n_spk = a.sl.cap;
   const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
   const float* hts = a.sl.height + (size_t)lst * a.sl.cap;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int n_wv = blockDim.x / kWave;
+  const long long c0 = (long long)tile * tc;
+  for (int k = threadIdx.x; k < n_acc; k += blockDim.x) acc[k] = 0.0;
+  __syncthreads();
+  const unsigned int* off = a.ts.off + tile;
+  const size_t row = (size_t)a.ts.n_tiles + 1;
   unsigned long long n_ev = 0;
-  for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const long long c0 = (long long)tile * tc;
-    for (int k = threadIdx.x; k < ns * tc; k += blockDim.x) acc[k] = 0.0;
-    __syncthreads();
-    const uint2* wk = a.ts.work + (size_t)tile * n_spk;
-    for (int s = threadIdx.x; s < n_spk; s += blockDim.x) {
+  for (int g0 = wv * kWave; g0 < n_spk; g0 += n_wv * kWave) {
+    // lane l: spike g0 + l (or an empty run past the list's end)
+    const int s = g0 + lane;
+    unsigned int beg = 0, cnt = 0, meta = 0;
+    float hgt = 0.f;
+    if (s < n_spk) {
       const unsigned int raw = (unsigned int)srcs[s];
-      const float hgt = hts[s];
-      const int base = spike_base_slot(raw, a.ctx.t_idx, ns);
-      const bool skip_d0 = a.skip_lagged_d0 && (raw >> 31);
-      const uint2 r = wk[s];
-      for (unsigned int k = r.x; k < r.y; ++k) {
-        const TileRec rec = a.ts.rec[k];
-        if (skip_d0 && rec.d == 0) continue;
-        int slot = base + rec.d;
+      const unsigned int* o = off + (size_t)(raw & ~kSpikeLagBit) * row;
+      beg = o[0];
+      cnt = o[1] - beg;
+      hgt = hts[s];
+      meta = (unsigned int)spike_base_slot(raw, a.ctx.t_idx, ns) |
+             ((a.skip_lagged_d0 && (raw >> 31)) ? 0x10000u : 0u);
+    }
+    unsigned int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const unsigned int t = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += t;
+    }
+    const unsigned int pre = incl - cnt;
+    const unsigned int total = __shfl(incl, kWave - 1, kWave);
+    for (unsigned int j0 = 0; j0 < total; j0 += kWave * kTiledUnroll) {
+      unsigned int jj[kTiledUnroll], lo[kTiledUnroll];
+#pragma unroll
+      for (int u = 0; u < kTiledUnroll; ++u) {
+        jj[u] = j0 + u * kWave + lane;
+        lo[u] = 0;
+      }
+      // spike of flat record j: the last lane whose run starts at or before
+      // j (a lane with an empty run shares its start with the next lane)
+#pragma unroll
+      for (int step = kWave / 2; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < kTiledUnroll; ++u) {
+          const unsigned int p = __shfl(pre, (int)lo[u] + step, kWave);
+          if (p <= jj[u]) lo[u] += step;
+        }
+      }
+      TileRec rec[kTiledUnroll];
+      unsigned int m[kTiledUnroll];
+      float h[kTiledUnroll];
+#pragma unroll
+      for (int u = 0; u < kTiledUnroll; ++u) {
+        const unsigned int p = __shfl(pre, (int)lo[u], kWave);
+        const unsigned int b0 = __shfl(beg, (int)lo[u], kWave);
+        m[u] = __shfl(meta, (int)lo[u], kWave);
+        h[u] = __shfl(hgt, (int)lo[u], kWave);
+        if (jj[u] < total) rec[u] = a.ts.rec[b0 + (jj[u] - p)];
+      }
+#pragma unroll
+      for (int u = 0; u < kTiledUnroll; ++u) {
+        if (jj[u] >= total) continue;
+        if ((m[u] >> 16) && rec[u].d == 0) continue;
+        int slot = (int)(m[u] & 0xffffu) + rec[u].d;
         if (slot >= ns) slot -= ns;
-        atomicAdd(&acc[slot * tc + rec.ch], (double)(hgt * rec.w));
+        atomicAdd(&acc[slot * tc + rec[u].ch], (double)(h[u] * rec[u].w));
         ++n_ev;
       }
     }
-    __syncthreads();
-    const long long left = a.n_ch - c0;
-    const int w = left < tc ? (int)left : tc;
-    for (int k = threadIdx.x; k < ns * tc; k += blockDim.x) {
-      const int slot = k / tc, c = k - slot * tc;
-      if (c < w) {
-        const double v = acc[k];
-        if (v != 0.0) a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + c)] += v;
-      }
+  }
+  __syncthreads();
+  const long long left = a.n_ch - c0;
+  const int w = left < tc ? (int)left : tc;
+  for (int k = threadIdx.x; k < n_acc; k += blockDim.x) {
+    const int slot = k / tc, c = k - slot * tc;
+    if (c < w) {
+      const double v = acc[k];
+      if (v != 0.0) a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + c)] += v;
     }
-    __syncthreads();
   }
   n_ev = wave_sum(n_ev);
-  const int lane = threadIdx.x & (kWave - 1);
+  // tiles can outnumber the counter rows: spread the waves over the first
+  // kCtrMinRows rows with atomics
   if (lane == 0 && a.ctr && n_ev)
-    a.ctr[((size_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kCtrCols + 3] += n_ev;
+    atomicAdd(&a.ctr[(size_t)(((long long)blockIdx.x * n_wv + wv) % kCtrMinRows) * kCtrCols + 3], n_ev);
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
     // the next step writes list (lst + 1) mod 3: clear its count now (it held
@@ -328,12 +360,9 @@ __global__ void __launch_bounds__(256) deliver_tiled_kernel(TiledDeliverArgs a) 
 
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
   const size_t lds = (size_t)a.n_slot * a.ts.tile_ch * sizeof(double);
-  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > 256)
+  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > 256 || a.n_slot > 256)
     throw std::runtime_error("tiled delivery: tile does not fit its LDS budget");
-  const dim3 wgrid((unsigned)((a.ts.n_tiles + kWorkChunk - 1) / kWorkChunk), 64);
-  hipLaunchKernelGGL(tile_work_kernel, wgrid, dim3(256), 0, s, a);
-  const int grid = a.ts.n_tiles < kDeliverBlocks ? a.ts.n_tiles : kDeliverBlocks;
-  hipLaunchKernelGGL(deliver_tiled_kernel, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(deliver_tiled_kernel, dim3((unsigned)a.ts.n_tiles), dim3(kTiledThreads), lds, s, a);
 }
 
 // Fused delivery, right after the step's update launch: the d_int = 0
