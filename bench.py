@@ -326,6 +326,9 @@ def main():
                 % (ncand, args.tune_steps, best_ms))
         batch["config"] = ngpu.GetBatchConfig()
         log("batch configuration: %s" % batch["config"])
+    else:
+        batch = {"config": ngpu.GetBatchConfig()}
+        log("batch configuration: %s" % batch["config"])
     wp = timed_window(ngpu, args.steps, timing=win_timing)
     sp = window_summary(wp, args.steps, n)
     log("plain: %d steps, wall %.3f s, RTF %.4f, events %d, kernel ms %s"
@@ -470,8 +473,8 @@ def main():
         "rooflines": roof_all,
         "cpu_baseline": cpu,
     }
+    out["batch"] = batch
     if bt:
-        out["batch"] = batch
         out["roofline"]["note"] = ("flops counted at aeif_cond_beta_multisynapse (2 ports) per RK5 "
                                    "attempt for every model of the batch; iaf_psc_exp updates not "
                                    "counted")

@@ -2207,6 +2207,8 @@ void Engine::GetBatchConfig(int* cfg) {
   cfg[6] = (int)batch_groups_.size();
   cfg[7] = tiled_on_ ? 1 : 0;
   cfg[8] = tiled_use_ ? 1 : 0;
+  cfg[9] = tiled_on_ ? tile_ch_ : 0;
+  cfg[10] = tiled_on_ ? n_tiles_ : 0;
 }
 
 void Engine::SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled) {
@@ -2277,28 +2279,32 @@ double Engine::AutotuneBatch(int steps, int* n_candidates) {
 // ring entry; NGPU_TILED=1 builds it whenever it fits, 0 never), delay slots
 // stay below 256 and the (source, tile) table stays below 2^28 entries.
 // Within a source the records keep their order inside a tile.  tile_ch (a
-// multiple of 8, at most 256, the all-slot FP64 tile within kTileLdsBytes)
-// minimises rounds x (tile_ch + 32): a round is one tile block on every
-// block slot of the chip (LDS per CU / tile bytes, at most 4 blocks of 8
-// waves per CU), and 32 channels stand for a tile's fixed cost (reading and
-// scanning the step's (first, end) runs).  NGPU_TILE_CH=c sets it.
+// multiple of 8, at most 256, the all-slot FP64 tile within kTileLdsBytes
+// and the device's LDS per block) minimises rounds x (tile_ch + 32): a round
+// is one tile block on every block slot of the chip (LDS per CU / tile
+// bytes, at most 2 blocks of 16 waves per CU), and 32 channels stand for a
+// tile's fixed cost (reading and scanning the step's (first, end) runs).
+// NGPU_TILE_CH=c sets it.
 int Engine::choose_tile_ch() const {
   const int row = n_slot_ * (int)sizeof(double);
-  const int tc_max = std::min(256, kTileLdsBytes / row / 8 * 8);
+  int dev = 0, cus = 256, lds_cu = 160 * 1024, lds_block = kTileLdsBytes;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  HIPCHK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const int budget = std::min(kTileLdsBytes, lds_block);
+  const int tc_max = std::min(256, budget / row / 8 * 8);
+  if (tc_max < 8) return 0;
   if (const char* env = getenv("NGPU_TILE_CH")) {
     const int c = atoi(env) / 8 * 8;
     if (c >= 8 && c <= tc_max) return c;
   }
-  int dev = 0, cus = 256, lds_cu = 160 * 1024;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-  lds_cu = std::max(lds_cu, kTileLdsBytes);
+  lds_cu = std::max(lds_cu, budget);
   int best = 0;
   long long best_cost = 0;
   for (int tc = 8; tc <= tc_max; tc += 8) {
     const long long nt = (n_ch_ + tc - 1) / tc;
-    const long long per_cu = std::min(4, lds_cu / (tc * row));
+    const long long per_cu = std::min(2, lds_cu / (tc * row));
     const long long rounds = (nt + cus * per_cu - 1) / (cus * per_cu);
     const long long cost = rounds * (tc + 32);
     if (!best || cost < best_cost) { best = tc; best_cost = cost; }
@@ -2311,8 +2317,8 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   d_toff_.free(); d_trec_.free();
   const size_t nc = ch.size();
   if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
-  if (n_slot_ * (int)sizeof(double) * 8 > kTileLdsBytes) return;
   const int tc = choose_tile_ch();
+  if (tc == 0) return;
   const long long nt = (n_ch_ + tc - 1) / tc;
   const long long entries = (long long)n_nodes_ * (nt + 1);
   if (entries >= (1ll << 28)) return;

@@ -245,8 +245,8 @@ class Engine {
   // with the next step; results do not depend on it), and the autotuner,
   // which times every candidate over `steps` simulated steps each (so the
   // simulation advances by candidates x steps) and keeps the fastest.
-  // cfg[9] = {on, variant, iaf_unroll, rk5_weight_x100, graph, grid, populations,
-  //          tiled delivery available, tiled delivery used}
+  // cfg[11] = {on, variant, iaf_unroll, rk5_weight_x100, graph, grid, populations,
+  //           tiled delivery available, tiled delivery used, channels per tile, tiles}
   void GetBatchConfig(int* cfg);
   void SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled = -1);
   double AutotuneBatch(int steps, int* n_candidates);

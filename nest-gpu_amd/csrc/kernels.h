@@ -241,10 +241,10 @@ struct TiledDeliverArgs {
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
-// LDS bytes of one tile block at most (tile_ch * n_slot * 8); tile_ch at
-// most 256 (the record's channel byte), a multiple of 8; delay slots below
-// 256
-constexpr int kTileLdsBytes = 64 * 1024;
+// LDS bytes of one tile block at most (tile_ch * n_slot * 8): two blocks of
+// 16 waves fill a CU's 160 KB; tile_ch at most 256 (the record's channel
+// byte), a multiple of 8; delay slots below 256
+constexpr int kTileLdsBytes = 80 * 1024;
 
 struct DeliverArgs {
   StepCtx ctx;           // this step's time (by value)

@@ -247,8 +247,9 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 // kernel runs meanwhile).  The sums are the exact FP64 sums of the float
 // products, as the global-atomic delivery's, so the ring receives the same
 // values bit for bit.
-constexpr int kTiledThreads = 512;
+constexpr int kTiledThreads = 1024;
 constexpr int kTiledUnroll = 4;
+constexpr int kRingBatch = 4;
 
 __device__ inline int xcd_tile(int b, int n) {
   const int xcd = b & 7, local = b >> 3, base = n >> 3, rem = n & 7;
@@ -274,25 +275,40 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
   const unsigned int* off = a.ts.off + tile;
   const size_t row = (size_t)a.ts.n_tiles + 1;
   unsigned long long n_ev = 0;
-  for (int g0 = wv * kWave; g0 < n_spk; g0 += n_wv * kWave) {
+  // Software pipeline over the wave's groups: while group k is walked, the
+  // offsets of group k + 1 and the sources and heights of group k + 2 are
+  // in flight.
+  const int gs = n_wv * kWave;
+  int s = wv * kWave + lane;
+  unsigned int r0 = 0, b0 = 0, e0 = 0, r1 = 0;
+  float h0 = 0.f, h1 = 0.f;
+  if (s < n_spk) { r0 = (unsigned int)srcs[s]; h0 = hts[s]; }
+  if (s + gs < n_spk) { r1 = (unsigned int)srcs[s + gs]; h1 = hts[s + gs]; }
+  if (s < n_spk) {
+    const unsigned int* o = off + (size_t)(r0 & ~kSpikeLagBit) * row;
+    b0 = o[0];
+    e0 = o[1];
+  }
+  for (int g0 = wv * kWave; g0 < n_spk; g0 += gs, s += gs) {
     // lane l: spike g0 + l (or an empty run past the list's end)
-    const int s = g0 + lane;
-    unsigned int beg = 0, cnt = 0, meta = 0;
-    float hgt = 0.f;
-    if (s < n_spk) {
-      const unsigned int raw = (unsigned int)srcs[s];
-      const unsigned int* o = off + (size_t)(raw & ~kSpikeLagBit) * row;
-      beg = o[0];
-      cnt = o[1] - beg;
-      hgt = hts[s];
-      meta = (unsigned int)spike_base_slot(raw, a.ctx.t_idx, ns) |
-             ((a.skip_lagged_d0 && (raw >> 31)) ? 0x10000u : 0u);
+    const bool valid = s < n_spk;
+    const unsigned int beg = b0, cnt = valid ? e0 - b0 : 0u;
+    const unsigned int meta = (unsigned int)spike_base_slot(r0, a.ctx.t_idx, ns) |
+                              ((a.skip_lagged_d0 && (r0 >> 31)) ? 0x10000u : 0u);
+    const float hgt = h0;
+    unsigned int nb = 0, ne = 0, r2 = 0;
+    float h2 = 0.f;
+    if (s + gs < n_spk) {
+      const unsigned int* o = off + (size_t)(r1 & ~kSpikeLagBit) * row;
+      nb = o[0];
+      ne = o[1];
     }
+    if (s + 2 * gs < n_spk) { r2 = (unsigned int)srcs[s + 2 * gs]; h2 = hts[s + 2 * gs]; }
     unsigned int incl = cnt;
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const unsigned int t = __shfl_up(incl, off, kWave);
-      if (lane >= off) incl += t;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const unsigned int t = __shfl_up(incl, d, kWave);
+      if (lane >= d) incl += t;
     }
     const unsigned int pre = incl - cnt;
     const unsigned int total = __shfl(incl, kWave - 1, kWave);
@@ -318,11 +334,11 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
       float h[kTiledUnroll];
 #pragma unroll
       for (int u = 0; u < kTiledUnroll; ++u) {
-        const unsigned int p = __shfl(pre, (int)lo[u], kWave);
-        const unsigned int b0 = __shfl(beg, (int)lo[u], kWave);
+        // record index beg + (j - pre), as one shuffled value (mod 2^32)
+        const unsigned int bp = __shfl(beg - pre, (int)lo[u], kWave);
         m[u] = __shfl(meta, (int)lo[u], kWave);
         h[u] = __shfl(hgt, (int)lo[u], kWave);
-        if (jj[u] < total) rec[u] = a.ts.rec[b0 + (jj[u] - p)];
+        if (jj[u] < total) rec[u] = a.ts.rec[bp + jj[u]];
       }
 #pragma unroll
       for (int u = 0; u < kTiledUnroll; ++u) {
@@ -334,16 +350,30 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
         ++n_ev;
       }
     }
+    r0 = r1; h0 = h1; b0 = nb; e0 = ne;
+    r1 = r2; h1 = h2;
   }
   __syncthreads();
+  // tile -> ring, kRingBatch entries per thread at a time: every ring load
+  // of the batch is issued before its stores
   const long long left = a.n_ch - c0;
   const int w = left < tc ? (int)left : tc;
-  for (int k = threadIdx.x; k < n_acc; k += blockDim.x) {
-    const int slot = k / tc, c = k - slot * tc;
-    if (c < w) {
-      const double v = acc[k];
-      if (v != 0.0) a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + c)] += v;
+  for (int k0 = threadIdx.x; k0 < n_acc; k0 += kRingBatch * blockDim.x) {
+    double v[kRingBatch], r[kRingBatch];
+    size_t gi[kRingBatch];
+#pragma unroll
+    for (int u = 0; u < kRingBatch; ++u) {
+      const int k = k0 + u * (int)blockDim.x;
+      const int slot = k / tc, c = k - slot * tc;
+      v[u] = (k < n_acc && c < w) ? acc[k] : 0.0;
+      gi[u] = (size_t)slot * a.n_ch + (size_t)(c0 + c);
     }
+#pragma unroll
+    for (int u = 0; u < kRingBatch; ++u)
+      if (v[u] != 0.0) r[u] = a.ring[gi[u]];
+#pragma unroll
+    for (int u = 0; u < kRingBatch; ++u)
+      if (v[u] != 0.0) a.ring[gi[u]] = r[u] + v[u];
   }
   n_ev = wave_sum(n_ev);
   // tiles can outnumber the counter rows: spread the waves over the first
