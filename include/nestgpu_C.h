@@ -232,10 +232,10 @@ int NESTGPU_AMD_Synchronize();
  * aeif_cond_alpha_multisynapse (4 ports), aeif_psc_exp,
  * izhikevich_cond_beta (2 ports) with uniform parameters, or iaf_psc_exp
  * (2 to 8 populations), each step is one launch, replayed as a one-node HIP
- * graph.  cfg[11] = {on, variant (0: 3 waves per SIMD, 1: 4), iaf_psc_exp
+ * graph.  cfg[12] = {on, variant (0: 3 waves per SIMD, 1: 4), iaf_psc_exp
  * neurons per lane per pass (1, 2, 4), RK5 block weight x 100, graph,
  * grid blocks, populations, dense (tiled) delivery available, used,
- * channels per tile, tiles}.
+ * channels per tile, tiles, delay rows per tile}.
  * tiled: 1 deliver each step's list by the dense tiled launch, 0 inside the
  * next step's launch, -1 unchanged.  A configuration changes scheduling
  * only, never results.  AutotuneBatch times every candidate configuration

@@ -2209,6 +2209,7 @@ void Engine::GetBatchConfig(int* cfg) {
   cfg[8] = tiled_use_ ? 1 : 0;
   cfg[9] = tiled_on_ ? tile_ch_ : 0;
   cfg[10] = tiled_on_ ? n_tiles_ : 0;
+  cfg[11] = tiled_on_ ? tile_rows_ : 0;
 }
 
 void Engine::SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled) {
@@ -2278,22 +2279,25 @@ double Engine::AutotuneBatch(int steps, int* n_candidates) {
 // a step's events may outnumber the ring's entries (at least 4 synapses per
 // ring entry; NGPU_TILED=1 builds it whenever it fits, 0 never), delay slots
 // stay below 256 and the (source, tile) table stays below 2^28 entries.
-// Within a source the records keep their order inside a tile.  tile_ch (a
-// multiple of 8, at most 256, the all-slot FP64 tile within kTileLdsBytes
-// and the device's LDS per block) minimises rounds x (tile_ch + 32): a round
-// is one tile block on every block slot of the chip (LDS per CU / tile
-// bytes, at most 2 blocks of 16 waves per CU), and 32 channels stand for a
-// tile's fixed cost (reading and scanning the step's (first, end) runs).
-// NGPU_TILE_CH=c sets it.
-int Engine::choose_tile_ch() const {
-  const int row = n_slot_ * (int)sizeof(double);
+// Within a source the records keep their order inside a tile.  The LDS tile
+// keeps the relative slots r = d_int + 1 - lag below rows, rows the smallest
+// count that holds the delays of all but 1% of the synapses (records beyond
+// go to the ring by global atomics).  tile_ch (a multiple of 8, at most
+// kTileChMax, the rows x tile_ch FP64 tile within kTileLdsBytes and the
+// device's LDS per block) minimises rounds x (tile_ch + 32): a round is one
+// tile block on every block slot of the chip (LDS per CU / tile bytes, at
+// most 2 blocks of 16 waves per CU), and 32 channels stand for a tile's
+// fixed cost (reading and scanning the step's (first, end) runs).
+// NGPU_TILE_CH=c sets tile_ch, NGPU_TILE_ROWS=r the rows.
+int Engine::choose_tile_ch(int rows) const {
+  const int row = rows * (int)sizeof(double);
   int dev = 0, cus = 256, lds_cu = 160 * 1024, lds_block = kTileLdsBytes;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   HIPCHK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const int budget = std::min(kTileLdsBytes, lds_block);
-  const int tc_max = std::min(256, budget / row / 8 * 8);
+  const int tc_max = std::min(kTileChMax, budget / row / 8 * 8);
   if (tc_max < 8) return 0;
   if (const char* env = getenv("NGPU_TILE_CH")) {
     const int c = atoi(env) / 8 * 8;
@@ -2317,7 +2321,16 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   d_toff_.free(); d_trec_.free();
   const size_t nc = ch.size();
   if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
-  const int tc = choose_tile_ch();
+  std::vector<long long> hist(max_d_int_ + 1, 0);
+  for (size_t k = 0; k < nc; ++k) hist[dd[k]]++;
+  int rows = n_slot_;            // r = d + 1 <= rows - 1 for every synapse
+  long long beyond = 0;          // synapses whose r = d + 1 is rows or more
+  while (rows > 2 && (beyond + hist[rows - 2]) * 100 <= (long long)nc) beyond += hist[--rows - 1];
+  if (const char* env = getenv("NGPU_TILE_ROWS")) {
+    const int r = atoi(env);
+    if (r >= 1 && r <= n_slot_) rows = r;
+  }
+  const int tc = choose_tile_ch(rows);
   if (tc == 0) return;
   const long long nt = (n_ch_ + tc - 1) / tc;
   const long long entries = (long long)n_nodes_ * (nt + 1);
@@ -2344,7 +2357,7 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
       for (long long k = b; k < e; ++k) {
         const unsigned int t = ch[k] / tc;
         TileRec& r = rec[pos[t]++];
-        r.ch = (unsigned char)(ch[k] - t * (unsigned int)tc);
+        r.ch = (unsigned short)(ch[k] - t * (unsigned int)tc);
         r.d = (unsigned char)dd[k];
         r.pad = 0;
         r.w = c_w_[k];
@@ -2355,6 +2368,7 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   upload(d_trec_, rec, stream_);
   sync_streams();
   tile_ch_ = tc;
+  tile_rows_ = rows;
   n_tiles_ = (int)nt;
   tiled_on_ = true;
   tiled_use_ = true;
@@ -2365,7 +2379,7 @@ void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagge
   if (tiled_use_) {
     TiledDeliverArgs ta{};
     ta.ctx = sc; ta.sl = sl;
-    ta.ts = TiledSyn{d_toff_.p, d_trec_.p, n_tiles_, tile_ch_};
+    ta.ts = TiledSyn{d_toff_.p, d_trec_.p, n_tiles_, tile_ch_, tile_rows_};
     ta.ring = d_ring_.p; ta.n_ch = n_ch_; ta.n_slot = n_slot_;
     ta.skip_lagged_d0 = skip_lagged_d0;
     ta.ctr = d_ctr_.p; ta.spikes = d_stats_.p;

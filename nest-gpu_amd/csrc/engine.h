@@ -245,8 +245,9 @@ class Engine {
   // with the next step; results do not depend on it), and the autotuner,
   // which times every candidate over `steps` simulated steps each (so the
   // simulation advances by candidates x steps) and keeps the fastest.
-  // cfg[11] = {on, variant, iaf_unroll, rk5_weight_x100, graph, grid, populations,
-  //           tiled delivery available, tiled delivery used, channels per tile, tiles}
+  // cfg[12] = {on, variant, iaf_unroll, rk5_weight_x100, graph, grid, populations,
+  //           tiled delivery available, tiled delivery used, channels per tile, tiles,
+  //           delay rows per tile}
   void GetBatchConfig(int* cfg);
   void SetBatchConfig(int variant, int iaf_unroll, float rk5_weight, int graph, int tiled = -1);
   double AutotuneBatch(int steps, int* n_candidates);
@@ -341,8 +342,8 @@ class Engine {
   int tiled_env_ = -1;                 // NGPU_TILED: -1 when dense, 0 never, 1 whenever the table fits
   bool tiled_on_ = false;              // the tiled copy of the synapse table exists
   bool tiled_use_ = false;             // deliver with it (a batch autotuner knob)
-  int tile_ch_ = 0, n_tiles_ = 0;
-  int choose_tile_ch() const;
+  int tile_ch_ = 0, n_tiles_ = 0, tile_rows_ = 0;
+  int choose_tile_ch(int rows) const;
   DBuf<unsigned int> d_toff_;
   DBuf<TileRec> d_trec_;
   bool defer_ok_ = false;              // inside Simulate() and not its last step

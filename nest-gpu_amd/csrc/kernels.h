@@ -208,17 +208,21 @@ struct ParrotArgs {
 // channels are cut into tiles of tile_ch consecutive channels; within each
 // source's segment the synapses are grouped by tile, each as one 8-byte
 // record (channel within the tile, delay slot, weight), and off holds the
-// first record of every (source, tile) pair.
+// first record of every (source, tile) pair.  A spike of step t reaches
+// ring slot t + r, r = d_int + 1 - lag (lag 1 for a spike emitted by a split
+// step one step late); the LDS tile holds the rows r < rows (the delays of
+// nearly all synapses), the rest go to the ring with FP64 global atomics.
 struct TileRec {
-  unsigned char ch;      // channel - tile * tile_ch
+  unsigned short ch;     // channel - tile * tile_ch
   unsigned char d;       // delay slot d_int
-  unsigned short pad;
+  unsigned char pad;
   float w;
 };
 struct TiledSyn {
   const unsigned int* off;     // [n_nodes][n_tiles + 1]
   const TileRec* rec;          // [n_syn]
   int n_tiles, tile_ch;
+  int rows;                    // relative slots r kept in LDS: 0 .. rows - 1
 };
 
 // Dense delivery of one step's spike list (deliver_tiled_kernel): each block
@@ -241,10 +245,11 @@ struct TiledDeliverArgs {
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
 };
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
-// LDS bytes of one tile block at most (tile_ch * n_slot * 8): two blocks of
-// 16 waves fill a CU's 160 KB; tile_ch at most 256 (the record's channel
-// byte), a multiple of 8; delay slots below 256
+// LDS bytes of one tile block at most (tile_ch * rows * 8): two blocks of
+// 16 waves fill a CU's 160 KB; tile_ch a multiple of 8, at most
+// kTileChMax; delay slots below 256 (the record's delay byte)
 constexpr int kTileLdsBytes = 80 * 1024;
+constexpr int kTileChMax = 2048;
 
 struct DeliverArgs {
   StepCtx ctx;           // this step's time (by value)

@@ -241,12 +241,14 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 // records per pass, each lane finding its spike by a 6-step binary search
 // over the prefix sums (lane shuffles) -- every lane busy, and one pass reads
 // a few whole runs instead of one record of 64 different runs.  Every record
-// adds (double)(height * weight) into the LDS tile acc[slot][channel] with
-// FP64 LDS atomics; then the block adds the tile into the ring (plain
-// read-modify-write: no other block touches these ring entries, and no other
-// kernel runs meanwhile).  The sums are the exact FP64 sums of the float
-// products, as the global-atomic delivery's, so the ring receives the same
-// values bit for bit.
+// adds (double)(height * weight) for ring slot t + r (r = d_int + 1 - lag)
+// into the LDS tile acc[r][channel] with FP64 LDS atomics when r < rows,
+// else straight into the ring with an FP64 global atomic; then the block
+// adds the tile into the ring (plain read-modify-write of rows r < rows: no
+// other block touches these ring entries, no atomic of this block targets
+// them, and no other kernel runs meanwhile).  The sums are the exact FP64
+// sums of the float products, as the global-atomic delivery's, so the ring
+// receives the same values bit for bit.
 constexpr int kTiledThreads = 1024;
 constexpr int kTiledUnroll = 4;
 constexpr int kRingBatch = 4;
@@ -258,8 +260,9 @@ __device__ inline int xcd_tile(int b, int n) {
 
 __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliverArgs a) {
   extern __shared__ double acc[];
-  const int tc = a.ts.tile_ch, ns = a.n_slot;
-  const int n_acc = ns * tc;
+  const int tc = a.ts.tile_ch, ns = a.n_slot, rows = a.ts.rows;
+  const int n_acc = rows * tc;
+  const int ts0 = (int)(a.ctx.t_idx % ns);   // ring slot of r = 0
   const int tile = xcd_tile(blockIdx.x, a.ts.n_tiles);
   const int lst = a.ctx.lst;
   int n_spk = a.sl.count[lst];
@@ -293,8 +296,9 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
     // lane l: spike g0 + l (or an empty run past the list's end)
     const bool valid = s < n_spk;
     const unsigned int beg = b0, cnt = valid ? e0 - b0 : 0u;
-    const unsigned int meta = (unsigned int)spike_base_slot(r0, a.ctx.t_idx, ns) |
-                              ((a.skip_lagged_d0 && (r0 >> 31)) ? 0x10000u : 0u);
+    // bit 0: lagged spike (r = d_int), bit 1: its d_int = 0 synapses are
+    // delivered already
+    const unsigned int meta = (r0 >> 31) | ((a.skip_lagged_d0 && (r0 >> 31)) ? 2u : 0u);
     const float hgt = h0;
     unsigned int nb = 0, ne = 0, r2 = 0;
     float h2 = 0.f;
@@ -343,10 +347,16 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
 #pragma unroll
       for (int u = 0; u < kTiledUnroll; ++u) {
         if (jj[u] >= total) continue;
-        if ((m[u] >> 16) && rec[u].d == 0) continue;
-        int slot = (int)(m[u] & 0xffffu) + rec[u].d;
-        if (slot >= ns) slot -= ns;
-        atomicAdd(&acc[slot * tc + rec[u].ch], (double)(h[u] * rec[u].w));
+        if ((m[u] & 2u) && rec[u].d == 0) continue;
+        const int r = (int)rec[u].d + 1 - (int)(m[u] & 1u);
+        const double v = (double)(h[u] * rec[u].w);
+        if (r < rows) {
+          atomicAdd(&acc[r * tc + rec[u].ch], v);
+        } else {
+          int slot = ts0 + r;
+          if (slot >= ns) slot -= ns;
+          atomicAdd(&a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + rec[u].ch)], v);
+        }
         ++n_ev;
       }
     }
@@ -364,8 +374,9 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
 #pragma unroll
     for (int u = 0; u < kRingBatch; ++u) {
       const int k = k0 + u * (int)blockDim.x;
-      const int slot = k / tc, c = k - slot * tc;
+      const int r = k / tc, c = k - r * tc;
       v[u] = (k < n_acc && c < w) ? acc[k] : 0.0;
+      const int slot = ts0 + r >= ns ? ts0 + r - ns : ts0 + r;
       gi[u] = (size_t)slot * a.n_ch + (size_t)(c0 + c);
     }
 #pragma unroll
@@ -389,8 +400,9 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
 }
 
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
-  const size_t lds = (size_t)a.n_slot * a.ts.tile_ch * sizeof(double);
-  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > 256 || a.n_slot > 256)
+  const size_t lds = (size_t)a.ts.rows * a.ts.tile_ch * sizeof(double);
+  if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > kTileChMax ||
+      a.ts.rows < 1 || a.ts.rows > a.n_slot || a.n_slot > 257)
     throw std::runtime_error("tiled delivery: tile does not fit its LDS budget");
   hipLaunchKernelGGL(deliver_tiled_kernel, dim3((unsigned)a.ts.n_tiles), dim3(kTiledThreads), lds, s, a);
 }

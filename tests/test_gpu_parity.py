@@ -274,10 +274,15 @@ def test_iaf_psc_exp_network_bitexact(ngpu, runs, monkeypatch):
     np.testing.assert_array_equal(ng, no)
 
 
-def test_iaf_psc_exp_network_tiled_bitexact(ngpu, monkeypatch):
+@pytest.mark.parametrize("rows", [None, "2", "5"])
+def test_iaf_psc_exp_network_tiled_bitexact(ngpu, monkeypatch, rows):
     # the same network with the dense tiled delivery (LDS tile sums added to
-    # the ring once per step) instead of global atomics: identical raster
+    # the ring once per step) instead of global atomics: identical raster;
+    # rows = the delay rows kept in LDS (default: all but 1% of the synapses;
+    # 2 and 5 send most records to the ring by global atomics)
     monkeypatch.setenv("NGPU_TILED", "1")
+    if rows:
+        monkeypatch.setenv("NGPU_TILE_ROWS", rows)
     net = netbuild.c1_network(seed=3, n=600, k=60, t_rate=8000.0)
     net["exc"] = (net["exc"][0], net["exc"][1], net["exc"][2] * 60.0, net["exc"][3])
     net["inh"] = (net["inh"][0], net["inh"][1], net["inh"][2] * -60.0, net["inh"][3])
@@ -286,21 +291,27 @@ def test_iaf_psc_exp_network_tiled_bitexact(ngpu, monkeypatch):
     to, no = _oracle_network(net, "iaf_psc_exp", 300.0, poiss_w=30.0)
     print("iaf tiled spikes gpu", len(tg), "oracle", len(to), cfg)
     assert cfg["tiled"] == 1
+    if rows:
+        assert cfg["tile_rows"] == int(rows)
     assert len(to) > 500
     np.testing.assert_array_equal(tg, to)
     np.testing.assert_array_equal(ng, no)
 
 
-def test_tiled_delivery_rk5_split_steps(ngpu):
+@pytest.mark.parametrize("rows", [None, "2"])
+def test_tiled_delivery_rk5_split_steps(ngpu, rows):
     # one RK5 population with split steps (lagged spikes, their d_int = 0
     # synapses one slot earlier): the tiled delivery launch after every step
     # against fused delivery inside the next launch: identical rasters, state
-    # and attempt counts, over two Simulate calls
+    # and attempt counts, over two Simulate calls (rows 2: only the slots
+    # t and t + 1 in LDS, the rest by global atomics)
     net = netbuild.c1_network(seed=7)
     res = []
     for tiled in ("1", "0"):
         os.environ["NGPU_TILED"] = tiled
         os.environ["NGPU_RK5_DEFER"] = "3"
+        if rows:
+            os.environ["NGPU_TILE_ROWS"] = rows
         try:
             ngpu.ResetKernel()
             ngpu.SetVerbosityLevel(0)
@@ -315,6 +326,7 @@ def test_tiled_delivery_rk5_split_steps(ngpu):
         finally:
             os.environ.pop("NGPU_TILED", None)
             os.environ.pop("NGPU_RK5_DEFER", None)
+            os.environ.pop("NGPU_TILE_ROWS", None)
     (t1, n1, v1, st1, s1, c1), (t0, n0, v0, st0, s0, c0) = res
     print("tiled", c1, "split steps", s1["split_steps"], "spikes", len(t1))
     assert c1["tiled"] == 1 and c0["tiled"] == 0
