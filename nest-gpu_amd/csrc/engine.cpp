@@ -2296,7 +2296,7 @@ int Engine::choose_tile_ch(int rows) const {
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
   HIPCHK(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  const int budget = std::min(kTileLdsBytes, lds_block);
+  const int budget = std::min(kTileLdsBytes, lds_block - kTileStaticLds);
   const int tc_max = std::min(kTileChMax, budget / row / 8 * 8);
   if (tc_max < 8) return 0;
   if (const char* env = getenv("NGPU_TILE_CH")) {
@@ -2308,7 +2308,7 @@ int Engine::choose_tile_ch(int rows) const {
   long long best_cost = 0;
   for (int tc = 8; tc <= tc_max; tc += 8) {
     const long long nt = (n_ch_ + tc - 1) / tc;
-    const long long per_cu = std::min(2, lds_cu / (tc * row));
+    const long long per_cu = std::min(2, lds_cu / (tc * row + kTileStaticLds));
     const long long rounds = (nt + cus * per_cu - 1) / (cus * per_cu);
     const long long cost = rounds * (tc + 32);
     if (!best || cost < best_cost) { best = tc; best_cost = cost; }
@@ -2318,7 +2318,7 @@ int Engine::choose_tile_ch(int rows) const {
 
 void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd) {
   tiled_on_ = tiled_use_ = false;
-  d_toff_.free(); d_trec_.free();
+  d_toff_.free(); d_trec_.free(); d_tev_.free(); d_tplan_.free(); d_tsched_.free();
   const size_t nc = ch.size();
   if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
   std::vector<long long> hist(max_d_int_ + 1, 0);
@@ -2326,6 +2326,7 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   int rows = n_slot_;            // r = d + 1 <= rows - 1 for every synapse
   long long beyond = 0;          // synapses whose r = d + 1 is rows or more
   while (rows > 2 && (beyond + hist[rows - 2]) * 100 <= (long long)nc) beyond += hist[--rows - 1];
+  if (const char* env = getenv("NGPU_TILE_SPLIT_MIN")) tile_split_min_ = std::max(1, atoi(env));
   if (const char* env = getenv("NGPU_TILE_ROWS")) {
     const int r = atoi(env);
     if (r >= 1 && r <= n_slot_) rows = r;
@@ -2366,6 +2367,25 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   }
   upload(d_toff_, off, stream_);
   upload(d_trec_, rec, stream_);
+  // first step's plan: one item per tile; then the kernel plans each next step
+  std::vector<uint2> plan((size_t)nt * kTileSplitMax, make_uint2(0u, 0u));
+  for (long long t = 0; t < nt; ++t) plan[t] = make_uint2((unsigned int)t, 1u << 16);
+  const std::vector<unsigned int> tev((size_t)nt, 0u);
+  const std::vector<int> sched{(int)nt, 0, 0, 0};
+  upload(d_tplan_, plan, stream_);
+  upload(d_tev_, tev, stream_);
+  upload(d_tsched_, sched, stream_);
+  {
+    // persistent grid: every block slot of the chip (tile LDS per CU, at
+    // most 2 blocks of 16 waves)
+    int dev = 0, cus = 256, lds_cu = 160 * 1024;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    const int tile_bytes = rows * tc * (int)sizeof(double) + kTileStaticLds;
+    const int per_cu = std::max(1, std::min(2, lds_cu / tile_bytes));
+    tiled_blocks_ = cus * per_cu;
+  }
   sync_streams();
   tile_ch_ = tc;
   tile_rows_ = rows;
@@ -2383,6 +2403,9 @@ void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagge
     ta.ring = d_ring_.p; ta.n_ch = n_ch_; ta.n_slot = n_slot_;
     ta.skip_lagged_d0 = skip_lagged_d0;
     ta.ctr = d_ctr_.p; ta.spikes = d_stats_.p;
+    ta.tile_ev = d_tev_.p; ta.plan = d_tplan_.p; ta.sched = d_tsched_.p;
+    ta.blocks = tiled_blocks_;
+    ta.split_min = tile_split_min_;
     timed(KT_DELIVER, [&] { launch_deliver_tiled(ta, stream_); });
     return;
   }

@@ -346,6 +346,11 @@ class Engine {
   int choose_tile_ch(int rows) const;
   DBuf<unsigned int> d_toff_;
   DBuf<TileRec> d_trec_;
+  DBuf<unsigned int> d_tev_;          // events per tile of the last step
+  DBuf<uint2> d_tplan_;               // work items of the next step
+  DBuf<int> d_tsched_;                // {items, next item, finished blocks, 0}
+  int tiled_blocks_ = 0;              // persistent grid of the tiled delivery
+  int tile_split_min_ = kTileSplitMin;  // NGPU_TILE_SPLIT_MIN
   bool defer_ok_ = false;              // inside Simulate() and not its last step
   int rk5_max_blocks_ = 1024;
   // batch kernel

@@ -225,14 +225,16 @@ struct TiledSyn {
   int rows;                    // relative slots r kept in LDS: 0 .. rows - 1
 };
 
-// Dense delivery of one step's spike list (deliver_tiled_kernel): each block
-// owns one tile of channels; its waves take the list's spikes 64 at a time,
-// read their (first, end) record runs for the tile from the source-major
-// offset rows, walk the 64 runs as one flat range of records, add every
-// record into the LDS tile acc[slot][channel] (FP64), and the block adds the
-// tile into the ring once.  Replaces global atomics by LDS atomics and one
-// coalesced read-modify-write per ring entry that received input, for
-// networks where a step's events outnumber the ring's entries.
+// Dense delivery of one step's spike list (deliver_tiled_kernel): persistent
+// blocks take work items (a channel tile, and every K-th group of 64 spikes
+// of the list from group k) from a list planned from the previous step's
+// events per tile; for an item, the block's waves read the spikes'
+// (first, end) record runs for the tile from the source-major offset rows,
+// walk 64 runs at a time as one flat range of records, add every record into
+// the LDS tile acc[r][channel] (FP64), and the block adds the tile into the
+// ring once.  Replaces global atomics by LDS atomics and one coalesced
+// read-modify-write per ring entry that received input, for networks where a
+// step's events outnumber the ring's entries.
 struct TiledDeliverArgs {
   StepCtx ctx;           // this step's time (by value)
   SpikeList sl;
@@ -243,13 +245,23 @@ struct TiledDeliverArgs {
   int skip_lagged_d0;    // the d_int = 0 synapses of lagged spikes were delivered already
   unsigned long long* ctr;     // per-wave counter rows (column 3) or null
   unsigned long long* spikes;  // delivered spikes (one add per launch) or null
+  unsigned int* tile_ev;       // [n_tiles] events of this step per tile (zeroed by the planner)
+  uint2* plan;                 // [n_tiles * kTileSplitMax] items: (tile, k | K << 16)
+  int* sched;                  // {items, next item, finished blocks}
+  int blocks;                  // persistent grid
+  int split_min;               // no split below this many events per item
 };
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s);
+// a tile is split over up to kTileSplitMax items; by default no split below
+// kTileSplitMin events per item
+constexpr int kTileSplitMax = 16;
+constexpr int kTileSplitMin = 16384;
 // LDS bytes of one tile block at most (tile_ch * rows * 8): two blocks of
 // 16 waves fill a CU's 160 KB; tile_ch a multiple of 8, at most
 // kTileChMax; delay slots below 256 (the record's delay byte)
-constexpr int kTileLdsBytes = 80 * 1024;
+constexpr int kTileLdsBytes = 80 * 1024 - 512;
 constexpr int kTileChMax = 2048;
+constexpr int kTileStaticLds = 512;   // the kernel's own LDS beside the tile
 
 struct DeliverArgs {
   StepCtx ctx;           // this step's time (by value)

@@ -230,40 +230,103 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(deliver_kernel, dim3(kDeliverBlocks), dim3(256), 0, s, a);
 }
 
-// Dense delivery (TiledDeliverArgs): block b owns one channel tile.  Blocks
-// are handed out round-robin over the 8 XCDs, so block b takes tile
-// xcd_tile(b): XCD x gets a contiguous run of tiles, and what neighbouring
-// tiles read of a source (its offsets off[src][tile], and its record runs,
-// adjacent in memory) meets in the same L2.  Each wave takes the list's
-// spikes 64 at a time: lane l reads the (first, end) run of (spike l, tile)
-// from its source's offset row, a wave prefix sum turns the 64 runs into one
-// flat range of records, and the wave walks that range 64
-// records per pass, each lane finding its spike by a 6-step binary search
-// over the prefix sums (lane shuffles) -- every lane busy, and one pass reads
-// a few whole runs instead of one record of 64 different runs.  Every record
-// adds (double)(height * weight) for ring slot t + r (r = d_int + 1 - lag)
-// into the LDS tile acc[r][channel] with FP64 LDS atomics when r < rows,
-// else straight into the ring with an FP64 global atomic; then the block
-// adds the tile into the ring (plain read-modify-write of rows r < rows: no
-// other block touches these ring entries, no atomic of this block targets
-// them, and no other kernel runs meanwhile).  The sums are the exact FP64
+// Dense delivery (TiledDeliverArgs).  The work of a step is a list of items
+// (tile, k, K): the spike groups k, k + K, k + 2K, ... of the step's list
+// into one channel tile.  Persistent blocks take items from the list with an
+// atomic counter.  For an item, the block's waves take the list's spikes 64
+// at a time: lane l reads the (first, end) run of (spike l, tile) from its
+// source's offset row, a wave prefix sum turns the 64 runs into one flat
+// range of records, and the wave walks that range 64 records per pass, each
+// lane finding its spike by a 6-step binary search over the prefix sums
+// (lane shuffles) -- every lane busy, and one pass reads a few whole runs
+// instead of one record of 64 different runs.  Every record adds
+// (double)(height * weight) for ring slot t + r (r = d_int + 1 - lag) into
+// the LDS tile acc[r][channel] with FP64 LDS atomics when r < rows, else
+// straight into the ring with an FP64 global atomic; then the block adds the
+// tile into the ring: a plain read-modify-write of rows r < rows when the
+// tile is one item (no other block touches these entries, no atomic of this
+// block targets them, and no other kernel runs meanwhile), FP64 global
+// atomics when it is split over K > 1 items.  The sums are the exact FP64
 // sums of the float products, as the global-atomic delivery's, so the ring
-// receives the same values bit for bit.
+// receives the same values bit for bit whatever the split and order.
+//
+// The plan: each item adds its event count to tile_ev[tile]; the last block
+// to finish (a counter of finished blocks) reads the step's counts and
+// writes the next step's items: tile t split into K_t = ceil(ev_t / E)
+// items (at most kTileSplitMax), E = max(events / blocks, split_min), so
+// that no item holds much more than a block's share of the step; items
+// ordered by size, largest first (log2 buckets), so the blocks that take
+// the heavy items early and the light ones fill in behind.  A burst's load
+// moves from tile to tile over several steps, which the one-step-old counts
+// follow.
 constexpr int kTiledThreads = 1024;
 constexpr int kTiledUnroll = 4;
-constexpr int kRingBatch = 4;
+constexpr int kRingBatch = 2;
 
-__device__ inline int xcd_tile(int b, int n) {
-  const int xcd = b & 7, local = b >> 3, base = n >> 3, rem = n & 7;
-  return xcd < rem ? xcd * (base + 1) + local : rem * (base + 1) + (xcd - rem) * base + local;
+__device__ inline void tiled_plan(const TiledDeliverArgs& a) {
+  __shared__ unsigned long long s_total;
+  __shared__ unsigned int s_hist[32];
+  const int nt = a.ts.n_tiles;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x == 0) s_total = 0;
+  if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long part = 0;
+  for (int t = threadIdx.x; t < nt; t += blockDim.x)
+    part += __hip_atomic_load(&a.tile_ev[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  part = wave_sum(part);
+  if (lane == 0) atomicAdd(&s_total, part);
+  __syncthreads();
+  // a heuristic: float arithmetic is exact enough here
+  const float inv_e = 1.0f / fmaxf((float)s_total / (float)gridDim.x, (float)a.split_min);
+  auto split = [&](unsigned int ev, unsigned int& bucket) {
+    const float q = ceilf((float)ev * inv_e);
+    const unsigned int k = q < 1.0f ? 1u : (q > (float)kTileSplitMax ? (unsigned int)kTileSplitMax : (unsigned int)q);
+    const unsigned int size = ev / k;
+    bucket = 31u - (unsigned int)__builtin_clz(size | 1u);
+    return k;
+  };
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    unsigned int bk;
+    const unsigned int k = split(__hip_atomic_load(&a.tile_ev[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), bk);
+    atomicAdd(&s_hist[bk], k);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // largest bucket first: exclusive bases in descending bucket order
+    unsigned int run = 0;
+    for (int b = 31; b >= 0; --b) {
+      const unsigned int c = s_hist[b];
+      s_hist[b] = run;
+      run += c;
+    }
+    a.sched[0] = (int)run;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+    unsigned int bk;
+    const unsigned int ev = __hip_atomic_load(&a.tile_ev[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int k = split(ev, bk);
+    const unsigned int pos = atomicAdd(&s_hist[bk], k);
+    for (unsigned int j = 0; j < k; ++j) a.plan[pos + j] = make_uint2((unsigned int)t, j | (k << 16));
+    a.tile_ev[t] = 0;
+  }
+  if (threadIdx.x == 0) {
+    a.sched[1] = 0;
+    a.sched[2] = 0;
+  }
 }
 
-__global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliverArgs a) {
+// at most 64 VGPRs: two 16-wave blocks per CU
+__global__ void __launch_bounds__(kTiledThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
+deliver_tiled_kernel(TiledDeliverArgs a) {
   extern __shared__ double acc[];
+  __shared__ int s_item;
+  __shared__ unsigned int s_ev;
+  __shared__ int s_last;
   const int tc = a.ts.tile_ch, ns = a.n_slot, rows = a.ts.rows;
   const int n_acc = rows * tc;
   const int ts0 = (int)(a.ctx.t_idx % ns);   // ring slot of r = 0
-  const int tile = xcd_tile(blockIdx.x, a.ts.n_tiles);
   const int lst = a.ctx.lst;
   int n_spk = a.sl.count[lst];
   if (n_spk > a.sl.cap) n_spk = a.sl.cap;
@@ -272,139 +335,172 @@ __global__ void __launch_bounds__(kTiledThreads) deliver_tiled_kernel(TiledDeliv
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_wv = blockDim.x / kWave;
-  const long long c0 = (long long)tile * tc;
-  for (int k = threadIdx.x; k < n_acc; k += blockDim.x) acc[k] = 0.0;
-  __syncthreads();
-  const unsigned int* off = a.ts.off + tile;
+  const int n_items = a.sched[0];
   const size_t row = (size_t)a.ts.n_tiles + 1;
-  unsigned long long n_ev = 0;
-  // Software pipeline over the wave's groups: while group k is walked, the
-  // offsets of group k + 1 and the sources and heights of group k + 2 are
-  // in flight.
-  const int gs = n_wv * kWave;
-  int s = wv * kWave + lane;
-  unsigned int r0 = 0, b0 = 0, e0 = 0, r1 = 0;
-  float h0 = 0.f, h1 = 0.f;
-  if (s < n_spk) { r0 = (unsigned int)srcs[s]; h0 = hts[s]; }
-  if (s + gs < n_spk) { r1 = (unsigned int)srcs[s + gs]; h1 = hts[s + gs]; }
-  if (s < n_spk) {
-    const unsigned int* o = off + (size_t)(r0 & ~kSpikeLagBit) * row;
-    b0 = o[0];
-    e0 = o[1];
-  }
-  for (int g0 = wv * kWave; g0 < n_spk; g0 += gs, s += gs) {
-    // lane l: spike g0 + l (or an empty run past the list's end)
-    const bool valid = s < n_spk;
-    const unsigned int beg = b0, cnt = valid ? e0 - b0 : 0u;
-    // bit 0: lagged spike (r = d_int), bit 1: its d_int = 0 synapses are
-    // delivered already
-    const unsigned int meta = (r0 >> 31) | ((a.skip_lagged_d0 && (r0 >> 31)) ? 2u : 0u);
-    const float hgt = h0;
-    unsigned int nb = 0, ne = 0, r2 = 0;
-    float h2 = 0.f;
-    if (s + gs < n_spk) {
-      const unsigned int* o = off + (size_t)(r1 & ~kSpikeLagBit) * row;
-      nb = o[0];
-      ne = o[1];
+  // tiles can outnumber the counter rows: the waves add their counts into
+  // the first kCtrMinRows rows with atomics
+  unsigned long long* ctr_row =
+      a.ctr ? a.ctr + (size_t)(((long long)blockIdx.x * n_wv + wv) % kCtrMinRows) * kCtrCols + 3 : nullptr;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(&a.sched[1], 1);
+      s_ev = 0;
     }
-    if (s + 2 * gs < n_spk) { r2 = (unsigned int)srcs[s + 2 * gs]; h2 = hts[s + 2 * gs]; }
-    unsigned int incl = cnt;
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= n_items) break;
+    const uint2 it = a.plan[item];
+    const int tile = __builtin_amdgcn_readfirstlane((int)it.x);
+    const int kk = __builtin_amdgcn_readfirstlane((int)(it.y & 0xffffu));
+    const int nk = __builtin_amdgcn_readfirstlane((int)(it.y >> 16));
+    const long long c0 = (long long)tile * tc;
+    const unsigned int* off = a.ts.off + tile;
+    for (int k = threadIdx.x; k < n_acc; k += blockDim.x) acc[k] = 0.0;
+    __syncthreads();
+    unsigned int item_ev = 0;
+    // the item's groups kk, kk + nk, ...; wave wv takes every n_wv-th of
+    // them.  Every load is issued by every lane (a lane past the list's end
+    // reads entry 0 and source 0's offsets, and masks them), so the compiler
+    // counts the loads in flight and waits only for the ones it uses.
+    const int gs = nk * n_wv * kWave;
+    for (int s = (kk + nk * wv) * kWave + lane; s - lane < n_spk; s += gs) {
+      // lane l: spike s (or an empty run past the list's end)
+      const bool valid = s < n_spk;
+      const unsigned int raw = (unsigned int)srcs[valid ? s : 0];
+      const unsigned int r0 = valid ? raw : 0u;
+      const float hgt = hts[valid ? s : 0];
+      const unsigned int* o = off + (size_t)(r0 & ~kSpikeLagBit) * row;
+      const unsigned int beg = o[0];
+      const unsigned int cnt = valid ? o[1] - beg : 0u;
+      // bit 0: lagged spike (r = d_int), bit 1: its d_int = 0 synapses are
+      // delivered already
+      const unsigned int meta = (r0 >> 31) | ((a.skip_lagged_d0 && (r0 >> 31)) ? 2u : 0u);
+      unsigned int incl = cnt;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const unsigned int t = __shfl_up(incl, d, kWave);
-      if (lane >= d) incl += t;
-    }
-    const unsigned int pre = incl - cnt;
-    const unsigned int total = __shfl(incl, kWave - 1, kWave);
-    for (unsigned int j0 = 0; j0 < total; j0 += kWave * kTiledUnroll) {
-      unsigned int jj[kTiledUnroll], lo[kTiledUnroll];
-#pragma unroll
-      for (int u = 0; u < kTiledUnroll; ++u) {
-        jj[u] = j0 + u * kWave + lane;
-        lo[u] = 0;
+      for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned int t = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += t;
       }
-      // spike of flat record j: the last lane whose run starts at or before
-      // j (a lane with an empty run shares its start with the next lane)
-#pragma unroll
-      for (int step = kWave / 2; step > 0; step >>= 1) {
+      const unsigned int pre = incl - cnt;
+      const unsigned int total = __shfl(incl, kWave - 1, kWave);
+      for (unsigned int j0 = 0; j0 < total; j0 += kWave * kTiledUnroll) {
+        unsigned int jj[kTiledUnroll], lo[kTiledUnroll];
 #pragma unroll
         for (int u = 0; u < kTiledUnroll; ++u) {
-          const unsigned int p = __shfl(pre, (int)lo[u] + step, kWave);
-          if (p <= jj[u]) lo[u] += step;
+          jj[u] = j0 + u * kWave + lane;
+          lo[u] = 0;
+        }
+        // spike of flat record j: the last lane whose run starts at or
+        // before j (a lane with an empty run shares its start with the next)
+#pragma unroll
+        for (int step = kWave / 2; step > 0; step >>= 1) {
+#pragma unroll
+          for (int u = 0; u < kTiledUnroll; ++u) {
+            const unsigned int p = __shfl(pre, (int)lo[u] + step, kWave);
+            if (p <= jj[u]) lo[u] += step;
+          }
+        }
+        TileRec rec[kTiledUnroll];
+        unsigned int m[kTiledUnroll];
+        float h[kTiledUnroll];
+#pragma unroll
+        for (int u = 0; u < kTiledUnroll; ++u) {
+          // record index beg + (j - pre), as one shuffled value (mod 2^32);
+          // a lane past the range loads record 0 (no branch around the
+          // loads, so all of them are in flight together)
+          const unsigned int bp = __shfl(beg - pre, (int)lo[u], kWave);
+          m[u] = __shfl(meta, (int)lo[u], kWave);
+          h[u] = __shfl(hgt, (int)lo[u], kWave);
+          rec[u] = a.ts.rec[jj[u] < total ? bp + jj[u] : 0u];
+        }
+#pragma unroll
+        for (int u = 0; u < kTiledUnroll; ++u) {
+          if (jj[u] >= total) continue;
+          if ((m[u] & 2u) && rec[u].d == 0) continue;
+          const int r = (int)rec[u].d + 1 - (int)(m[u] & 1u);
+          const double v = (double)(h[u] * rec[u].w);
+          if (r < rows) {
+            atomicAdd(&acc[r * tc + rec[u].ch], v);
+          } else {
+            int slot = ts0 + r;
+            if (slot >= ns) slot -= ns;
+            atomicAdd(&a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + rec[u].ch)], v);
+          }
+          ++item_ev;
         }
       }
-      TileRec rec[kTiledUnroll];
-      unsigned int m[kTiledUnroll];
-      float h[kTiledUnroll];
-#pragma unroll
-      for (int u = 0; u < kTiledUnroll; ++u) {
-        // record index beg + (j - pre), as one shuffled value (mod 2^32)
-        const unsigned int bp = __shfl(beg - pre, (int)lo[u], kWave);
-        m[u] = __shfl(meta, (int)lo[u], kWave);
-        h[u] = __shfl(hgt, (int)lo[u], kWave);
-        if (jj[u] < total) rec[u] = a.ts.rec[bp + jj[u]];
+    }
+    {
+      const unsigned long long w_ev = wave_sum((unsigned long long)item_ev);
+      if (lane == 0 && w_ev) {
+        atomicAdd(&s_ev, (unsigned int)w_ev);
+        if (ctr_row) atomicAdd(ctr_row, w_ev);
       }
+    }
+    __syncthreads();
+    // tile -> ring, kRingBatch entries per thread at a time: every ring load
+    // of the batch is issued before its stores
+    const long long left = a.n_ch - c0;
+    const int w = left < tc ? (int)left : tc;
+    if (nk == 1) {
+      for (int k0 = threadIdx.x; k0 < n_acc; k0 += kRingBatch * blockDim.x) {
+        double v[kRingBatch], r[kRingBatch];
+        size_t gi[kRingBatch];
 #pragma unroll
-      for (int u = 0; u < kTiledUnroll; ++u) {
-        if (jj[u] >= total) continue;
-        if ((m[u] & 2u) && rec[u].d == 0) continue;
-        const int r = (int)rec[u].d + 1 - (int)(m[u] & 1u);
-        const double v = (double)(h[u] * rec[u].w);
-        if (r < rows) {
-          atomicAdd(&acc[r * tc + rec[u].ch], v);
-        } else {
-          int slot = ts0 + r;
-          if (slot >= ns) slot -= ns;
-          atomicAdd(&a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + rec[u].ch)], v);
+        for (int u = 0; u < kRingBatch; ++u) {
+          const int k = k0 + u * (int)blockDim.x;
+          const int rr = k / tc, c = k - rr * tc;
+          v[u] = (k < n_acc && c < w) ? acc[k] : 0.0;
+          const int slot = ts0 + rr >= ns ? ts0 + rr - ns : ts0 + rr;
+          gi[u] = (size_t)slot * a.n_ch + (size_t)(c0 + c);
         }
-        ++n_ev;
+#pragma unroll
+        for (int u = 0; u < kRingBatch; ++u)
+          if (v[u] != 0.0) r[u] = a.ring[gi[u]];
+#pragma unroll
+        for (int u = 0; u < kRingBatch; ++u)
+          if (v[u] != 0.0) a.ring[gi[u]] = r[u] + v[u];
+      }
+    } else {
+      for (int k = threadIdx.x; k < n_acc; k += blockDim.x) {
+        const int rr = k / tc, c = k - rr * tc;
+        const double v = c < w ? acc[k] : 0.0;
+        const int slot = ts0 + rr >= ns ? ts0 + rr - ns : ts0 + rr;
+        if (v != 0.0) atomicAdd(&a.ring[(size_t)slot * a.n_ch + (size_t)(c0 + c)], v);
       }
     }
-    r0 = r1; h0 = h1; b0 = nb; e0 = ne;
-    r1 = r2; h1 = h2;
+    if (threadIdx.x == 0 && s_ev) atomicAdd(&a.tile_ev[tile], s_ev);
+    __syncthreads();   // acc, s_item and s_ev are reused by the next item
   }
-  __syncthreads();
-  // tile -> ring, kRingBatch entries per thread at a time: every ring load
-  // of the batch is issued before its stores
-  const long long left = a.n_ch - c0;
-  const int w = left < tc ? (int)left : tc;
-  for (int k0 = threadIdx.x; k0 < n_acc; k0 += kRingBatch * blockDim.x) {
-    double v[kRingBatch], r[kRingBatch];
-    size_t gi[kRingBatch];
-#pragma unroll
-    for (int u = 0; u < kRingBatch; ++u) {
-      const int k = k0 + u * (int)blockDim.x;
-      const int r = k / tc, c = k - r * tc;
-      v[u] = (k < n_acc && c < w) ? acc[k] : 0.0;
-      const int slot = ts0 + r >= ns ? ts0 + r - ns : ts0 + r;
-      gi[u] = (size_t)slot * a.n_ch + (size_t)(c0 + c);
-    }
-#pragma unroll
-    for (int u = 0; u < kRingBatch; ++u)
-      if (v[u] != 0.0) r[u] = a.ring[gi[u]];
-#pragma unroll
-    for (int u = 0; u < kRingBatch; ++u)
-      if (v[u] != 0.0) a.ring[gi[u]] = r[u] + v[u];
-  }
-  n_ev = wave_sum(n_ev);
-  // tiles can outnumber the counter rows: spread the waves over the first
-  // kCtrMinRows rows with atomics
-  if (lane == 0 && a.ctr && n_ev)
-    atomicAdd(&a.ctr[(size_t)(((long long)blockIdx.x * n_wv + wv) % kCtrMinRows) * kCtrCols + 3], n_ev);
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     if (a.spikes && n_spk) *a.spikes += (unsigned long long)n_spk;
     // the next step writes list (lst + 1) mod 3: clear its count now (it held
     // the list of step t_idx - 2, delivered two steps ago)
     a.sl.count[lst == kSpikeLists - 1 ? 0 : lst + 1] = 0;
   }
+  // The last block to finish plans the next step.  Its inputs are the
+  // tile_ev counts, which every block's thread 0 adds with device-scope
+  // atomics (performed beyond the per-XCD L2s) before its finished-block
+  // increment: a workgroup-scope release makes that thread wait for its
+  // atomics to complete, and the planner reads the counts with device-scope
+  // atomic loads.  (A device-scope fence here would write back and
+  // invalidate the whole L2 of the XCD under every other block still at
+  // work.)
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    s_last = atomicAdd(&a.sched[2], 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) tiled_plan(a);
 }
 
 void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
   const size_t lds = (size_t)a.ts.rows * a.ts.tile_ch * sizeof(double);
   if (lds > (size_t)kTileLdsBytes || a.ts.n_tiles <= 0 || a.ts.tile_ch > kTileChMax ||
-      a.ts.rows < 1 || a.ts.rows > a.n_slot || a.n_slot > 257)
-    throw std::runtime_error("tiled delivery: tile does not fit its LDS budget");
-  hipLaunchKernelGGL(deliver_tiled_kernel, dim3((unsigned)a.ts.n_tiles), dim3(kTiledThreads), lds, s, a);
+      a.ts.rows < 1 || a.ts.rows > a.n_slot || a.n_slot > 257 || a.blocks <= 0 ||
+      !a.tile_ev || !a.plan || !a.sched)
+    throw std::runtime_error("tiled delivery: tile does not fit its LDS budget or no work list");
+  hipLaunchKernelGGL(deliver_tiled_kernel, dim3((unsigned)a.blocks), dim3(kTiledThreads), lds, s, a);
 }
 
 // Fused delivery, right after the step's update launch: the d_int = 0
