@@ -17,9 +17,10 @@
 // Inference runs per wavefront on the matrix cores: with the 64 neurons of a
 // wave as the N dimension in two blocks of 32, layer 1 is one
 // v_mfma_f32_32x32x8_bf16 per block (M = 32 hidden units, K = 8 inputs) and
-// layer 2 four (K = 32 in k-blocks of 8); the layer-1 result layout of a lane
-// (rows 8j + 4*(lane/32) + r, column lane%32) is exactly the B-operand layout
-// of layer 2's k-block j, so the hidden activations never leave registers.
+// layer 2 two v_mfma_f32_32x32x16_bf16 (K = 32 in k-blocks of 16); a lane's
+// layer-1 results (rows 8j + 4*(lane/32) + r, column lane%32) serve as its
+// layer-2 B operand as they are, with the W2 columns of the A operand
+// numbered to match, so the hidden activations never leave registers.
 // Inputs and W1, W2 and the layer-1 activations are bfloat16 (round to
 // nearest even); accumulation, biases, layer 3 and the output are FP32.
 #pragma once

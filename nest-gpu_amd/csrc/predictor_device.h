@@ -55,6 +55,19 @@ __device__ inline void pred_raw_inputs(const float* y, float h_ctrl, float err_l
 
 typedef short pred_s4 __attribute__((ext_vector_type(4)));
 typedef float pred_f16 __attribute__((ext_vector_type(16)));
+typedef __bf16 pred_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int pred_u4 __attribute__((ext_vector_type(4)));
+
+// eight bfloat16 from four packed pairs
+__device__ inline pred_bf8 pred_pack8(unsigned int p0, unsigned int p1, unsigned int p2,
+                                      unsigned int p3) {
+  pred_u4 v;
+  v[0] = p0;
+  v[1] = p1;
+  v[2] = p2;
+  v[3] = p3;
+  return __builtin_bit_cast(pred_bf8, v);
+}
 
 __device__ inline pred_s4 pred_pack4(unsigned int lo, unsigned int hi) {
   pred_s4 v;
@@ -113,14 +126,22 @@ __device__ inline float pred_infer_wave(const PredLDS& s_in, const float* raw) {
   for (int blk = 0; blk < 2; ++blk) {
     pred_f16 h1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, blk ? b1 : b0, c1, 0, 0, 0);
     pred_f16 h2 = c2;
+    // layer 2 on gfx950's v_mfma_f32_32x32x16_bf16, two k-blocks of 16
+    // hidden units: lane (half, slot j) of k-block kb carries hidden unit
+    // 16 kb + 8 (j / 4) + 4 half + j % 4 in both operands -- its own layer-1
+    // results h1[8 kb + j] as B, the matching W2 columns as A (the sum over
+    // k does not depend on how the units are numbered within a block)
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      const unsigned short* w2 = s.w2 + col * kPH + 8 * kb + 4 * half;
-      const pred_s4 a2 = pred_pack4(w2[0] | ((unsigned int)w2[1] << 16),
-                                    w2[2] | ((unsigned int)w2[3] << 16));
-      const pred_s4 bb = pred_pack4(pred_pair(fmaxf(h1[4 * kb], 0.0f), fmaxf(h1[4 * kb + 1], 0.0f)),
-                                    pred_pair(fmaxf(h1[4 * kb + 2], 0.0f), fmaxf(h1[4 * kb + 3], 0.0f)));
-      h2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, bb, h2, 0, 0, 0);
+    for (int kb = 0; kb < 2; ++kb) {
+      const unsigned short* w2 = s.w2 + col * kPH + 16 * kb + 4 * half;
+      const pred_bf8 a2 = pred_pack8(w2[0] | ((unsigned int)w2[1] << 16), w2[2] | ((unsigned int)w2[3] << 16),
+                                     w2[8] | ((unsigned int)w2[9] << 16), w2[10] | ((unsigned int)w2[11] << 16));
+      const int i0 = 8 * kb;
+      const pred_bf8 bb = pred_pack8(pred_pair(fmaxf(h1[i0], 0.0f), fmaxf(h1[i0 + 1], 0.0f)),
+                                     pred_pair(fmaxf(h1[i0 + 2], 0.0f), fmaxf(h1[i0 + 3], 0.0f)),
+                                     pred_pair(fmaxf(h1[i0 + 4], 0.0f), fmaxf(h1[i0 + 5], 0.0f)),
+                                     pred_pair(fmaxf(h1[i0 + 6], 0.0f), fmaxf(h1[i0 + 7], 0.0f)));
+      h2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, bb, h2, 0, 0, 0);
     }
     float part = 0.0f;
 #pragma unroll
