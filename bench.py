@@ -66,6 +66,13 @@ DELIVER_BYTES_PER_EVENT = 4 + 4 + 2 + 16
 # generators, as in every workload here) none: weight, delay and stream are
 # launch arguments and the generator's table entry stays in cache
 FOLD_BYTES_PER_CHANNEL = 24
+# dense (tiled) delivery: an 8-byte record per event, the (first, end)
+# offsets of every (spike, tile) pair, and one FP64 ring read and write per
+# ring entry that received input (at most one per event, at most every
+# entry of the LDS rows per launch)
+TILE_REC_BYTES = 8
+TILE_OFF_BYTES = 8
+RING_RMW_BYTES = 16
 # iaf_psc_exp update: 4 state floats read and written per neuron-step
 IAF_BYTES_PER_NEURON = 32
 POISSON_BYTES_PER_EDGE = 20
@@ -184,7 +191,7 @@ def window_spike_stats(times, t0_ms, t1_ms, n):
     return cnt / n / ((t1_ms - t0_ms) * 1e-3), (float(np.mean(cvs)) if cvs else None), len(cvs)
 
 
-def rooflines(w, n, n_port, steps, n_poisson_edges):
+def rooflines(w, n, n_port, steps, n_poisson_edges, tiled=None):
     kt, st = w["kt"], w["stats"]
     out = {}
     ms, cnt = kt["rk5_update"]
@@ -206,6 +213,17 @@ def rooflines(w, n, n_port, steps, n_poisson_edges):
     if cnt and cnt < steps // 2:
         out["deliver"] = {"note": "delivery runs inside the update launches (fused); the %d "
                                   "delivery launch(es) only close Simulate calls" % cnt}
+    elif cnt and tiled:
+        ev = st["syn_events"]
+        entries = n * n_port * tiled["tile_rows"]
+        b = (ev * TILE_REC_BYTES + st["spikes"] * tiled["n_tiles"] * TILE_OFF_BYTES
+             + RING_RMW_BYTES * min(ev, cnt * entries))
+        a = b / (ms * 1e-3) / 1e9
+        out["deliver"] = {"bound": "hbm", "unit": "GB/s", "achieved": a, "peak": PEAK_HBM_GBS,
+                          "frac": a / PEAK_HBM_GBS, "avg_launch_us": ms * 1e3 / cnt,
+                          "bytes_per_launch": b / cnt,
+                          "note": "dense tiled delivery: 8 B record per event + 8 B offsets per "
+                                  "(spike, tile) + 16 B per ring entry with input"}
     elif cnt:
         b = st["syn_events"] * DELIVER_BYTES_PER_EVENT
         a = b / (ms * 1e-3) / 1e9
@@ -394,17 +412,19 @@ def main():
     nb = 2000
     wb = timed_window(ngpu, nb, timing=1)
     log("breakdown: %d steps, wall %.3f s, kernel ms %s" % (nb, wb["wall"], wb["kt"]))
-    roof_all = rooflines(wb, n, n_port, nb, n_poisson_csr_edges)
+    tiled = batch["config"] if batch and batch["config"].get("tiled") else None
+    roof_all = rooflines(wb, n, n_port, nb, n_poisson_csr_edges, tiled)
     breakdown = {k: v[0] * 1e3 / nb for k, v in wb["kt"].items()}
     # the dominant kernel's roofline from the plain timed window (its RK5
     # launches were bracketed by events there); HBM traffic per launch from
     # the committed FETCH_SIZE/WRITE_SIZE summary of this workload, if any
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
-    roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges).get(dom, roof_all.get(dom, {})))
+    roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges, tiled).get(dom, roof_all.get(dom, {})))
     rk5_kname = ("rk5_batch_kernel<%d>" % (3 + batch["config"]["variant"]) if bt else
                  "rk5_persistent_kernel<AeifCondBeta<%d>, true, 0>" % n_port)
     kname = {"rk5_update": rk5_kname,
-             "deliver": "deliver_kernel", "fold_poisson": "fold_kernel<%d>" % n_port,
+             "deliver": "deliver_tiled_kernel" if tiled else "deliver_kernel",
+             "fold_poisson": "fold_kernel<%d>" % n_port,
              "iaf_update": "iaf_kernel"}.get(dom, dom)
     roof["kernel"] = kname
     roof["traffic"] = None
