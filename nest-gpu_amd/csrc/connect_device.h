@@ -1,0 +1,33 @@
+// Connection store flattening on the GPU (connect_device.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace ngpu {
+
+// Stable order of n connections by (source, d_int): perm[k] = index of the
+// k-th connection in that order; row_ptr[s] = first position of source s
+// (n_nodes + 1 entries).  src[k] in [0, n_nodes), d[k] in [0, 65535].  Returns
+// false (nothing done) when n is 0 or at least 2^32; throws on a device error.
+// Synchronous on `stream`.
+bool device_sort_src_delay(const int* src, const int* d, size_t n, int n_nodes,
+                           std::vector<unsigned int>& perm, std::vector<long long>& row_ptr,
+                           hipStream_t stream);
+
+// fixed_indegree / fixed_outdegree on the GPU: out[o * deg + i] = the i-th
+// distinct pool index (of n_pool) drawn for owner o, the same draws as the
+// host rule under the call's 64-bit key.  out is host memory of
+// n_own * deg ints.  Synchronous on `stream`.
+void device_pick_distinct(int n_own, int deg, int n_pool, uint64_t key, int* out,
+                          hipStream_t stream);
+
+// n 32-bit words of the host generator's sequence from block counter ctr0
+// (4 words per block, the last block cut): the same values as
+// Engine::RandomInt.  out is host memory.  Synchronous on `stream`.
+void device_random_words(uint64_t ctr0, uint64_t seed, size_t n, unsigned int* out,
+                         hipStream_t stream);
+
+}  // namespace ngpu

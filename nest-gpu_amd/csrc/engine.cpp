@@ -12,6 +12,7 @@
 #include <omp.h>
 #endif
 
+#include "connect_device.h"
 #include "philox.h"
 #include "predictor_host.h"
 
@@ -1013,7 +1014,14 @@ int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* t
       break;
     case 2:
       n = cs.total_num;
-      rnd = RandomInt(2 * (size_t)cs.total_num);
+      if (device_connect(n)) {
+        // the host generator's next 2n words, drawn on the GPU
+        rnd = new unsigned int[2 * (size_t)n];
+        device_random_words(rng_ctr_, seed_, 2 * (size_t)n, rnd, stream_);
+        rng_ctr_ += (2 * (uint64_t)n + 3) / 4;
+      } else {
+        rnd = RandomInt(2 * (size_t)cs.total_num);
+      }
       break;
     case 3:    // fixed_indegree: distinct sources per target, drawn per target
     case 4: {  // fixed_outdegree: distinct targets per source, drawn per source
@@ -1027,6 +1035,10 @@ int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* t
       n = (long long)n_own * deg;
       const uint64_t key = seed_ ^ ((uint64_t)(++rng_ctr_) << 40);
       chosen.resize(n);
+      if (device_connect(n)) {
+        device_pick_distinct(n_own, deg, n_pool, key, chosen.data(), stream_);
+        break;
+      }
 #pragma omp parallel
       {
         std::vector<char> used(n_pool, 0);
@@ -1110,11 +1122,61 @@ int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* t
   return 0;
 }
 
+// Random connection rules draw on the GPU (connect_device.hip, the same
+// Philox words as the host loops) from 2^20 connections on;
+// NGPU_DEVICE_CONNECT=1 always, 0 never.
+bool Engine::device_connect(long long n) const {
+  if (const char* env = getenv("NGPU_DEVICE_CONNECT")) return atoi(env) != 0 && n > 0;
+  return n >= (1ll << 20);
+}
+
 // Stable sort of the store by (source, d_int): counting sort by source, then
 // a stable sort by delay inside each source segment.
 void Engine::sort_connections() {
   const size_t nc = c_src_.size();
   if (sorted_ && row_ptr_.size() == (size_t)n_nodes_ + 1) return;
+  if (!sort_connections_device()) sort_connections_host();
+  // delay groups
+  grp_ptr_.assign(n_nodes_ + 1, 0);
+  grp_start_.clear();
+  for (int s = 0; s < n_nodes_; ++s) {
+    grp_ptr_[s] = (long long)grp_start_.size();
+    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k)
+      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) grp_start_.push_back(k);
+  }
+  grp_ptr_[n_nodes_] = (long long)grp_start_.size();
+  grp_start_.push_back((long long)nc);
+  sorted_ = true;
+}
+
+// The same order on the GPU (device_sort_src_delay: a stable radix sort of
+// (source << 16 | d_int) keys), then one gather per array on the host.  Used
+// from 2^20 connections on (NGPU_DEVICE_SORT=1 always, 0 never); false when
+// not used.
+bool Engine::sort_connections_device() {
+  const size_t nc = c_src_.size();
+  int mode = -1;
+  if (const char* env = getenv("NGPU_DEVICE_SORT")) mode = atoi(env) != 0 ? 1 : 0;
+  if (mode == 0 || nc == 0 || nc >= (1ull << 32) || (mode < 0 && nc < (1u << 20))) return false;
+  std::vector<unsigned int> perm;
+  std::vector<long long> rp;
+  if (!device_sort_src_delay(c_src_.data(), c_d_.data(), nc, n_nodes_, perm, rp, stream_)) return false;
+  auto gather = [&](auto& v) {
+    using T = typename std::decay<decltype(v)>::type::value_type;
+    std::vector<T> o(nc);
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)nc; ++k) o[k] = v[perm[k]];
+    v.swap(o);
+  };
+  gather(c_src_); gather(c_tgt_); gather(c_port_); gather(c_syn_); gather(c_w_); gather(c_d_);
+  row_ptr_.swap(rp);
+  return true;
+}
+
+// Counting sort by source (creation order kept), then a stable sort by
+// delay inside each source segment.
+void Engine::sort_connections_host() {
+  const size_t nc = c_src_.size();
   std::vector<long long> cnt(n_nodes_ + 1, 0);
   for (size_t k = 0; k < nc; ++k) cnt[c_src_[k] + 1]++;
   for (int s = 0; s < n_nodes_; ++s) cnt[s + 1] += cnt[s];
@@ -1161,17 +1223,6 @@ void Engine::sort_connections() {
       }
     }
   }
-  // delay groups
-  grp_ptr_.assign(n_nodes_ + 1, 0);
-  grp_start_.clear();
-  for (int s = 0; s < n_nodes_; ++s) {
-    grp_ptr_[s] = (long long)grp_start_.size();
-    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k)
-      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) grp_start_.push_back(k);
-  }
-  grp_ptr_[n_nodes_] = (long long)grp_start_.size();
-  grp_start_.push_back((long long)nc);
-  sorted_ = true;
 }
 
 std::vector<long long> Engine::GetConnections(const int* src_list, int i_src0, int n_src,
@@ -2321,6 +2372,10 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   d_toff_.free(); d_trec_.free(); d_tev_.free(); d_tplan_.free(); d_tsched_.free();
   const size_t nc = ch.size();
   if (tiled_env_ == 0 || nc == 0 || nc >= (1ull << 32) || n_ch_ <= 0 || max_d_int_ > 255) return;
+  const bool dense = (double)nc >= 4.0 * (double)n_ch_ * (double)n_slot_;
+  if (tiled_env_ < 0 && !dense) return;
+  // even the widest tile gives too large an offset table: no tiled copy
+  if ((long long)n_nodes_ * ((n_ch_ + kTileChMax - 1) / kTileChMax + 1) >= (1ll << 28)) return;
   std::vector<long long> hist(max_d_int_ + 1, 0);
   for (size_t k = 0; k < nc; ++k) hist[dd[k]]++;
   int rows = n_slot_;            // r = d + 1 <= rows - 1 for every synapse
@@ -2336,8 +2391,6 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   const long long nt = (n_ch_ + tc - 1) / tc;
   const long long entries = (long long)n_nodes_ * (nt + 1);
   if (entries >= (1ll << 28)) return;
-  const bool dense = (double)nc >= 4.0 * (double)n_ch_ * (double)n_slot_;
-  if (tiled_env_ < 0 && !dense) return;
   std::vector<unsigned int> off((size_t)entries);
   std::vector<TileRec> rec(nc);
 #pragma omp parallel

@@ -261,6 +261,9 @@ class Engine {
   void set_param_index(Group& g, int j, const std::vector<int>& rel, const float* vals, bool whole);
   void set_var_index(Group& g, int j, const std::vector<int>& rel, const float* vals);
   void sort_connections();
+  bool sort_connections_device();
+  bool device_connect(long long n) const;
+  void sort_connections_host();
   void build_device_tables();
   void calibrate_group(Group& g);
   void upload_poisson_tables();
