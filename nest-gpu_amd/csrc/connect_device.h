@@ -1,4 +1,4 @@
-// Connection store flattening on the GPU (connect_device.hip).
+// Device-side connectivity (connect_device.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,14 +8,16 @@
 
 namespace ngpu {
 
-// Stable order of n connections by (source, d_int): perm[k] = index of the
-// k-th connection in that order; row_ptr[s] = first position of source s
-// (n_nodes + 1 entries).  src[k] in [0, n_nodes), d[k] in [0, 65535].  Returns
-// false (nothing done) when n is 0 or at least 2^32; throws on a device error.
-// Synchronous on `stream`.
-bool device_sort_src_delay(const int* src, const int* d, size_t n, int n_nodes,
-                           std::vector<unsigned int>& perm, std::vector<long long>& row_ptr,
-                           hipStream_t stream);
+// Sorts the connection store (host arrays of n entries) in place into the
+// stable order by (source, d_int): the sort and the gathers run on the GPU,
+// tgt, port, w and d come back sorted (src is left to the caller to refill
+// from row_ptr: source s at [row_ptr[s], row_ptr[s + 1])); row_ptr[s] =
+// first position of source s (n_nodes + 1 entries).  src[k] in
+// [0, n_nodes), d[k] in [0, 65535].  Returns false (nothing done) when n is
+// 0 or at least 2^32; throws on a device error (the arrays are then
+// unchanged).  Synchronous on `stream`.
+bool device_sort_connections(int* src, int* tgt, unsigned char* port, float* w, int* d, size_t n,
+                             int n_nodes, std::vector<long long>& row_ptr, hipStream_t stream);
 
 // fixed_indegree / fixed_outdegree on the GPU: out[o * deg + i] = the i-th
 // distinct pool index (of n_pool) drawn for owner o, the same draws as the

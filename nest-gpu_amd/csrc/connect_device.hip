@@ -1,4 +1,5 @@
-// Connection store flattening on the GPU: the stable order of the
+// Device-side connectivity: the random connection rules' draws, and the
+// connection store flattening -- the stable order of the
 // connections by (source, delay slot) that the synapse table needs (the
 // reference flattens its per-source connection lists on the host,
 // src/spike_buffer.cu:399-437, after NetConnection::Insert kept each
@@ -40,12 +41,20 @@ __global__ void sort_keys_kernel(const int* src, const int* d, size_t n, unsigne
 struct DevMem {
   void* p = nullptr;
   ~DevMem() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipDeviceSynchronize();   // an error path may leave work queued
+      (void)hipFree(p);
+    }
   }
   template <class T>
   T* alloc(size_t n, const char* what) {
     check(hipMalloc(&p, n * sizeof(T) + 1), what);
     return static_cast<T*>(p);
+  }
+  // only after the work that uses the buffer has completed
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
   }
 };
 
@@ -158,40 +167,75 @@ void device_random_words(uint64_t ctr0, uint64_t seed, size_t n, unsigned int* o
   check(hipStreamSynchronize(stream), "sync");
 }
 
-bool device_sort_src_delay(const int* src, const int* d, size_t n, int n_nodes,
-                           std::vector<unsigned int>& perm, std::vector<long long>& row_ptr,
-                           hipStream_t stream) {
+__global__ void gather_kernel(const unsigned int* perm, size_t n, const int* tgt, const unsigned char* port,
+                              const float* w, const int* d, int* tgt_o, unsigned char* port_o, float* w_o,
+                              int* d_o) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const unsigned int i = perm[k];
+    tgt_o[k] = tgt[i];
+    port_o[k] = port[i];
+    w_o[k] = w[i];
+    d_o[k] = d[i];
+  }
+}
+
+bool device_sort_connections(int* src, int* tgt, unsigned char* port, float* w, int* d, size_t n,
+                             int n_nodes, std::vector<long long>& row_ptr, hipStream_t stream) {
   if (n == 0 || n >= (1ull << 32) || n_nodes <= 0) return false;
   int end_bit = 16;
   while (end_bit < 64 && (1ull << (end_bit - 16)) < (unsigned long long)n_nodes) ++end_bit;
-  DevMem m_src, m_d, m_k0, m_k1, m_v0, m_v1, m_cnt, m_tmp;
-  int* d_src = m_src.alloc<int>(n, "source upload buffer");
-  int* d_d = m_d.alloc<int>(n, "delay upload buffer");
-  unsigned long long* k0 = m_k0.alloc<unsigned long long>(n, "keys");
-  unsigned long long* k1 = m_k1.alloc<unsigned long long>(n, "keys");
-  unsigned int* v0 = m_v0.alloc<unsigned int>(n, "indices");
-  unsigned int* v1 = m_v1.alloc<unsigned int>(n, "indices");
-  unsigned long long* cnt = m_cnt.alloc<unsigned long long>((size_t)n_nodes, "counts");
-  check(hipMemcpyAsync(d_src, src, n * sizeof(int), hipMemcpyHostToDevice, stream), "upload");
-  check(hipMemcpyAsync(d_d, d, n * sizeof(int), hipMemcpyHostToDevice, stream), "upload");
-  check(hipMemsetAsync(cnt, 0, (size_t)n_nodes * sizeof(unsigned long long), stream), "clear");
-  hipLaunchKernelGGL(sort_keys_kernel, dim3(4096), dim3(256), 0, stream, d_src, d_d, n, k0, v0, cnt);
-  check(hipGetLastError(), "key kernel");
-  hipcub::DoubleBuffer<unsigned long long> keys(k0, k1);
-  hipcub::DoubleBuffer<unsigned int> vals(v0, v1);
-  size_t tmp_bytes = 0;
-  check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, vals, n, 0, end_bit, stream),
-        "sort size");
-  void* tmp = m_tmp.alloc<unsigned char>(tmp_bytes, "sort scratch");
-  check(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, vals, n, 0, end_bit, stream), "sort");
-  perm.resize(n);
   std::vector<unsigned long long> counts((size_t)n_nodes);
-  check(hipMemcpyAsync(perm.data(), vals.Current(), n * sizeof(unsigned int), hipMemcpyDeviceToHost, stream),
-        "download");
-  check(hipMemcpyAsync(counts.data(), cnt, (size_t)n_nodes * sizeof(unsigned long long),
-                       hipMemcpyDeviceToHost, stream),
-        "download");
-  check(hipStreamSynchronize(stream), "sync");
+  {
+    DevMem m_src, m_d, m_k0, m_k1, m_v0, m_v1, m_cnt, m_tmp, m_tgt, m_port, m_w, m_to, m_po, m_wo;
+    int* d_src = m_src.alloc<int>(n, "source upload buffer");
+    int* d_d = m_d.alloc<int>(n, "delay upload buffer");
+    unsigned long long* k0 = m_k0.alloc<unsigned long long>(n, "keys");
+    unsigned long long* k1 = m_k1.alloc<unsigned long long>(n, "keys");
+    unsigned int* v0 = m_v0.alloc<unsigned int>(n, "indices");
+    unsigned int* v1 = m_v1.alloc<unsigned int>(n, "indices");
+    unsigned long long* cnt = m_cnt.alloc<unsigned long long>((size_t)n_nodes, "counts");
+    check(hipMemcpyAsync(d_src, src, n * sizeof(int), hipMemcpyHostToDevice, stream), "upload");
+    check(hipMemcpyAsync(d_d, d, n * sizeof(int), hipMemcpyHostToDevice, stream), "upload");
+    check(hipMemsetAsync(cnt, 0, (size_t)n_nodes * sizeof(unsigned long long), stream), "clear");
+    hipLaunchKernelGGL(sort_keys_kernel, dim3(4096), dim3(256), 0, stream, d_src, d_d, n, k0, v0, cnt);
+    check(hipGetLastError(), "key kernel");
+    check(hipStreamSynchronize(stream), "sync");
+    m_src.release();   // the sources follow from the counts
+    hipcub::DoubleBuffer<unsigned long long> keys(k0, k1);
+    hipcub::DoubleBuffer<unsigned int> vals(v0, v1);
+    size_t tmp_bytes = 0;
+    check(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, vals, n, 0, end_bit, stream),
+          "sort size");
+    void* tmp = m_tmp.alloc<unsigned char>(tmp_bytes, "sort scratch");
+    check(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, vals, n, 0, end_bit, stream), "sort");
+    check(hipStreamSynchronize(stream), "sync");
+    m_tmp.release();
+    m_k0.release();
+    m_k1.release();
+    int* d_tgt = m_tgt.alloc<int>(n, "target upload buffer");
+    unsigned char* d_port = m_port.alloc<unsigned char>(n, "port upload buffer");
+    float* d_w = m_w.alloc<float>(n, "weight upload buffer");
+    check(hipMemcpyAsync(d_tgt, tgt, n * sizeof(int), hipMemcpyHostToDevice, stream), "upload");
+    check(hipMemcpyAsync(d_port, port, n, hipMemcpyHostToDevice, stream), "upload");
+    check(hipMemcpyAsync(d_w, w, n * sizeof(float), hipMemcpyHostToDevice, stream), "upload");
+    int* t_o = m_to.alloc<int>(n, "sorted targets");
+    unsigned char* p_o = m_po.alloc<unsigned char>(n, "sorted ports");
+    float* w_o = m_wo.alloc<float>(n, "sorted weights");
+    // the sorted delays reuse the index buffer that is not the sort's result
+    int* d_o = reinterpret_cast<int*>(vals.Current() == v0 ? v1 : v0);
+    hipLaunchKernelGGL(gather_kernel, dim3(4096), dim3(256), 0, stream, vals.Current(), n, d_tgt, d_port, d_w,
+                       d_d, t_o, p_o, w_o, d_o);
+    check(hipGetLastError(), "gather kernel");
+    check(hipMemcpyAsync(tgt, t_o, n * sizeof(int), hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(port, p_o, n, hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(w, w_o, n * sizeof(float), hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(d, d_o, n * sizeof(int), hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(counts.data(), cnt, (size_t)n_nodes * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, stream),
+          "download");
+    check(hipStreamSynchronize(stream), "sync");
+  }
   row_ptr.assign((size_t)n_nodes + 1, 0);
   for (int s = 0; s < n_nodes; ++s) row_ptr[s + 1] = row_ptr[s] + (long long)counts[s];
   return true;

@@ -1136,21 +1136,29 @@ void Engine::sort_connections() {
   const size_t nc = c_src_.size();
   if (sorted_ && row_ptr_.size() == (size_t)n_nodes_ + 1) return;
   if (!sort_connections_device()) sort_connections_host();
-  // delay groups
+  // delay groups: count per source, prefix, fill (sources in parallel)
   grp_ptr_.assign(n_nodes_ + 1, 0);
-  grp_start_.clear();
+#pragma omp parallel for schedule(dynamic, 1024)
   for (int s = 0; s < n_nodes_; ++s) {
-    grp_ptr_[s] = (long long)grp_start_.size();
+    long long g = 0;
     for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k)
-      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) grp_start_.push_back(k);
+      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) ++g;
+    grp_ptr_[s + 1] = g;
   }
-  grp_ptr_[n_nodes_] = (long long)grp_start_.size();
-  grp_start_.push_back((long long)nc);
+  for (int s = 0; s < n_nodes_; ++s) grp_ptr_[s + 1] += grp_ptr_[s];
+  grp_start_.assign((size_t)grp_ptr_[n_nodes_] + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int s = 0; s < n_nodes_; ++s) {
+    long long g = grp_ptr_[s];
+    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k)
+      if (k == row_ptr_[s] || c_d_[k] != c_d_[k - 1]) grp_start_[g++] = k;
+  }
+  grp_start_[grp_ptr_[n_nodes_]] = (long long)nc;
   sorted_ = true;
 }
 
-// The same order on the GPU (device_sort_src_delay: a stable radix sort of
-// (source << 16 | d_int) keys), then one gather per array on the host.  Used
+// The same order on the GPU (device_sort_connections: a stable radix sort of
+// (source << 16 | d_int) keys and the gathers on the device).  Used
 // from 2^20 connections on (NGPU_DEVICE_SORT=1 always, 0 never); false when
 // not used.
 bool Engine::sort_connections_device() {
@@ -1158,18 +1166,19 @@ bool Engine::sort_connections_device() {
   int mode = -1;
   if (const char* env = getenv("NGPU_DEVICE_SORT")) mode = atoi(env) != 0 ? 1 : 0;
   if (mode == 0 || nc == 0 || nc >= (1ull << 32) || (mode < 0 && nc < (1u << 20))) return false;
-  std::vector<unsigned int> perm;
   std::vector<long long> rp;
-  if (!device_sort_src_delay(c_src_.data(), c_d_.data(), nc, n_nodes_, perm, rp, stream_)) return false;
-  auto gather = [&](auto& v) {
-    using T = typename std::decay<decltype(v)>::type::value_type;
-    std::vector<T> o(nc);
-#pragma omp parallel for schedule(static)
-    for (long long k = 0; k < (long long)nc; ++k) o[k] = v[perm[k]];
-    v.swap(o);
-  };
-  gather(c_src_); gather(c_tgt_); gather(c_port_); gather(c_syn_); gather(c_w_); gather(c_d_);
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!device_sort_connections(c_src_.data(), c_tgt_.data(), c_port_.data(), c_w_.data(), c_d_.data(), nc,
+                               n_nodes_, rp, stream_))
+    return false;
   row_ptr_.swap(rp);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int s = 0; s < n_nodes_; ++s)
+    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k) c_src_[k] = s;
+  // c_syn_ holds synapse group 0 everywhere (no other group is accepted)
+  if (getenv("NGPU_CALIB_LOG") && atoi(getenv("NGPU_CALIB_LOG")) != 0)
+    fprintf(stderr, "calibrate:   device sort and gathers %.3f s\n",
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   return true;
 }
 
@@ -1591,7 +1600,17 @@ void Engine::calibrate_group(Group& g) {
 int Engine::Calibrate() {
   check_uncalibrated("Calibration can be made only once");
   if (!stream_) HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // NGPU_CALIB_LOG=1: the phases' wall times on stderr
+  const bool clog = getenv("NGPU_CALIB_LOG") && atoi(getenv("NGPU_CALIB_LOG")) != 0;
+  auto t_last = std::chrono::steady_clock::now();
+  auto phase = [&](const char* what) {
+    if (!clog) return;
+    const auto t = std::chrono::steady_clock::now();
+    fprintf(stderr, "calibrate: %-24s %8.3f s\n", what, std::chrono::duration<double>(t - t_last).count());
+    t_last = t;
+  };
   sort_connections();
+  phase("sort");
   const size_t nc = c_src_.size();
   // channels and port checks
   n_ch_ = 0;
@@ -1617,14 +1636,17 @@ int Engine::Calibrate() {
   if (bad_port) throw Error("Connection port index larger than the number of ports of the target");
   for (int s = 0; s < n_nodes_; ++s) has_out_[s] = row_ptr_[s + 1] > row_ptr_[s];
   n_slot_ = max_d_int_ + 2;
+  phase("channels");
   // device synapse table
   upload(d_row_ptr_, row_ptr_, stream_);
   upload(d_syn_ch_, ch, stream_);
   upload(d_syn_w_, c_w_, stream_);
   upload(d_syn_d_, dd, stream_);
   sync_streams();
+  phase("synapse table upload");
   if (const char* env = getenv("NGPU_TILED")) tiled_env_ = atoi(env);
   build_tiled(ch, dd);
+  phase("tiled table");
   ch.clear(); ch.shrink_to_fit(); dd.clear(); dd.shrink_to_fit();
   d_ring_.alloc((size_t)n_slot_ * std::max<long long>(n_ch_, 1));
   HIPCHK(hipMemsetAsync(d_ring_.p, 0, d_ring_.n * sizeof(double), stream_));
@@ -1740,6 +1762,7 @@ int Engine::Calibrate() {
     upload(g->pe_delay, dl, stream_);
     upload(g->pe_stream, st, stream_);
   }
+  phase("poisson edges");
   d_gen_chunks_.alloc(std::max(n_gen_, 1));
   d_gen_exp_.alloc(std::max(n_gen_, 1));
   d_gen_origin_.alloc(std::max(n_gen_, 1));
@@ -1788,6 +1811,7 @@ int Engine::Calibrate() {
     if (g->max_rec > 0) g->rec_times.alloc((size_t)g->n * g->max_rec);
   }
   sync_streams();
+  phase("populations");
   for (size_t k = 0; k < groups_.size(); ++k) {
     stage_var_[k].clear(); stage_var_[k].shrink_to_fit();
     stage_param_[k].clear(); stage_param_[k].shrink_to_fit();
@@ -1810,6 +1834,7 @@ int Engine::Calibrate() {
   }
   if (const char* env = getenv("NGPU_BATCH")) batch_env_ = atoi(env) != 0;
   plan_batch();
+  phase("launch plan");
   calibrated_ = true;
   neural_time_ = t_min_;
   return 0;
