@@ -2414,26 +2414,38 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   const int tc = choose_tile_ch(rows);
   if (tc == 0) return;
   const long long nt = (n_ch_ + tc - 1) / tc;
-  const long long entries = (long long)n_nodes_ * (nt + 1);
+  const long long entries = (long long)n_nodes_ * nt;
   if (entries >= (1ll << 28)) return;
-  std::vector<unsigned int> off((size_t)entries);
-  std::vector<TileRec> rec(nc);
+  // runs packed source by source, tile by tile (aligning each run to a
+  // whole line was measured slower)
+  std::vector<unsigned int> cnt((size_t)entries, 0u);
+  std::vector<long long> base((size_t)n_nodes_ + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int s = 0; s < n_nodes_; ++s) {
+    unsigned int* c = cnt.data() + (size_t)s * nt;
+    for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k) c[ch[k] / tc]++;
+    long long sz = 0;
+    for (long long t = 0; t < nt; ++t) sz += c[t];
+    base[s + 1] = sz;
+  }
+  for (int s = 0; s < n_nodes_; ++s) base[s + 1] += base[s];
+  if (base[n_nodes_] >= (1ll << 32)) return;
+  std::vector<uint2> runs((size_t)entries);
+  std::vector<TileRec> rec((size_t)base[n_nodes_]);
 #pragma omp parallel
   {
-    std::vector<unsigned int> pos(nt + 1);
+    std::vector<unsigned int> pos(nt);
 #pragma omp for schedule(dynamic, 64)
     for (int s = 0; s < n_nodes_; ++s) {
-      const long long b = row_ptr_[s], e = row_ptr_[s + 1];
-      std::fill(pos.begin(), pos.end(), 0u);
-      for (long long k = b; k < e; ++k) pos[ch[k] / tc + 1]++;
-      unsigned int* o = off.data() + (size_t)s * (nt + 1);
-      unsigned int run = (unsigned int)b;
-      for (long long t = 0; t <= nt; ++t) {
-        run += (t > 0) ? pos[t] : 0u;
-        o[t] = run;
-        pos[t] = run;   // next free position of tile t
+      const unsigned int* c = cnt.data() + (size_t)s * nt;
+      uint2* o = runs.data() + (size_t)s * nt;
+      long long at = base[s];
+      for (long long t = 0; t < nt; ++t) {
+        o[t] = make_uint2((unsigned int)at, (unsigned int)(at + c[t]));
+        pos[t] = (unsigned int)at;   // next free position of tile t
+        at += c[t];
       }
-      for (long long k = b; k < e; ++k) {
+      for (long long k = row_ptr_[s]; k < row_ptr_[s + 1]; ++k) {
         const unsigned int t = ch[k] / tc;
         TileRec& r = rec[pos[t]++];
         r.ch = (unsigned short)(ch[k] - t * (unsigned int)tc);
@@ -2443,7 +2455,8 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
       }
     }
   }
-  upload(d_toff_, off, stream_);
+  cnt.clear(); cnt.shrink_to_fit();
+  upload(d_toff_, runs, stream_);
   upload(d_trec_, rec, stream_);
   // first step's plan: one item per tile; then the kernel plans each next step
   std::vector<uint2> plan((size_t)nt * kTileSplitMax, make_uint2(0u, 0u));

@@ -347,7 +347,7 @@ class Engine {
   bool tiled_use_ = false;             // deliver with it (a batch autotuner knob)
   int tile_ch_ = 0, n_tiles_ = 0, tile_rows_ = 0;
   int choose_tile_ch(int rows) const;
-  DBuf<unsigned int> d_toff_;
+  DBuf<uint2> d_toff_;                // (first, end) record of each (source, tile)
   DBuf<TileRec> d_trec_;
   DBuf<unsigned int> d_tev_;          // events per tile of the last step
   DBuf<uint2> d_tplan_;               // work items of the next step

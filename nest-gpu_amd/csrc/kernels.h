@@ -207,8 +207,8 @@ struct ParrotArgs {
 // Copy of the synapse table for dense delivery (deliver_tiled_kernel): input
 // channels are cut into tiles of tile_ch consecutive channels; within each
 // source's segment the synapses are grouped by tile, each as one 8-byte
-// record (channel within the tile, delay slot, weight), and off holds the
-// first record of every (source, tile) pair.  A spike of step t reaches
+// record (channel within the tile, delay slot, weight), and runs holds the
+// (first, end) record of every (source, tile).  A spike of step t reaches
 // ring slot t + r, r = d_int + 1 - lag (lag 1 for a spike emitted by a split
 // step one step late); the LDS tile holds the rows r < rows (the delays of
 // nearly all synapses), the rest go to the ring with FP64 global atomics.
@@ -219,7 +219,7 @@ struct TileRec {
   float w;
 };
 struct TiledSyn {
-  const unsigned int* off;     // [n_nodes][n_tiles + 1]
+  const uint2* runs;           // [n_nodes][n_tiles]
   const TileRec* rec;          // [n_syn]
   int n_tiles, tile_ch;
   int rows;                    // relative slots r kept in LDS: 0 .. rows - 1

@@ -235,7 +235,8 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 // into one channel tile.  Persistent blocks take items from the list with an
 // atomic counter.  For an item, the block's waves take the list's spikes 64
 // at a time: lane l reads the (first, end) run of (spike l, tile) from its
-// source's offset row, a wave prefix sum turns the 64 runs into one flat
+// source's row of runs (one 8-byte load), a wave prefix sum turns the 64
+// runs into one flat
 // range of records, and the wave walks that range 64 records per pass, each
 // lane finding its spike by a 6-step binary search over the prefix sums
 // (lane shuffles) -- every lane busy, and one pass reads a few whole runs
@@ -336,7 +337,7 @@ deliver_tiled_kernel(TiledDeliverArgs a) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_wv = blockDim.x / kWave;
   const int n_items = a.sched[0];
-  const size_t row = (size_t)a.ts.n_tiles + 1;
+  const size_t row = (size_t)a.ts.n_tiles;
   // tiles can outnumber the counter rows: the waves add their counts into
   // the first kCtrMinRows rows with atomics
   unsigned long long* ctr_row =
@@ -354,13 +355,13 @@ deliver_tiled_kernel(TiledDeliverArgs a) {
     const int kk = __builtin_amdgcn_readfirstlane((int)(it.y & 0xffffu));
     const int nk = __builtin_amdgcn_readfirstlane((int)(it.y >> 16));
     const long long c0 = (long long)tile * tc;
-    const unsigned int* off = a.ts.off + tile;
+    const uint2* runs = a.ts.runs + tile;
     for (int k = threadIdx.x; k < n_acc; k += blockDim.x) acc[k] = 0.0;
     __syncthreads();
     unsigned int item_ev = 0;
     // the item's groups kk, kk + nk, ...; wave wv takes every n_wv-th of
     // them.  Every load is issued by every lane (a lane past the list's end
-    // reads entry 0 and source 0's offsets, and masks them), so the compiler
+    // reads entry 0 and source 0's run, and masks them), so the compiler
     // counts the loads in flight and waits only for the ones it uses.
     const int gs = nk * n_wv * kWave;
     for (int s = (kk + nk * wv) * kWave + lane; s - lane < n_spk; s += gs) {
@@ -369,9 +370,9 @@ deliver_tiled_kernel(TiledDeliverArgs a) {
       const unsigned int raw = (unsigned int)srcs[valid ? s : 0];
       const unsigned int r0 = valid ? raw : 0u;
       const float hgt = hts[valid ? s : 0];
-      const unsigned int* o = off + (size_t)(r0 & ~kSpikeLagBit) * row;
-      const unsigned int beg = o[0];
-      const unsigned int cnt = valid ? o[1] - beg : 0u;
+      const uint2 run = runs[(size_t)(r0 & ~kSpikeLagBit) * row];
+      const unsigned int beg = run.x;
+      const unsigned int cnt = valid ? run.y - run.x : 0u;
       // bit 0: lagged spike (r = d_int), bit 1: its d_int = 0 synapses are
       // delivered already
       const unsigned int meta = (r0 >> 31) | ((a.skip_lagged_d0 && (r0 >> 31)) ? 2u : 0u);
