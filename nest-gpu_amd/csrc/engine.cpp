@@ -2462,10 +2462,9 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
   std::vector<uint2> plan((size_t)nt * kTileSplitMax, make_uint2(0u, 0u));
   for (long long t = 0; t < nt; ++t) plan[t] = make_uint2((unsigned int)t, 1u << 16);
   const std::vector<unsigned int> tev((size_t)nt, 0u);
-  const std::vector<int> sched{(int)nt, 0, 0, 0};
+  std::vector<int> sched{(int)nt, 0, 0, 0};   // items, next item (set below), finished blocks
   upload(d_tplan_, plan, stream_);
   upload(d_tev_, tev, stream_);
-  upload(d_tsched_, sched, stream_);
   {
     // persistent grid: every block slot of the chip (tile LDS per CU, at
     // most 2 blocks of 16 waves)
@@ -2476,7 +2475,9 @@ void Engine::build_tiled(const std::vector<unsigned int>& ch, const std::vector<
     const int tile_bytes = rows * tc * (int)sizeof(double) + kTileStaticLds;
     const int per_cu = std::max(1, std::min(2, lds_cu / tile_bytes));
     tiled_blocks_ = cus * per_cu;
+    sched[1] = tiled_blocks_;   // items below the grid size are taken by block index
   }
+  upload(d_tsched_, sched, stream_);
   sync_streams();
   tile_ch_ = tc;
   tile_rows_ = rows;

@@ -232,8 +232,8 @@ void launch_deliver(const DeliverArgs& a, hipStream_t s) {
 
 // Dense delivery (TiledDeliverArgs).  The work of a step is a list of items
 // (tile, k, K): the spike groups k, k + K, k + 2K, ... of the step's list
-// into one channel tile.  Persistent blocks take items from the list with an
-// atomic counter.  For an item, the block's waves take the list's spikes 64
+// into one channel tile.  Persistent blocks take items from the list: block
+// b first item b, then the next free one from an atomic counter.  For an item, the block's waves take the list's spikes 64
 // at a time: lane l reads the (first, end) run of (spike l, tile) from its
 // source's row of runs (one 8-byte load), a wave prefix sum turns the 64
 // runs into one flat
@@ -313,7 +313,7 @@ __device__ inline void tiled_plan(const TiledDeliverArgs& a) {
     a.tile_ev[t] = 0;
   }
   if (threadIdx.x == 0) {
-    a.sched[1] = 0;
+    a.sched[1] = (int)gridDim.x;
     a.sched[2] = 0;
   }
 }
@@ -342,9 +342,11 @@ deliver_tiled_kernel(TiledDeliverArgs a) {
   // the first kCtrMinRows rows with atomics
   unsigned long long* ctr_row =
       a.ctr ? a.ctr + (size_t)(((long long)blockIdx.x * n_wv + wv) % kCtrMinRows) * kCtrCols + 3 : nullptr;
-  for (;;) {
+  // block b's first item is item b; the counter hands out the rest (it
+  // starts at the grid size)
+  for (bool first = true;; first = false) {
     if (threadIdx.x == 0) {
-      s_item = atomicAdd(&a.sched[1], 1);
+      s_item = first ? (int)blockIdx.x : atomicAdd(&a.sched[1], 1);
       s_ev = 0;
     }
     __syncthreads();
