@@ -2037,7 +2037,7 @@ void Engine::launch_step() {
       DeliverArgs da{};
       da.ctx = sc; da.sl = sl; da.st = syn_table();
       da.ctr = d_ctr_.p;
-      timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
+      if (d_syn_w_.n > 0) timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
     }
     prev_list_pending_ = true;
   }
@@ -2249,7 +2249,7 @@ void Engine::batch_step(const StepCtx& sc, const SpikeList& sl) {
       DeliverArgs da{};
       da.ctx = sc; da.sl = sl; da.st = syn_table();
       da.ctr = d_ctr_.p;
-      timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
+      if (d_syn_w_.n > 0) timed(KT_OTHER, [&] { launch_deliver_lagged_d0(da, stream_); });
     }
     prev_list_pending_ = fu;
     batch_fold_pending_ = true;
