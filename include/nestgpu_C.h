@@ -160,7 +160,11 @@ int NESTGPU_GetConnectionStatus(int i_source, int i_group, int i_conn, int *i_ta
                                 unsigned char *port, unsigned char *syn_group,
                                 float *delay, float *weight);
 
-/* ---- synapse groups (src/nestgpu_C.h:292-306): outside this build ---- */
+/* ---- synapse groups (src/nestgpu_C.h:292-306): models "stdp" and
+ * "test_syn_model" (src/syn_model.cu:136-238); group 0 is the static synapse.
+ * Plastic synapses are delivered at their arrival step; a network with any
+ * runs without split RK5 steps, fused delivery or the batch launch, and a
+ * dendritic delay on a target of plastic synapses is an error. ---- */
 int NESTGPU_CreateSynGroup(char *model_name);
 int NESTGPU_GetSynGroupNParam(int i_syn_group);
 char **NESTGPU_GetSynGroupParamNames(int i_syn_group);

@@ -139,6 +139,10 @@ __device__ inline void add_wave_counters(unsigned long long* ctr, int slot, int 
 // 1: only d_int = 0, 2: only d_int >= 1.  Returns the synapses delivered.
 constexpr int kDeliverUnroll = 4;
 
+// kPlastic: the table may hold plastic synapses (t.grp set), queued for
+// their arrival step instead of added into the ring (the delivery launch
+// only; fused delivery runs for networks without them).
+template <bool kPlastic = false>
 __device__ inline unsigned long long deliver_segment(const SynTable& t, int base_slot, float hgt,
                                                       long long b, long long e, int tid,
                                                       int nthr, int mode) {
@@ -148,13 +152,16 @@ __device__ inline unsigned long long deliver_segment(const SynTable& t, int base
     unsigned int ch[kDeliverUnroll];
     float w[kDeliverUnroll];
     int d[kDeliverUnroll];
+    int g[kDeliverUnroll];
 #pragma unroll
     for (int u = 0; u < kDeliverUnroll; ++u) {
       const long long k = k0 + (long long)u * nthr;
+      g[u] = 0;
       if (k < e) {
         ch[u] = t.syn_ch[k];
         w[u] = t.syn_w[k];
         d[u] = t.syn_d[k];
+        if (kPlastic) g[u] = t.grp[k];
       } else {
         d[u] = -1;
       }
@@ -164,7 +171,17 @@ __device__ inline unsigned long long deliver_segment(const SynTable& t, int base
       if (d[u] < 0 || (mode == 1 && d[u] != 0) || (mode == 2 && d[u] == 0)) continue;
       int slot = base_slot + d[u];
       if (slot >= t.n_slot) slot -= t.n_slot;
-      unsafeAtomicAdd(&t.ring[(size_t)slot * t.n_ch + ch[u]], (double)(hgt * w[u]));
+      if (kPlastic && g[u]) {
+        // a plastic synapse: its weight is read when the spike arrives
+        const int pos = atomicAdd(&t.q_n[slot], 1);
+        if (pos < t.q_cap)
+          t.q[(size_t)slot * t.q_cap + pos] =
+              make_uint2((unsigned int)(k0 + (long long)u * nthr), __float_as_uint(hgt));
+        else
+          atomicAdd(t.q_over, 1);
+      } else {
+        unsafeAtomicAdd(&t.ring[(size_t)slot * t.n_ch + ch[u]], (double)(hgt * w[u]));
+      }
       ++n_ev;
     }
     // the d_int = 0 group is the segment's head: past it, nothing more

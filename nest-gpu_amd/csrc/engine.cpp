@@ -962,14 +962,83 @@ float* Engine::RandomNormalClipped(size_t n, float mean, float sd, float vmin, f
   return a;
 }
 
+// ---- synapse groups, src/syn_model.cu:136-238 -------------------------------
+// parameter names and defaults: src/stdp.cu:64-79, src/test_syn_model.cu:43-52
+static const std::vector<std::string> kStdpParams = {"tau_plus", "tau_minus", "lambda", "alpha",
+                                                     "mu_plus", "mu_minus", "Wmax"};
+static const std::vector<std::string> kTestSynParams = {"fact", "offset"};
+
+static const std::vector<std::string>& syn_param_names(int model) {
+  return model == kSynModelStdp ? kStdpParams : kTestSynParams;
+}
+
+int Engine::CreateSynGroup(const std::string& model) {
+  check_uncalibrated("Nodes cannot be created after calibration");
+  SynGroupS g;
+  if (model == "stdp") {
+    g.model = kSynModelStdp;
+    g.p = {20.0f, 20.0f, 1.0e-4f, 1.0f, 1.0f, 1.0f, 100.0f};
+  } else if (model == "test_syn_model") {
+    g.model = kSynModelTest;
+    g.p = {0.1f, 0.0f};
+  } else {
+    throw Error("Unknown synapse model name: " + model);
+  }
+  if (syn_groups_.size() >= 256) throw Error("Too many synapse groups");
+  syn_groups_.push_back(g);
+  return (int)syn_groups_.size() - 1;
+}
+
+const Engine::SynGroupS& Engine::syn_group_at(int g) const {
+  if (g < 1 || g >= (int)syn_groups_.size()) throw Error("Unrecognized synapse group");
+  return syn_groups_[g];
+}
+
+int Engine::GetSynGroupNParam(int g) { return (int)syn_group_at(g).p.size(); }
+
+std::vector<std::string> Engine::GetSynGroupParamNames(int g) {
+  return syn_param_names(syn_group_at(g).model);
+}
+
+bool Engine::IsSynGroupParam(int g, const std::string& name) {
+  const auto& v = syn_param_names(syn_group_at(g).model);
+  return std::find(v.begin(), v.end(), name) != v.end();
+}
+
+int Engine::GetSynGroupParamIdx(int g, const std::string& name) {
+  const auto& v = syn_param_names(syn_group_at(g).model);
+  auto it = std::find(v.begin(), v.end(), name);
+  if (it == v.end()) throw Error("Unrecognized parameter " + name);
+  return (int)(it - v.begin());
+}
+
+float Engine::GetSynGroupParam(int g, const std::string& name) {
+  if (!IsSynGroupParam(g, name)) throw Error("Unrecognized synapse parameter " + name);
+  return syn_groups_[g].p[GetSynGroupParamIdx(g, name)];
+}
+
+void Engine::SetSynGroupParam(int g, const std::string& name, float v) {
+  if (!IsSynGroupParam(g, name)) throw Error("Unrecognized synapse parameter " + name);
+  syn_groups_[g].p[GetSynGroupParamIdx(g, name)] = v;
+  if (calibrated_ && plastic_) {
+    std::vector<float> tab(syn_groups_.size() * kSynGroupParams, 0.0f);
+    for (size_t i = 1; i < syn_groups_.size(); ++i) {
+      tab[i * kSynGroupParams] = (float)syn_groups_[i].model;
+      for (size_t j = 0; j < syn_groups_[i].p.size(); ++j) tab[i * kSynGroupParams + 1 + j] = syn_groups_[i].p[j];
+    }
+    upload(d_sg_param_, tab, stream_);
+    sync_streams();
+  }
+}
+
 // ---- connections ------------------------------------------------------------
 int Engine::Connect(int src, int tgt, int port, int syn_group, float w, float delay) {
   check_uncalibrated("Connections cannot be created after calibration");
   if (src < 0 || src >= n_nodes_ || tgt < 0 || tgt >= n_nodes_)
     throw Error("Unrecognized node in connection");
   if (delay < dt_) throw Error("Delay must be >= time resolution");
-  if (syn_group != 0)
-    throw Error("Synapse groups (STDP) are outside the scope of this build");
+  if (syn_group < 0 || syn_group >= (int)syn_groups_.size())
+    throw Error("Unrecognized synapse group");
   if (port < 0 || port > 255) throw Error("Port index must be >=0 and < 256");
   int d_int = (int)std::round(delay / dt_) - 1;   // src/connect.cu:45
   if (d_int > 65535) throw Error("Delay too large (max 65536 steps)");
@@ -986,7 +1055,8 @@ int Engine::Connect(int src, int tgt, int port, int syn_group, float w, float de
 int Engine::ConnectRule(const int* src_list, int i_src0, int n_src, const int* tgt_list,
                         int i_tgt0, int n_tgt, const ConnSpecS& cs, const SynSpecS& ss) {
   check_uncalibrated("Connections cannot be created after calibration");
-  if (ss.syn_group != 0) throw Error("Synapse groups (STDP) are outside the scope of this build");
+  if (ss.syn_group < 0 || ss.syn_group >= (int)syn_groups_.size())
+    throw Error("Unrecognized synapse group");
   auto S = [&](int k) { return src_list ? src_list[k] : i_src0 + k; };
   auto T = [&](int k) { return tgt_list ? tgt_list[k] : i_tgt0 + k; };
   for (int k = 0; k < n_src; ++k)
@@ -1235,7 +1305,8 @@ void Engine::sort_connections_host() {
 }
 
 std::vector<long long> Engine::GetConnections(const int* src_list, int i_src0, int n_src,
-                                              const int* tgt_list, int i_tgt0, int n_tgt) {
+                                              const int* tgt_list, int i_tgt0, int n_tgt,
+                                              int syn_group) {
   sort_connections();
   if (!src_list && n_src <= 0) { i_src0 = 0; n_src = n_nodes_; }
   if (!tgt_list && n_tgt <= 0) { i_tgt0 = 0; n_tgt = n_nodes_; }
@@ -1250,6 +1321,7 @@ std::vector<long long> Engine::GetConnections(const int* src_list, int i_src0, i
         int t = c_tgt_[c];
         bool hit = tgt_list ? std::binary_search(tv.begin(), tv.end(), t)
                             : (t >= i_tgt0 && t < i_tgt0 + n_tgt);
+        if (syn_group >= 0 && c_syn_[c] != syn_group) hit = false;
         if (hit) {
           out.push_back(s);
           out.push_back(gi - grp_ptr_[s]);
@@ -1274,6 +1346,8 @@ void Engine::GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned 
   *syn = c_syn_[c];
   *delay = dt_ * (c_d_[c] + 1);   // src/connect.cu:168
   *w = c_w_[c];
+  if (plastic_ && calibrated_ && c_syn_[c] != 0)   // its weight changes on the device
+    copy_on(stream_, w, d_syn_w_.p + c, sizeof(float), hipMemcpyDeviceToHost);
 }
 
 void Engine::OutDegrees(int i_node, int n, long long* out) {
@@ -1645,7 +1719,12 @@ int Engine::Calibrate() {
   sync_streams();
   phase("synapse table upload");
   if (const char* env = getenv("NGPU_TILED")) tiled_env_ = atoi(env);
+  plastic_ = false;
+  for (size_t k = 0; k < nc && !plastic_; ++k) plastic_ = c_syn_[k] != 0;
+  if (plastic_) tiled_env_ = 0;   // the tiled copy holds the weights of one moment
   build_tiled(ch, dd);
+  // (before the populations' host staging is released: it reads den_delay)
+  if (plastic_) build_plastic();
   phase("tiled table");
   ch.clear(); ch.shrink_to_fit(); dd.clear(); dd.shrink_to_fit();
   d_ring_.alloc((size_t)n_slot_ * std::max<long long>(n_ch_, 1));
@@ -1680,6 +1759,13 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
     rk5_defer_env_ = true;
+  }
+  // plastic synapses need every spike in its own step (a post-synaptic spike
+  // time is its step) and the delivery launch (which queues them for their
+  // arrival): no split steps, no fused delivery, no batch launch
+  if (plastic_) {
+    rk5_defer_cap_ = 0;
+    fused_deliver_env_ = false;
   }
   // per-wave counter rows: as many as the widest launch that adds to them
   long long rows = kCtrMinRows;
@@ -1833,6 +1919,7 @@ int Engine::Calibrate() {
     prev_list_pending_ = false;
   }
   if (const char* env = getenv("NGPU_BATCH")) batch_env_ = atoi(env) != 0;
+  if (plastic_) batch_env_ = false;
   plan_batch();
   phase("launch plan");
   calibrated_ = true;
@@ -2029,6 +2116,7 @@ void Engine::launch_step() {
   // (or a step driven by SimulationStep), which leaves nothing pending
   if (!fused() || !defer_ok_) {
     deliver_list(sc, sl, fused() && lag_d0_mode_ != 0 ? 1 : 0);
+    if (plastic_) plastic_step(sc, sl);
     prev_list_pending_ = false;
   } else {
     if (lag_d0_mode_ == 0) {
@@ -2508,6 +2596,80 @@ void Engine::deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagge
   timed(KT_DELIVER, [&] { launch_deliver(da, stream_); });
 }
 
+// Device state of plastic synapses (synapse group >= 1): the group of every
+// synapse, the node of every input channel, the arrival step of each
+// synapse's last spike and each node's last spike step (both "long ago" at
+// first, as the reference's 0x8000 marks, src/rev_spike.cu:171-173), the
+// incoming plastic synapses of every node (src/spike_buffer.cu:399-460
+// builds the reference's TargetRevConnection), the groups' rules, and one
+// arrival queue per ring slot.  A dendritic delay other than 0 on a node
+// with incoming plastic synapses is not supported (error).
+void Engine::build_plastic() {
+  const size_t nc = c_src_.size();
+  std::vector<unsigned char> grp(c_syn_.begin(), c_syn_.end());
+  std::vector<long long> rptr((size_t)n_nodes_ + 1, 0);
+  size_t n_pl = 0;
+  for (size_t k = 0; k < nc; ++k)
+    if (grp[k]) { rptr[c_tgt_[k] + 1]++; ++n_pl; }
+  for (int i = 0; i < n_nodes_; ++i) rptr[i + 1] += rptr[i];
+  std::vector<unsigned int> ridx(std::max<size_t>(n_pl, 1));
+  {
+    std::vector<long long> pos(rptr.begin(), rptr.end() - 1);
+    for (size_t k = 0; k < nc; ++k)
+      if (grp[k]) ridx[pos[c_tgt_[k]]++] = (unsigned int)k;
+  }
+  for (auto* g : groups_) {
+    const int j = param_idx(*g, "den_delay", 0);
+    if (j < 0) continue;
+    const int gi = group_index(groups_, g);
+    for (int i = 0; i < g->n; ++i)
+      if (rptr[g->i0 + i + 1] > rptr[g->i0 + i] && stage_param_[gi][(size_t)j * g->n + i] != 0.0f)
+        throw Error("A dendritic delay on a target of plastic synapses is not supported");
+  }
+  std::vector<int> ch_node((size_t)std::max<long long>(n_ch_, 1), 0);
+  for (auto* g : groups_)
+    for (int q = 0; q < g->n_port; ++q)
+      for (int i = 0; i < g->n; ++i) ch_node[(size_t)(g->ch_base + (long long)q * g->n + i)] = g->i0 + i;
+  std::vector<float> tab(syn_groups_.size() * kSynGroupParams, 0.0f);
+  for (size_t i = 1; i < syn_groups_.size(); ++i) {
+    tab[i * kSynGroupParams] = (float)syn_groups_[i].model;
+    for (size_t j = 0; j < syn_groups_[i].p.size(); ++j) tab[i * kSynGroupParams + 1 + j] = syn_groups_[i].p[j];
+  }
+  // queue per slot: every plastic synapse once (a source fires at most once
+  // per step, so a slot receives each synapse at most once)
+  pq_cap_ = (int)std::min<size_t>(std::max<size_t>(n_pl, 1024), (size_t)1 << 30);
+  if ((size_t)pq_cap_ * n_slot_ > ((size_t)1 << 28)) throw Error("Too many plastic synapses for the arrival queues");
+  const std::vector<int> far_pre(nc, -(1 << 30)), far_post(n_nodes_, -(1 << 30));
+  const std::vector<int> zeros((size_t)n_slot_ + 1, 0), zero1(1, 0);
+  upload(d_syn_grp_, grp, stream_);
+  upload(d_rev_ptr_, rptr, stream_);
+  upload(d_rev_idx_, ridx, stream_);
+  upload(d_ch_node_, ch_node, stream_);
+  upload(d_sg_param_, tab, stream_);
+  upload(d_last_pre_, far_pre, stream_);
+  upload(d_last_post_, far_post, stream_);
+  upload(d_pq_n_, zeros, stream_);
+  upload(d_pq_over_, zero1, stream_);
+  d_pq_.alloc((size_t)pq_cap_ * n_slot_);
+  sync_streams();
+}
+
+void Engine::plastic_step(const StepCtx& sc, const SpikeList& sl) {
+  StdpArgs a{};
+  a.st = syn_table();
+  a.w = d_syn_w_.p;
+  a.ch_node = d_ch_node_.p;
+  a.last_pre = d_last_pre_.p;
+  a.last_post = d_last_post_.p;
+  a.rev_ptr = d_rev_ptr_.p;
+  a.rev_idx = d_rev_idx_.p;
+  a.grp_param = d_sg_param_.p;
+  a.ctx = sc;
+  a.sl = sl;
+  a.dt = (float)dt_;
+  timed(KT_DELIVER, [&] { launch_stdp_step(a, stream_); });
+}
+
 SynTable Engine::syn_table() const {
   SynTable t;
   t.row_ptr = d_row_ptr_.p;
@@ -2517,6 +2679,11 @@ SynTable Engine::syn_table() const {
   t.ring = d_ring_.p;
   t.n_ch = n_ch_;
   t.n_slot = n_slot_;
+  t.grp = plastic_ ? d_syn_grp_.p : nullptr;
+  t.q = d_pq_.p;
+  t.q_n = d_pq_n_.p;
+  t.q_cap = pq_cap_;
+  t.q_over = d_pq_over_.p;
   return t;
 }
 

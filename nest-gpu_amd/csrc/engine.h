@@ -195,10 +195,20 @@ class Engine {
 
   // ---- connections (src/connect.cu:38-74, src/connect_rules.h:54-338)
   int Connect(int src, int tgt, int port, int syn_group, float w, float delay);
+  // synapse groups, src/syn_model.cu:136-238 (group 0: static synapses)
+  int CreateSynGroup(const std::string& model);
+  int GetSynGroupNParam(int g);
+  std::vector<std::string> GetSynGroupParamNames(int g);
+  bool IsSynGroupParam(int g, const std::string& name);
+  int GetSynGroupParamIdx(int g, const std::string& name);
+  float GetSynGroupParam(int g, const std::string& name);
+  void SetSynGroupParam(int g, const std::string& name, float v);
   int ConnectRule(const int* src_list, int i_src0, int n_src, const int* tgt_list,
                   int i_tgt0, int n_tgt, const ConnSpecS& cs, const SynSpecS& ss);
+  // syn_group < 0: every synapse group
   std::vector<long long> GetConnections(const int* src_list, int i_src0, int n_src,
-                                        const int* tgt_list, int i_tgt0, int n_tgt);
+                                        const int* tgt_list, int i_tgt0, int n_tgt,
+                                        int syn_group = -1);
   void GetConnectionStatus(int src, int grp, int conn, int* tgt, unsigned char* port,
                            unsigned char* syn, float* delay, float* w);
   long long NConnections() const { return (long long)c_src_.size(); }
@@ -276,6 +286,21 @@ class Engine {
   void build_tiled(const std::vector<unsigned int>& ch, const std::vector<unsigned short>& dd);
   void deliver_list(const StepCtx& sc, const SpikeList& sl, int skip_lagged_d0);
   bool fused() const { return fused_deliver_ && !tiled_use_; }
+  // plastic synapses: groups (model id + parameters, index 0 unused), and
+  // the device state once a network has any (build_plastic)
+  struct SynGroupS { int model; std::vector<float> p; };
+  std::vector<SynGroupS> syn_groups_ = std::vector<SynGroupS>(1);
+  const SynGroupS& syn_group_at(int g) const;
+  bool plastic_ = false;
+  DBuf<unsigned char> d_syn_grp_;
+  DBuf<int> d_ch_node_, d_last_pre_, d_last_post_, d_pq_n_, d_pq_over_;
+  DBuf<long long> d_rev_ptr_;
+  DBuf<unsigned int> d_rev_idx_;
+  DBuf<float> d_sg_param_;
+  DBuf<uint2> d_pq_;
+  int pq_cap_ = 0;
+  void build_plastic();
+  void plastic_step(const StepCtx& sc, const SpikeList& sl);
   FoldArgs make_fold_args(const Group& g);
   SynTable syn_table() const;
   void fetch_records(long long n_rows);

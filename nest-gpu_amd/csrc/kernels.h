@@ -89,7 +89,43 @@ struct SynTable {
   double* ring;                // [n_slot][n_ch]
   long long n_ch;
   int n_slot;
+  // plastic synapses (synapse group >= 1; grp null when the network has
+  // none): delivered at their arrival step, through a queue per ring slot
+  const unsigned char* grp;    // [n_syn] synapse group, 0 = static
+  uint2* q;                    // [n_slot][q_cap] (synapse index, height bits)
+  int* q_n;                    // [n_slot] entries queued per slot
+  int q_cap;
+  int* q_over;                 // entries that found their slot's queue full
 };
+
+// Synapse groups' plasticity rules, src/syn_model.cu:41-56: "stdp"
+// (src/stdp.cu:36-61) and "test_syn_model" (src/test_syn_model.cu:36-41).
+constexpr int kSynModelTest = 1;
+constexpr int kSynModelStdp = 2;
+constexpr int kSynGroupParams = 8;   // model id, then up to 7 parameters
+constexpr int kMaxSynDt = 16384;     // src/syn_model.h:33
+
+// One step of plastic synapses (after the step's delivery, before the fold
+// of its slot): stdp_arrive_kernel takes the slot's queue -- each event adds
+// height * weight into the ring, records its arrival step, and depresses
+// the weight if the target fired earlier (src/get_spike.cu:80-95) --, then
+// stdp_post_kernel potentiates the incoming plastic synapses of every
+// neuron that fired this step (src/rev_spike.cu:40-85) and records its
+// spike time.
+struct StdpArgs {
+  SynTable st;
+  float* w;                    // = st.syn_w, updated in place
+  const int* ch_node;          // [n_ch] target node of each input channel
+  int* last_pre;               // [n_syn] arrival step of the last spike
+  int* last_post;              // [n_nodes] step of the node's last spike
+  const long long* rev_ptr;    // [n_nodes+1] incoming plastic synapses per node
+  const unsigned int* rev_idx; // their synapse indices
+  const float* grp_param;      // [n_groups][kSynGroupParams]
+  StepCtx ctx;
+  SpikeList sl;
+  float dt;
+};
+void launch_stdp_step(const StdpArgs& a, hipStream_t s);
 
 // Delivery done by the persistent RK5 kernel (one spike-emitting population)
 // or the batch kernel: its idle waves deliver the previous step's spike list

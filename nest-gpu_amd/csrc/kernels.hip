@@ -211,9 +211,13 @@ __global__ void __launch_bounds__(256) deliver_kernel(DeliverArgs a) {
     const unsigned int raw = (unsigned int)srcs[s];
     const int src = (int)(raw & ~kSpikeLagBit);
     const int mode = (a.skip_lagged_d0 && (raw >> 31)) ? 2 : 0;
-    n_ev += deliver_segment(a.st, spike_base_slot(raw, a.ctx.t_idx, a.st.n_slot), hts[s],
-                            a.st.row_ptr[src], a.st.row_ptr[src + 1], threadIdx.x, blockDim.x,
-                            mode);
+    const int base = spike_base_slot(raw, a.ctx.t_idx, a.st.n_slot);
+    if (a.st.grp)
+      n_ev += deliver_segment<true>(a.st, base, hts[s], a.st.row_ptr[src], a.st.row_ptr[src + 1],
+                                    threadIdx.x, blockDim.x, mode);
+    else
+      n_ev += deliver_segment(a.st, base, hts[s], a.st.row_ptr[src], a.st.row_ptr[src + 1],
+                              threadIdx.x, blockDim.x, mode);
   }
   n_ev = wave_sum(n_ev);
   if ((threadIdx.x & (kWave - 1)) == 0 && a.ctr && n_ev)
@@ -578,6 +582,78 @@ __global__ void __launch_bounds__(256) deliver_lagged_d0_kernel(DeliverArgs a) {
 
 void launch_deliver_lagged_d0(const DeliverArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(deliver_lagged_d0_kernel, dim3(kLagD0Blocks), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Plastic synapses (StdpArgs).  The weight rules are the reference's,
+// evaluated in double as there (src/stdp.cu:36-61, src/test_syn_model.cu:
+// 36-41); p[0] is the group's model, p[1..] its parameters in the
+// reference's order.
+__device__ inline float syn_rule(float w, float dt_ms, const float* p) {
+  if ((int)p[0] == kSynModelTest) return w + (p[2] + p[1] * dt_ms);
+  const double tau_plus = p[1], tau_minus = p[2], lambda = p[3], alpha = p[4];
+  const double mu_plus = p[5], mu_minus = p[6], w_max = p[7];
+  const double wd = w;
+  double w1;
+  if (dt_ms >= 0.0f) {
+    const double fact = lambda * exp(-(double)dt_ms / tau_plus);
+    w1 = wd + fact * w_max * pow(1.0 - wd / w_max, mu_plus);
+  } else {
+    const double fact = -alpha * lambda * exp((double)dt_ms / tau_minus);
+    w1 = wd + fact * w_max * pow(wd / w_max, mu_minus);
+  }
+  w1 = w1 > 0.0 ? w1 : 0.0;
+  w1 = w1 < w_max ? w1 : w_max;
+  return (float)w1;
+}
+
+// the slot's arrivals, one thread per queued event (distinct synapses: a
+// source fires at most once per step)
+__global__ void stdp_arrive_kernel(StdpArgs a) {
+  const int t = (int)a.ctx.t_idx;
+  const int slot = (int)(a.ctx.t_idx % a.st.n_slot);
+  int n = a.st.q_n[slot];
+  if (n > a.st.q_cap) n = a.st.q_cap;
+  const uint2* q = a.st.q + (size_t)slot * a.st.q_cap;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint2 e = q[i];
+    const unsigned int k = e.x;
+    const float h = __uint_as_float(e.y);
+    const unsigned int ch = a.st.syn_ch[k];
+    const float w = a.w[k];
+    unsafeAtomicAdd(&a.st.ring[(size_t)slot * a.st.n_ch + ch], (double)(h * w));
+    a.last_pre[k] = t;
+    const int dt_int = t - a.last_post[a.ch_node[ch]];
+    if (dt_int > 0 && dt_int < kMaxSynDt)
+      a.w[k] = syn_rule(w, -a.dt * (float)dt_int, a.grp_param + (size_t)a.st.grp[k] * kSynGroupParams);
+  }
+}
+
+// the step's spiking nodes: one block each over its incoming plastic
+// synapses; then the node's spike time.  Block 0 empties the slot's queue
+// for the step that reuses it.
+__global__ void stdp_post_kernel(StdpArgs a) {
+  const int t = (int)a.ctx.t_idx;
+  const int lst = a.ctx.lst;
+  int n_spk = a.sl.count[lst];
+  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
+  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
+  for (int s = blockIdx.x; s < n_spk; s += gridDim.x) {
+    const int node = (int)((unsigned int)srcs[s] & ~kSpikeLagBit);
+    for (long long r = a.rev_ptr[node] + threadIdx.x; r < a.rev_ptr[node + 1]; r += blockDim.x) {
+      const unsigned int k = a.rev_idx[r];
+      const int dt_int = t - a.last_pre[k];
+      if (dt_int >= 0 && dt_int < kMaxSynDt)
+        a.w[k] = syn_rule(a.w[k], a.dt * (float)dt_int, a.grp_param + (size_t)a.st.grp[k] * kSynGroupParams);
+    }
+    if (threadIdx.x == 0) a.last_post[node] = t;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st.q_n[(int)(a.ctx.t_idx % a.st.n_slot)] = 0;
+}
+
+void launch_stdp_step(const StdpArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(stdp_arrive_kernel, dim3(256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(stdp_post_kernel, dim3(1024), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

@@ -447,30 +447,34 @@ int NESTGPU_GetNArrayVar(int i_node) {
 }
 
 int* NESTGPU_GetSeqSeqConnections(int i_source, int n_source, int i_target, int n_target,
-                                  int, int* n_conn) {
+                                  int syn_group, int* n_conn) {
   return guarded((int*)nullptr, [&] {
-    return conn_array(E().GetConnections(nullptr, i_source, n_source, nullptr, i_target, n_target),
+    return conn_array(E().GetConnections(nullptr, i_source, n_source, nullptr, i_target, n_target,
+                                         syn_group),
                       n_conn);
   });
 }
 int* NESTGPU_GetSeqGroupConnections(int i_source, int n_source, int* i_target, int n_target,
-                                    int, int* n_conn) {
+                                    int syn_group, int* n_conn) {
   return guarded((int*)nullptr, [&] {
-    return conn_array(E().GetConnections(nullptr, i_source, n_source, i_target, 0, n_target),
+    return conn_array(E().GetConnections(nullptr, i_source, n_source, i_target, 0, n_target,
+                                         syn_group),
                       n_conn);
   });
 }
 int* NESTGPU_GetGroupSeqConnections(int* i_source, int n_source, int i_target, int n_target,
-                                    int, int* n_conn) {
+                                    int syn_group, int* n_conn) {
   return guarded((int*)nullptr, [&] {
-    return conn_array(E().GetConnections(i_source, 0, n_source, nullptr, i_target, n_target),
+    return conn_array(E().GetConnections(i_source, 0, n_source, nullptr, i_target, n_target,
+                                         syn_group),
                       n_conn);
   });
 }
 int* NESTGPU_GetGroupGroupConnections(int* i_source, int n_source, int* i_target, int n_target,
-                                      int, int* n_conn) {
+                                      int syn_group, int* n_conn) {
   return guarded((int*)nullptr, [&] {
-    return conn_array(E().GetConnections(i_source, 0, n_source, i_target, 0, n_target), n_conn);
+    return conn_array(E().GetConnections(i_source, 0, n_source, i_target, 0, n_target, syn_group),
+                      n_conn);
   });
 }
 int NESTGPU_GetConnectionStatus(int i_source, int i_group, int i_conn, int* i_target,
@@ -482,26 +486,29 @@ int NESTGPU_GetConnectionStatus(int i_source, int i_group, int i_conn, int* i_ta
   });
 }
 
-int NESTGPU_CreateSynGroup(char*) {
-  return guarded(0, [&]() -> int { out_of_scope("Synapse groups (STDP)"); });
+int NESTGPU_CreateSynGroup(char* model_name) {
+  return guarded(0, [&] { return E().CreateSynGroup(model_name); });
 }
-int NESTGPU_GetSynGroupNParam(int) {
-  return guarded(0, [&]() -> int { out_of_scope("Synapse groups (STDP)"); });
+int NESTGPU_GetSynGroupNParam(int i_syn_group) {
+  return guarded(0, [&] { return E().GetSynGroupNParam(i_syn_group); });
 }
-char** NESTGPU_GetSynGroupParamNames(int) {
-  return guarded((char**)nullptr, [&]() -> char** { out_of_scope("Synapse groups (STDP)"); });
+char** NESTGPU_GetSynGroupParamNames(int i_syn_group) {
+  return guarded((char**)nullptr, [&] { return name_array(E().GetSynGroupParamNames(i_syn_group)); });
 }
-int NESTGPU_IsSynGroupParam(int, char*) {
-  return guarded(0, [&]() -> int { out_of_scope("Synapse groups (STDP)"); });
+int NESTGPU_IsSynGroupParam(int i_syn_group, char* param_name) {
+  return guarded(0, [&] { return E().IsSynGroupParam(i_syn_group, param_name) ? 1 : 0; });
 }
-int NESTGPU_GetSynGroupParamIdx(int, char*) {
-  return guarded(0, [&]() -> int { out_of_scope("Synapse groups (STDP)"); });
+int NESTGPU_GetSynGroupParamIdx(int i_syn_group, char* param_name) {
+  return guarded(0, [&] { return E().GetSynGroupParamIdx(i_syn_group, param_name); });
 }
-float NESTGPU_GetSynGroupParam(int, char*) {
-  return guarded(0.0f, [&]() -> float { out_of_scope("Synapse groups (STDP)"); });
+float NESTGPU_GetSynGroupParam(int i_syn_group, char* param_name) {
+  return guarded(0.0f, [&] { return E().GetSynGroupParam(i_syn_group, param_name); });
 }
-int NESTGPU_SetSynGroupParam(int, char*, float) {
-  return guarded(0, [&]() -> int { out_of_scope("Synapse groups (STDP)"); });
+int NESTGPU_SetSynGroupParam(int i_syn_group, char* param_name, float val) {
+  return guarded(0, [&] {
+    E().SetSynGroupParam(i_syn_group, param_name, val);
+    return 0;
+  });
 }
 
 int NESTGPU_ActivateSpikeCount(int i_node, int n_node) {

@@ -157,8 +157,8 @@ def test_kernel_status_and_errors(ngpu):
     assert ngpu.GetKernelStatus("rnd_seed") == 1234
     with pytest.raises(ValueError, match="MPI is not available"):
         ngpu.MpiId()
-    with pytest.raises(ValueError, match="outside the scope"):
-        ngpu.CreateSynGroup("test_syn_model")
+    with pytest.raises(ValueError, match="Unknown synapse model name"):
+        ngpu.CreateSynGroup("no_such_synapse_model")
     n = ngpu.Create("aeif_cond_beta_multisynapse", 2, 1)
     with pytest.raises(ValueError, match="Delay must be >= time resolution"):
         ngpu.Connect(n, n, {"rule": "all_to_all"}, {"weight": 1.0, "delay": 0.05})
@@ -279,3 +279,30 @@ def test_connect_argument_errors(ngpu):
     w = [ngpu.GetStatus(c, "weight") for c in ngpu.GetConnections(a, a)]
     assert len(w) == 16 and all(0.5 <= x <= 1.5 for x in w)
     assert ngpu.RuleArraySize({"rule": "fixed_indegree", "indegree": 3}, a, a) == 12
+
+
+def test_syn_group_parameters(ngpu):
+    # synapse groups, src/syn_model.cu:136-238: names and defaults of "stdp"
+    # (src/stdp.cu:64-79) and "test_syn_model" (src/test_syn_model.cu:43-52)
+    g = ngpu.CreateSynGroup("stdp")
+    assert ngpu.GetSynGroupParamNames(g) == ["tau_plus", "tau_minus", "lambda", "alpha",
+                                              "mu_plus", "mu_minus", "Wmax"]
+    st = ngpu.GetStatus(g)
+    assert st == pytest.approx({"tau_plus": 20.0, "tau_minus": 20.0, "lambda": 1.0e-4,
+                                "alpha": 1.0, "mu_plus": 1.0, "mu_minus": 1.0, "Wmax": 100.0})
+    ngpu.SetStatus(g, {"Wmax": 3.0, "alpha": 0.5})
+    assert ngpu.GetSynGroupParam(g, "Wmax") == pytest.approx(3.0)
+    assert ngpu.GetSynGroupParam(g, "alpha") == pytest.approx(0.5)
+    t = ngpu.CreateSynGroup("test_syn_model", {"fact": 0.25})
+    assert t.i_syn_group == g.i_syn_group + 1
+    assert ngpu.GetSynGroupParamNames(t) == ["fact", "offset"]
+    assert ngpu.GetSynGroupParam(t, "fact") == pytest.approx(0.25)
+    with pytest.raises(ValueError, match="Unrecognized synapse parameter"):
+        ngpu.GetSynGroupParam(t, "Wmax")
+    # connections may name a group; an unknown group is an error
+    n = ngpu.Create("aeif_cond_beta_multisynapse", 4, 1)
+    ngpu.Connect(n, n, {"rule": "all_to_all"}, {"weight": 1.0, "delay": 1.0, "synapse_group": g})
+    assert len(ngpu.GetConnections(n, n, syn_group=g.i_syn_group)) == 16
+    assert len(ngpu.GetConnections(n, n, syn_group=0)) == 0
+    with pytest.raises(ValueError, match="Unrecognized synapse group"):
+        ngpu.Connect(n, n, {"rule": "one_to_one"}, {"weight": 1.0, "delay": 1.0, "synapse_group": 9})
