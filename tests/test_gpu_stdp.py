@@ -110,3 +110,36 @@ def test_stdp_asymmetric_parameters(ngpu):
     ngpu.Simulate(1000.0)
     w = np.array(ngpu.GetStatus(ngpu.GetConnections(pre, post), "weight"), dtype=np.float64)
     _check_weights(ngpu, w, w0, dts, pre, post, p)
+
+
+def test_syn_model_step_order(ngpu):
+    # python/test/test_syn_model.py: the linear rule w += offset + fact * Dt
+    # (src/test_syn_model.cu:36-41) on zero-weight synapses whose spikes
+    # arrive 2, 1 steps before, in the same step as, and 1, 2 steps after
+    # the target's spike: depression at arrival only for a target that
+    # fired in an earlier step, potentiation at the target's spike also for
+    # an arrival in the same step (Dt = 0), so w = offset + fact * dt
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    fact, offset, n, time_diff = 0.2, 0.03, 5, 10.0
+    g = ngpu.CreateSynGroup("test_syn_model")
+    ngpu.SetSynGroupParam(g, "fact", fact)
+    ngpu.SetSynGroupParam(g, "offset", offset)
+    sg = ngpu.Create("spike_generator", n)
+    neuron = ngpu.Create("aeif_cond_beta_multisynapse", 2 * n)
+    ngpu.SetStatus(neuron, {"t_ref": 10.0})
+    pre, post = neuron[0:n], neuron[n:2 * n]
+    dts = [0.1 * (-0.5 * (n - 1) + i) for i in range(n)]
+    ngpu.SetStatus(sg, {"spike_times": [50.0], "spike_heights": [1.0]})
+    ngpu.Connect(sg, pre, {"rule": "one_to_one"},
+                 {"weight": 17.9, "delay": 1.0, "receptor": 0, "synapse_group": 0})
+    ngpu.Connect(sg, post, {"rule": "one_to_one"},
+                 {"weight": 17.9, "delay": 1.0 + time_diff, "receptor": 0, "synapse_group": 0})
+    for i in range(n):
+        ngpu.Connect([pre[i]], [post[i]], {"rule": "one_to_one"},
+                     {"weight": 0.0, "delay": time_diff - dts[i], "receptor": 0, "synapse_group": g})
+    ngpu.Simulate(200.0)
+    w = np.array(ngpu.GetStatus(ngpu.GetConnections(pre, post), "weight"), dtype=np.float64)
+    expect = np.array([dt * fact + offset for dt in dts])
+    print("test_syn_model weights", w, "expected", expect)
+    np.testing.assert_allclose(w, expect, rtol=0, atol=1e-6)
