@@ -515,30 +515,39 @@ void launch_deliver_tiled(const TiledDeliverArgs& a, hipStream_t s) {
 // step's, which the next launch folds as its lanes take their neurons); the
 // rest of each lagged spike, and every other spike, is delivered with the
 // list by the next launch.  A wave takes kD0Batch list entries: lanes
-// 0..kD0Batch-1 load them and the segment bounds of the lagged ones, then
-// every lane loads one record of each (the d_int = 0 group heads the segment
-// and has far fewer than 64 records) before any atomic is issued, so a wave
-// waits for memory three times in all.
+// 0..kD0Batch-1 load them together with the list's count, then the segment
+// bounds of the lagged ones, then every lane loads one record of each (the
+// d_int = 0 group heads the segment and has far fewer than 64 records) before
+// any atomic is issued, so a wave waits for memory three times in all.
 constexpr int kLagD0Blocks = 64;
 constexpr int kD0Batch = 8;
 
 __global__ void __launch_bounds__(256) deliver_lagged_d0_kernel(DeliverArgs a) {
   const int lst = a.ctx.lst;
-  int n_spk = a.sl.count[lst];
-  if (n_spk > a.sl.cap) n_spk = a.sl.cap;
-  const int* srcs = a.sl.src + (size_t)lst * a.sl.cap;
-  const float* hts = a.sl.height + (size_t)lst * a.sl.cap;
+  const int cap = a.sl.cap;
+  const int* srcs = a.sl.src + (size_t)lst * cap;
+  const float* hts = a.sl.height + (size_t)lst * cap;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = blockIdx.x * (blockDim.x / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int n_waves = gridDim.x * (blockDim.x / kWave);
   const SynTable& t = a.st;
   unsigned long long n_ev = 0;
-  for (int e0 = wave * kD0Batch; e0 < n_spk; e0 += n_waves * kD0Batch) {
+  for (int e0 = wave * kD0Batch; e0 < cap; e0 += n_waves * kD0Batch) {
+    // entries are read below the list's capacity whatever its count (the
+    // count arrives with them; entries at or beyond it are masked after)
     const int e = e0 + lane;
-    const bool mine = lane < kD0Batch && e < n_spk;
-    const unsigned int raw = mine ? (unsigned int)srcs[e] : 0u;
-    const float hgt = mine ? hts[e] : 0.0f;
-    const bool lg = mine && (raw >> 31);
+    const bool inb = lane < kD0Batch && e < cap;
+    unsigned int raw = 0u;
+    float hgt = 0.0f;
+    if (inb) {
+      raw = (unsigned int)srcs[e];
+      hgt = hts[e];
+    }
+    int n_spk = a.sl.count[lst];
+    asm volatile("" ::"v"(raw), "v"(hgt));
+    if (n_spk > cap) n_spk = cap;
+    if (e0 >= n_spk) break;
+    const bool lg = inb && e < n_spk && (raw >> 31);
     long long sb = 0, se = 0;
     if (lg) {
       const int src = (int)(raw & ~kSpikeLagBit);

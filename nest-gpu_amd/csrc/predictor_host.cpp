@@ -38,9 +38,11 @@ void Predictor::ensure_params() {
   m_.alloc(kPNTrain);
   v_.alloc(kPNTrain);
   grad_.alloc(kPNTrain);
-  PCHK(hipMemset(params_.p, 0, kPNPar * sizeof(float)));
-  PCHK(hipMemset(m_.p, 0, kPNTrain * sizeof(float)));
-  PCHK(hipMemset(v_.p, 0, kPNTrain * sizeof(float)));
+  ensure_stream();
+  PCHK(hipMemsetAsync(params_.p, 0, kPNPar * sizeof(float), stream_));
+  PCHK(hipMemsetAsync(m_.p, 0, kPNTrain * sizeof(float), stream_));
+  PCHK(hipMemsetAsync(v_.p, 0, kPNTrain * sizeof(float), stream_));
+  PCHK(hipStreamSynchronize(stream_));
 }
 
 void Predictor::SetMode(int mode) {
@@ -55,7 +57,8 @@ void Predictor::StartLog(long long capacity, int stride) {
   ensure_stream();
   log_.alloc((size_t)capacity * kLogCols);
   log_count_.alloc(1);
-  PCHK(hipMemset(log_count_.p, 0, sizeof(unsigned long long)));
+  PCHK(hipMemsetAsync(log_count_.p, 0, sizeof(unsigned long long), stream_));
+  PCHK(hipStreamSynchronize(stream_));
   log_cap_ = capacity;
   log_stride_ = stride;
 }
@@ -90,8 +93,9 @@ void Predictor::SetParams(const float* p, int n) {
   ensure_stream();
   ensure_params();
   PCHK(hipMemcpy(params_.p, p, kPNPar * sizeof(float), hipMemcpyHostToDevice));
-  PCHK(hipMemset(m_.p, 0, kPNTrain * sizeof(float)));
-  PCHK(hipMemset(v_.p, 0, kPNTrain * sizeof(float)));
+  PCHK(hipMemsetAsync(m_.p, 0, kPNTrain * sizeof(float), stream_));
+  PCHK(hipMemsetAsync(v_.p, 0, kPNTrain * sizeof(float), stream_));
+  PCHK(hipStreamSynchronize(stream_));
   adam_t_ = 0;
   have_params_ = true;
 }
