@@ -157,10 +157,16 @@ def cpu_baseline(n_sample, sim_ms, seed):
             "syn_events_per_s": ev}
 
 
+# sampled kernel timing (modes 3 and 4): one step in TIMING_STRIDE is
+# bracketed by events
+TIMING_STRIDE = 16
+
+
 def timed_window(ngpu, steps, timing=2):
     """Simulate `steps` steps between two synchronisations; returns wall time,
-    kernel times (HIP events on the simulator stream: timing 2 brackets only
-    the RK5 launches, 1 every launch) and statistics deltas."""
+    kernel times (HIP events on the simulator stream: timing 1 brackets every
+    launch, 2 only the RK5 launches, 3 / 4 the same in one step of every
+    TIMING_STRIDE, scaled up here to the whole window) and statistics deltas."""
     st0 = ngpu.GetStepStats()
     ngpu.SetKernelTiming(timing)
     ngpu.Synchronize()
@@ -169,6 +175,8 @@ def timed_window(ngpu, steps, timing=2):
     ngpu.Synchronize()
     wall = time.perf_counter() - t0
     kt = ngpu.GetKernelTimes()
+    if timing in (3, 4):
+        kt = {k: (ms * TIMING_STRIDE, cnt * TIMING_STRIDE) for k, (ms, cnt) in kt.items()}
     st1 = ngpu.GetStepStats()
     ngpu.SetKernelTiming(False)
     d = {k: st1[k] - st0[k] for k in ("attempts", "rejected", "substeps", "syn_events", "spikes",
@@ -316,8 +324,10 @@ def main():
              n * 1000)
     log("built %s: %d neurons, %d synapses + %d Poisson edges in %.1f s"
         % (args.workload, n, n_syn, n, time.perf_counter() - t0))
-    # no RK5 in the microcircuit: its timed windows bracket every launch
-    win_timing = 1 if mc else 2
+    # events around a launch add a gap to its step: the timed windows bracket
+    # the launches of one step in TIMING_STRIDE only (the microcircuit, which
+    # has no RK5, every launch of those steps; the others their RK5 launch)
+    win_timing = 3 if mc else 4
     if predictor is not None:
         ngpu.ActivateRecSpikeTimes(neurons, 400)
     t0 = time.perf_counter()
@@ -367,19 +377,19 @@ def main():
         L = LEARN_STEPS
         ngpu.Simulate(200 * DT)     # untimed gap
         t_pa = t_now + 200 * DT
-        wa = timed_window(ngpu, L)
+        wa = timed_window(ngpu, L, timing=win_timing)
         sa = window_summary(wa, L, n)
         ngpu.SetPredictorParams(np.asarray(predictor["params"], np.float32))
         ngpu.SetStepPredictorMode("learned")
         ngpu.Simulate(200 * DT)     # switch-over, untimed
         t_learn0 = t_pa + (L + 200) * DT
-        wl = timed_window(ngpu, L)
+        wl = timed_window(ngpu, L, timing=win_timing)
         ngpu.SetStepPredictorMode("plain")
         sl = window_summary(wl, L, n)
         log("learned: wall %.3f s, RTF %.4f, kernel ms %s" % (wl["wall"], sl["rtf"], wl["kt"]))
         ngpu.Simulate(200 * DT)
         t_plain2 = t_learn0 + (L + 200) * DT
-        wp2 = timed_window(ngpu, L)
+        wp2 = timed_window(ngpu, L, timing=win_timing)
         sp2 = window_summary(wp2, L, n)
         log("plain before/after: RTF %.4f / %.4f" % (sa["rtf"], sp2["rtf"]))
         t0 = time.perf_counter()
@@ -415,8 +425,9 @@ def main():
     tiled = batch["config"] if batch and batch["config"].get("tiled") else None
     roof_all = rooflines(wb, n, n_port, nb, n_poisson_csr_edges, tiled)
     breakdown = {k: v[0] * 1e3 / nb for k, v in wb["kt"].items()}
-    # the dominant kernel's roofline from the plain timed window (its RK5
-    # launches were bracketed by events there); HBM traffic per launch from
+    # the dominant kernel's roofline from the plain timed window (its
+    # sampled launches), else from the breakdown window; HBM traffic per
+    # launch from
     # the committed FETCH_SIZE/WRITE_SIZE summary of this workload, if any
     dom = max(wb["kt"], key=lambda k: wb["kt"][k][0])
     roof = dict(rooflines(wp, n, n_port, args.steps, n_poisson_csr_edges, tiled).get(dom, roof_all.get(dom, {})))

@@ -218,7 +218,9 @@ int NESTGPU_RemoteCreate(int i_host, char *model_name, int n_neuron, int n_port)
 int NESTGPU_AMD_GetStepStats(unsigned long long *out);
 /* Kernel timing with HIP events on the simulator's stream: on = 1 times
  * every launch, on = 2 only the RK5 launches (two events per step instead
- * of two per launch), 0 stops; a call zeroes the accumulation.
+ * of two per launch), on = 3 / 4 the same in one step of every 16 (step
+ * index within the Simulate call = 7 mod 16; the events lengthen the steps
+ * they bracket), 0 stops; a call zeroes the accumulation.
  * ms[5]/launches[5] receive the summed device
  * time and launch count of: RK5 neuron update, iaf_psc_exp update, spike
  * delivery, fold + Poisson input, other (generators, records, step end). */

@@ -2689,7 +2689,8 @@ SynTable Engine::syn_table() const {
 
 void Engine::timed(int cat, const std::function<void()>& launch, hipStream_t st) {
   if (!st) st = stream_;
-  if (!timing_ || (timing_rk5_only_ && cat != KT_RK5)) {
+  if (!timing_ || (timing_rk5_only_ && cat != KT_RK5) ||
+      (timing_stride_ > 1 && it_ % timing_stride_ != kTimingPhase)) {
     launch();
     HIPCHK(hipGetLastError());   // a launch that cannot run fails the step loudly
     return;
@@ -2729,7 +2730,8 @@ void Engine::flush_timing() {
 void Engine::SetKernelTiming(int mode) {
   if (timing_) flush_timing();
   timing_ = mode != 0;
-  timing_rk5_only_ = mode == 2;
+  timing_rk5_only_ = mode == 2 || mode == 4;
+  timing_stride_ = mode >= 3 ? kTimingStride : 1;
   for (int k = 0; k < KT_N; ++k) { kt_ms_[k] = 0.0; kt_n_[k] = 0; }
 }
 

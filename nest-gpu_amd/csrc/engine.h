@@ -249,7 +249,12 @@ class Engine {
   // Kernel timing with HIP events on the simulator stream, per category:
   // 0 RK5 update, 1 iaf_psc_exp update, 2 delivery, 3 fold+Poisson, 4 other.
   enum { KT_RK5 = 0, KT_IAF, KT_DELIVER, KT_FOLD, KT_OTHER, KT_N };
-  void SetKernelTiming(int mode);   // 0 off, 1 every launch, 2 RK5 launches only
+  // 0 off, 1 every launch, 2 RK5 launches only; 3 and 4 the same in the
+  // steps it with it % kTimingStride == kTimingPhase only (events around a
+  // launch lengthen its step)
+  void SetKernelTiming(int mode);
+  static constexpr int kTimingStride = 16;
+  static constexpr int kTimingPhase = 7;
   // Batch kernel (all populations in one launch per step, replayed as a
   // one-node HIP graph): configuration in effect, a new one (takes effect
   // with the next step; results do not depend on it), and the autotuner,
@@ -410,6 +415,7 @@ class Engine {
   hipStream_t stream_ = nullptr;
   bool timing_ = false;
   bool timing_rk5_only_ = false;
+  int timing_stride_ = 1;
   std::vector<hipEvent_t> ev_free_;
   struct EvPair { hipEvent_t a, b; int cat; };
   std::vector<EvPair> ev_pending_;

@@ -613,7 +613,7 @@ int NESTGPU_AMD_GetStepStats(unsigned long long* out) {
 }
 int NESTGPU_AMD_SetKernelTiming(int on) {
   return guarded(0, [&] {
-    E().SetKernelTiming(on == 2 ? 2 : (on != 0 ? 1 : 0));
+    E().SetKernelTiming(on >= 2 && on <= 4 ? on : (on != 0 ? 1 : 0));
     return 0;
   });
 }
