@@ -773,3 +773,25 @@ def test_parrot_neuron_matches_oracle(ngpu, hold):
     np.testing.assert_array_equal(vg, vo)
     np.testing.assert_array_equal(hg, ho)
     assert np.count_nonzero(hg) == 1
+
+
+def test_sampled_kernel_timing(ngpu):
+    # timing modes 3 / 4 bracket the launches of the steps whose index in the
+    # Simulate call is 7 mod 16 only (modes 1 / 2: every step); 2 and 4 the
+    # RK5 launches alone
+    ngpu.ResetKernel()
+    ngpu.SetVerbosityLevel(0)
+    n = ngpu.Create("aeif_cond_beta_multisynapse", 100, 2)
+    ngpu.SetStatus(n, {"I_e": 600.0})
+    ngpu.Simulate(1.0)
+    steps = 64
+    sampled = len([i for i in range(steps) if i % 16 == 7])
+    for mode, expect in ((1, steps), (2, steps), (3, sampled), (4, sampled)):
+        ngpu.SetKernelTiming(mode)
+        ngpu.Simulate(steps * 0.1)
+        kt = ngpu.GetKernelTimes()
+        ngpu.SetKernelTiming(False)
+        assert kt["rk5_update"][1] == expect, (mode, kt)
+        assert kt["rk5_update"][0] > 0.0
+        if mode in (2, 4):
+            assert all(kt[k][1] == 0 for k in kt if k != "rk5_update"), (mode, kt)
