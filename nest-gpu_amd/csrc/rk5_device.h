@@ -520,10 +520,17 @@ __device__ __attribute__((always_inline)) inline void rk5_persistent_body(
         x = cx1;
         err = 0.0f;
       }
+      // the step factor of either branch: with a predictor (whose kernel
+      // already holds more registers) evaluated once for the wave; the plain
+      // kernel keeps one power per branch (the merged form costs it a
+      // register spill in the refill)
+      float pw = 0.0f;
+      if constexpr (MODE != 0)
+        pw = pow_step(err, err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5));
       if (!(err <= 1.0f || (double)h <= (double)h_min * (1.0 + 1.0e-5))) {
         ++n_rej;   // rejected: shrink h, retry with the same k1 and scale
         if constexpr (MODE == 1) rej_sub = true;
-        const float h_new = h * ck::safety * pow_m025(err);
+        const float h_new = h * ck::safety * (MODE == 0 ? pow_m025(err) : pw);
         h = (float)fmax((double)h_new, 0.1 * (double)h);
       } else {
         ++n_sub;   // accepted (src/rk5.h:135-159), then ExternalUpdate
@@ -547,7 +554,7 @@ __device__ __attribute__((always_inline)) inline void rk5_persistent_body(
         }
         if constexpr (MODE != 0) el = err;
         if (att_i <= kMaxAttempts) x += h;
-        h = (err > ck::err_min) ? h * ck::safety * pow_m02(err) : 5.0f * h;
+        h = (err > ck::err_min) ? h * ck::safety * (MODE == 0 ? pow_m02(err) : pw) : 5.0f * h;
 #pragma unroll
         for (int j = 0; j < M::NV; ++j) y[j] = yn[j];
         need_k1 = true;

@@ -83,6 +83,44 @@ NGPU_HD inline float pow_m02(float x) {
 // RN(1/d) for a divisor usable by div_rn (normal, |d| in [2^-32, 2^32]),
 // else 0.  1/d rounded to double and then to float is RN(1/d): the exact
 // reciprocal of a float is never within 2^-49 (relative) of a float midpoint.
+// err^-1/5 (five) or err^-1/4 (not five), lane by lane the operations of
+// pow_m02 / pow_m025 with the constants selected: one evaluation serves a
+// wave whose lanes took both the accept and the reject branch.
+NGPU_HD inline float pow_step(float x, bool five) {
+  if (!(x == x)) return x;
+  if (x == INFINITY) return 0.0f;
+  if (x == 0.0f) return INFINITY;
+  int e;
+  const double m = split_binade(x, &e);
+  int q4 = e / 4, r4 = e % 4;
+  if (r4 < 0) { r4 += 4; q4 -= 1; }
+  int q5 = e / 5, r5 = e % 5;
+  if (r5 < 0) { r5 += 5; q5 -= 1; }
+  const int q = five ? q5 : q4, r = five ? r5 : r4;
+  // both branches' constants stay literals (selected results, not selected
+  // operands), so the evaluation holds no more registers than one power
+  const double z5 = 1.2284 + m * (-0.28173 + m * 0.05166);
+  const double z4i = 1.2852 + m * (-0.35326 + m * 0.06590);
+  double z = five ? z5 : z4i;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double z2 = z * z;
+    const double z4 = z2 * z2;
+    const double mz = m * (five ? z4 * z : z4);
+    z = z * (five ? (6.0 - mz) * 0.2 : (5.0 - mz) * 0.25);
+  }
+  const double c = five ? (r == 0   ? 1.0
+                           : r == 1 ? 0.8705505632961241
+                           : r == 2 ? 0.757858283255199
+                           : r == 3 ? 0.6597539553864471
+                                    : 0.5743491774985174)
+                        : (r == 0   ? 1.0
+                           : r == 1 ? 0.8408964152537145
+                           : r == 2 ? 0.7071067811865476
+                                    : 0.5946035575013605);
+  return (float)ldexp(z * c, -q);
+}
+
 NGPU_HD inline float recip_rn(float d) {
   const float a = fabsf(d);
   if (!(a >= 0x1p-32f && a <= 0x1p32f)) return 0.0f;

@@ -52,10 +52,15 @@ int main() {
     ++nr;
     if (f_bits(ngpu::pow_m025(x)) != f_bits(w4)) ++bad_r;
     if (f_bits(ngpu::pow_m02(x)) != f_bits(w5)) ++bad_r;
+    // the one-evaluation form used by the persistent kernel
+    if (f_bits(ngpu::pow_step(x, false)) != f_bits(w4)) ++bad_r;
+    if (f_bits(ngpu::pow_step(x, true)) != f_bits(w5)) ++bad_r;
   }
   if (!(ngpu::pow_m025(0.0f) == INFINITY && ngpu::pow_m02(0.0f) == INFINITY &&
         ngpu::pow_m025(INFINITY) == 0.0f && ngpu::pow_m02(INFINITY) == 0.0f &&
-        std::isnan(ngpu::pow_m02(NAN))))
+        std::isnan(ngpu::pow_m02(NAN)) && ngpu::pow_step(0.0f, true) == INFINITY &&
+        ngpu::pow_step(0.0f, false) == INFINITY && ngpu::pow_step(INFINITY, true) == 0.0f &&
+        ngpu::pow_step(INFINITY, false) == 0.0f && std::isnan(ngpu::pow_step(NAN, true))))
     ++bad_r;
   std::printf("pow_m025/pow_m02: %lld inputs, %lld mismatches\n", nr, bad_r);
   return (bad || bad_r) ? 1 : 0;
