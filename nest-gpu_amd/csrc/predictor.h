@@ -3,7 +3,8 @@
 // smaller of its proposal and the RK5 controller's (src/rk5.h:135-153).  The
 // embedded error estimate still accepts or rejects every attempt, and a
 // rejected attempt falls back to the controller's shrink rule, so the plain
-// integrator stays the judge.  No counterpart exists in the reference.
+// integrator's error test decides every step.  No counterpart exists in the
+// reference.
 //
 // Parameter vector (floats; the C ABI uses the same layout):
 //   mu[8], inv_sigma[8]          input normalisation (from the training log)
