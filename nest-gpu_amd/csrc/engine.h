@@ -360,7 +360,7 @@ class Engine {
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
   int rk5_defer_cap_ = 24;             // attempts after which a step is split over two launches (0: never)
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
-  float rk5_hot_margin_ = 3.5f;        // hot list from V_th + margin * Delta_T (AdEx models)
+  float rk5_hot_margin_ = 4.0f;        // hot list from V_th + margin * Delta_T (AdEx models)
   bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
   bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
   int rk5_hot_lanes_ = 64;             // hot entries per wave
