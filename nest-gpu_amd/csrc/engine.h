@@ -358,12 +358,12 @@ class Engine {
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
-  int rk5_defer_cap_ = 24;             // attempts after which a step is split over two launches (0: never)
+  int rk5_defer_cap_ = 23;             // attempts after which a step is split over two launches (0: never)
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
   float rk5_hot_margin_ = 4.0f;        // hot list from V_th + margin * Delta_T (AdEx models)
   bool rk5_hot_only_ = true;           // waves holding hot entries take no bulk share
   bool poisson_runs_ = true;           // uniform poisson connections as runs, not CSR
-  int rk5_hot_lanes_ = 64;             // hot entries per wave
+  int rk5_hot_lanes_ = 32;             // hot entries per wave
   bool fold_in_rk5_ = true;            // split-step RK5 populations fold inside the next RK5 launch
   bool fused_deliver_env_ = true;      // allow the RK5 launch to deliver the previous step's spikes
   bool fused_deliver_ = false;         // ... and the network qualifies (one emitting RK5 population)
