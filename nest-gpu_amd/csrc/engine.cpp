@@ -1746,7 +1746,9 @@ int Engine::Calibrate() {
     int dev = 0, cus = 256;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    rk5_max_blocks_ = cus * 4;
+    int per_cu = 4;
+    if (const char* env = getenv("NGPU_RK5_BLOCKS_PER_CU")) per_cu = std::min(4, std::max(1, atoi(env)));
+    rk5_max_blocks_ = cus * per_cu;
   }
   if (const char* env = getenv("NGPU_RK5_PERSISTENT")) rk5_persistent_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_RK5_HOT_MARGIN")) rk5_hot_margin_ = (float)atof(env);
