@@ -1758,6 +1758,7 @@ int Engine::Calibrate() {
   if (const char* env = getenv("NGPU_FUSED_DELIVER")) fused_deliver_env_ = atoi(env) != 0;
   if (const char* env = getenv("NGPU_LAG_D0")) lag_d0_mode_ = atoi(env) == 1 ? 1 : 0;
   if (const char* env = getenv("NGPU_RK5_HOT_LANES")) rk5_hot_lanes_ = std::min(64, std::max(1, atoi(env)));
+  if (const char* env = getenv("NGPU_RK5_LEARNED_CAP")) rk5_learned_cap_ = std::max(0, atoi(env));
   if (const char* env = getenv("NGPU_RK5_DEFER")) {
     rk5_defer_cap_ = std::max(0, atoi(env));
     rk5_defer_env_ = true;
@@ -2029,7 +2030,7 @@ RK5Args Engine::rk5_args(Group& gr, const StepCtx& sc, const SpikeList& sl, cons
     // (state is complete between calls)
     // learned proposals shorten the spike upswing's attempt chain, so
     // the split point that balances two launches comes earlier
-    const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? 14 : rk5_defer_cap_);
+    const int cap = rk5_defer_env_ ? rk5_defer_cap_ : (a.pred_mode == 2 ? rk5_learned_cap_ : rk5_defer_cap_);
     a.defer_cap = (defer_ok_ && !g->recorded) ? cap : 0;
     if (g->fold_pending) {
       a.prefold = 1;

@@ -358,6 +358,7 @@ class Engine {
   DBuf<long long> d_last_idx_;
   DBuf<float> d_last_h_;
   bool rk5_persistent_ = true;         // persistent lane-refill RK5 kernel (else lane per neuron)
+  int rk5_learned_cap_ = 14;           // the same with learned step-size proposals
   int rk5_defer_cap_ = 23;             // attempts after which a step is split over two launches (0: never)
   bool rk5_defer_env_ = false;         // cap set by NGPU_RK5_DEFER (also for learned proposals)
   float rk5_hot_margin_ = 4.0f;        // hot list from V_th + margin * Delta_T (AdEx models)
